@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Benchmark: HLLC fluid step on the 3D Sedov-Taylor blast, FP64, one MI355X.
+
+Workload (the north-star configuration that fits one GPU): 256^3 cells on the
+unit cube, rho = 1, v = 0, P = 1e-5, E = 1 deposited in the 123 cells within
+r <= 3 dx of the centre, gamma = 5/3, constant-extrapolation boundaries,
+HLLC with 2nd-order MUSCL (van Leer limiter), SSP-RK2, CFL 0.4 with the
+halved HLLC time step rule. One "step" is one Imogen iteration: the CFL
+reduction, then two `fluidstep` calls (forward and backward sweep order),
+i.e. 2 cell-updates per cell and 6 directional sweeps.
+
+Prints one JSON line. `value` is cell-updates/s with the state resident in
+HBM. `roofline` is the dominant sweep kernel: 80 algorithmic bytes per cell
+per directional sweep (5 FP64 conserved variables in, 5 out) divided by its
+average launch duration, measured with HIP events on the library's stream
+during the timed region. `cpu_baseline` is the FP64 CPU oracle (a restatement
+of the reference's arithmetic; the reference has no CPU path) timed on the
+host's cores on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL_SWEEP = 80
+METRIC = "HLLC 3D cell-updates/s + % HBM peak; gas-dust speedup learned vs exact drag"
+LIMITERS = {"ospre": 0, "vanleer": 1}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_gpu(args, q0, h):
+    from gpuimogen_amd import abi, gpuclass as gc
+
+    n = q0.shape[-1]
+    gamma = 5.0 / 3.0
+    lim = LIMITERS[args.limiter]
+    f = gc.FluidManager(q0, gamma, minmass=1e-8, bc_modes=(abi.extrapConstant,) * 6)
+    gm = gc.GeometryManager((n, n, n), h)
+
+    state = {"it": 0}
+
+    def iteration():
+        tau = gc.cflTimestepFused(f, gm, abi.METHOD_HLLC)
+        dt = args.cfl * tau
+        it = state["it"]
+        gc.fluidstep([f], None, None, None, [dt, 1, 1, it, abi.METHOD_HLLC], gm, limiter=lim)
+        gc.fluidstep([f], None, None, None, [dt, 1, -1, it, abi.METHOD_HLLC], gm, limiter=lim)
+        state["it"] = it + 1
+        return dt
+
+    for _ in range(args.warmup):
+        iteration()
+    gc.synchronize()
+    abi.call("imogen_profile_enable", 1)
+    gc.synchronize()
+    t0 = time.perf_counter()
+    simtime = 0.0
+    for k in range(args.steps):
+        simtime += 2 * iteration()
+        if (k + 1) % max(1, args.steps // 4) == 0:
+            log(f"[bench] step {k + 1}/{args.steps}")
+    gc.synchronize()
+    elapsed = time.perf_counter() - t0
+    kern = {}
+    names = {0: "hllc_sweep_x", 1: "hllc_sweep_y", 2: "hllc_sweep_z", 3: "boundary", 4: "cfl"}
+    for kid, name in names.items():
+        cnt = ctypes.c_int64(0)
+        tot = ctypes.c_double(0.0)
+        abi.call("imogen_profile_read", kid, ctypes.byref(cnt), ctypes.byref(tot))
+        kern[name] = {"launches": int(cnt.value), "total_ms": tot.value,
+                      "avg_ms": tot.value / cnt.value if cnt.value else None}
+    abi.call("imogen_profile_enable", 0)
+    final = f.state if args.check else None
+    f.free()
+    return elapsed, kern, simtime, final
+
+
+def run_cpu_baseline(args, q0, h):
+    from oracle import pyoracle as po
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = max(1, min(threads, os.cpu_count() or 1))
+    po.set_threads(threads)
+    q = q0.copy()
+    n = q.shape[-1]
+    gamma = 5.0 / 3.0
+    lim = po.LIM_VANLEER if args.limiter == "vanleer" else po.LIM_OSPRE
+    t0 = time.perf_counter()
+    for it in range(args.cpu_iters):
+        cs = po.soundspeed(q, gamma)
+        tau = po.cfl_timestep(q, cs, h, po.HLLC, (n, n, n))
+        dt = args.cfl * tau
+        for direction in (1, -1):
+            po.fluidstep(q, h, dt, direction, it, gamma, po.HLLC, lim, modes=(po.BC_CONST,) * 6)
+    el = time.perf_counter() - t0
+    cu = q.shape[1] * q.shape[2] * q.shape[3] * 2 * args.cpu_iters
+    return {"value": cu / el, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{args.cpu_iters} iterations (CFL + 2 fluidsteps) of the same {n}^3 Sedov HLLC "
+                      f"workload from the same initial state, FP64 oracle, {threads} OpenMP threads, "
+                      f"{el:.2f} s"}
+
+
+def load_traffic():
+    p = os.path.join(ROOT, "profiles", "traffic_hllc_sweep_sedov256.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as fh:
+        return json.load(fh)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--cfl", type=float, default=0.4)
+    ap.add_argument("--limiter", choices=sorted(LIMITERS), default="vanleer")
+    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--check", action="store_true", help="verify the final state is finite and positive")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build has no "
+              "multi-GPU path: see DESIGN.md for how the fluid step would shard.", file=sys.stderr)
+        sys.exit(2)
+
+    from gpuimogen_amd import problems
+
+    q0, h, ndep = problems.sedov3d(args.n)
+    log(f"[bench] Sedov {args.n}^3, {ndep} deposit cells, {args.steps} timed iterations")
+    elapsed, kern, simtime, final = run_gpu(args, q0, h)
+    cells = args.n ** 3
+    cu = cells * 2 * args.steps
+    value = cu / elapsed
+
+    sweeps = {k: v for k, v in kern.items() if k.startswith("hllc_sweep") and v["avg_ms"]}
+    dom = max(sweeps, key=lambda k: sweeps[k]["total_ms"])
+    bytes_per_launch = BYTES_PER_CELL_SWEEP * cells
+    achieved = bytes_per_launch / (sweeps[dom]["avg_ms"] * 1e-3) / 1e9
+    for k, v in sweeps.items():
+        v["achieved_GBs"] = bytes_per_launch / (v["avg_ms"] * 1e-3) / 1e9
+    traffic = load_traffic()
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS,
+                "traffic": traffic.get(dom) if isinstance(traffic, dict) else None,
+                "algorithmic_bytes_per_launch": bytes_per_launch}
+    if args.check:
+        ok = bool(np.all(np.isfinite(final)) and np.all(final[0] > 0))
+        log(f"[bench] final state finite/positive: {ok}")
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"[bench] CPU oracle baseline: {args.cpu_iters} iterations")
+        cpu = run_cpu_baseline(args, q0, h)
+
+    line = {"metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"sedov3d_hllc_{args.n}cubed", "cells": cells, "method": "HLLC",
+                       "limiter": args.limiter, "cfl": args.cfl, "time_scheme": "SSP-RK2",
+                       "boundaries": "extrapConstant", "iteration": "CFL + 2 fluidsteps (6 sweeps)",
+                       "sim_time_advanced": simtime},
+            "roofline": roofline, "cpu_baseline": cpu,
+            "kernels": kern, "drag_speedup_learned_vs_exact": None}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,134 @@
+// cfl.hip -- sound speed and the CFL time-step reduction.
+//
+// Per-cell formulas follow cukern_Soundspeed_hd (gpuclass/cudaSoundspeed.cu:178-195)
+// and cukern_CFLtimestep (gpuclass/cflTimestep.cu:262-377, SQUARE geometry).
+// The reduction is this library's: each lane keeps a running minimum over a
+// grid-stride loop, the wave reduces with 64-lane xor shuffles, waves combine
+// through LDS, and a one-block second pass reduces the per-block minima, so
+// the result is deterministic and no host round trip per block is needed.
+// Comparisons use `u < m ? u : m` exactly as the reference does, so NaN cells
+// are skipped as they are there.
+#include "imogen_internal.h"
+
+namespace imogen {
+
+__device__ __forceinline__ double cs_of(double rho, double E, double px, double py, double pz, double gg1)
+{
+    double rhoinv = 1 / rho;
+    double csq = gg1 * (E - .5 * (px * px + py * py + pz * pz) * rhoinv) * rhoinv;
+    if (csq < 0.0) csq = 0.0;
+    return sqrt(csq);
+}
+
+__global__ void soundspeed_kernel(const double *__restrict__ q, int64_t pitch, int64_t n, double gg1,
+                                  double *__restrict__ cs)
+{
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x)
+        cs[x] = cs_of(q[x], q[pitch + x], q[2 * pitch + x], q[3 * pitch + x], q[4 * pitch + x], gg1);
+}
+
+__device__ __forceinline__ double wave_min(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        double o = __shfl_xor(v, off, 64);
+        v = (o < v) ? o : v;
+    }
+    return v;
+}
+
+// dim = simulation dimension (1..3), xinjin selects the per-direction form.
+// fused != 0: sound speed computed in-line from gamma instead of read from cs.
+__global__ __launch_bounds__(256) void cfl_pass1(const double *__restrict__ q, int64_t pitch, int64_t n,
+                                                 const double *__restrict__ cs, double gg1, int fused, int dim,
+                                                 int xinjin, double hx, double hy, double hz, double hmin,
+                                                 double *__restrict__ blockMin)
+{
+    double m = 1e37;
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
+        const double rho = q[x];
+        const double px = q[2 * pitch + x];
+        const double py = q[3 * pitch + x];
+        const double pz = q[4 * pitch + x];
+        const double c = fused ? cs_of(rho, q[pitch + x], px, py, pz, gg1) : cs[x];
+        double u;
+        if (!xinjin) {
+            double pm;
+            if (dim == 1) {
+                pm = fabs(px);
+            } else {
+                double p = px * px;
+                p = p + py * py;
+                if (dim > 2) p = p + pz * pz;
+                pm = sqrt(p);
+            }
+            u = pm / rho + c;
+            u = hmin / u;
+        } else {
+            u = hx / (fabs(px) / rho + c);
+            double v = hy / (fabs(py) / rho + c);
+            u = (u < v) ? u : v;
+            if (dim > 2) {
+                v = hz / (fabs(pz) / rho + c);
+                u = (u < v) ? u : v;
+            }
+        }
+        m = (u < m) ? u : m;
+    }
+    m = wave_min(m);
+    __shared__ double part[4];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) part[w] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = part[0];
+        for (int k = 1; k < (int)(blockDim.x >> 6); k++) r = (part[k] < r) ? part[k] : r;
+        blockMin[blockIdx.x] = r;
+    }
+}
+
+__global__ __launch_bounds__(256) void cfl_pass2(const double *__restrict__ blockMin, int nblocks,
+                                                 double halve, double *__restrict__ out)
+{
+    double m = 1e38;
+    for (int k = threadIdx.x; k < nblocks; k += blockDim.x) m = (blockMin[k] < m) ? blockMin[k] : m;
+    m = wave_min(m);
+    __shared__ double part[4];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) part[w] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = part[0];
+        for (int k = 1; k < (int)(blockDim.x >> 6); k++) r = (part[k] < r) ? part[k] : r;
+        out[0] = halve ? r / 2 : r;  // cflTimestep.cu:232-237
+    }
+}
+
+int launch_soundspeed(const double *base, int64_t pitch, int64_t n, double gamma, double *cs, hipStream_t s)
+{
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(soundspeed_kernel, dim3((unsigned)blocks), dim3(256), 0, s, base, pitch, n,
+                       gamma * (gamma - 1), cs);
+    return IM_LAUNCHED("soundspeed_kernel");
+}
+
+// devOut[0] receives tau; devOut[1..] is used as block scratch (<= 2048 blocks).
+int launch_cfl(const double *base, int64_t pitch, int64_t n, const double *cs, double gamma, int fused,
+               int dim, int xinjin, int halve, double hx, double hy, double hz, double hmin, double *devOut,
+               hipStream_t s)
+{
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    if (blocks < 1) blocks = 1;
+    double *blockMin = devOut + 8;
+    hipLaunchKernelGGL(cfl_pass1, dim3((unsigned)blocks), dim3(256), 0, s, base, pitch, n, cs,
+                       gamma * (gamma - 1), fused, dim, xinjin, hx, hy, hz, hmin, blockMin);
+    int rc = IM_LAUNCHED("cfl_pass1");
+    if (rc) return rc;
+    hipLaunchKernelGGL(cfl_pass2, dim3(1), dim3(256), 0, s, blockMin, (int)blocks, halve ? 1.0 : 0.0, devOut);
+    return IM_LAUNCHED("cfl_pass2");
+}
+
+}  // namespace imogen

@@ -1,0 +1,397 @@
+// fluid_hllc.hip -- fused HLLC SSP-RK2 directional sweeps for gfx950.
+//
+// One kernel launch performs both SSP-RK2 stages of one directional sweep
+// (the reference runs them as two launches through a 5-slab temporary plus a
+// physical transpose, gpuclass/cudaFluidStep.cu:485-627 and
+// gpuclass/flux.cu:92-116). The fluid is updated in place in the reference's
+// 5-slab layout (rho, E, px, py, pz; x fastest), with the reference's periodic
+// wrap along the swept line (cudaFluidStep.cu:1159-1162).
+//
+// Two data-movement schemes, both free of transposes:
+//  * X (contiguous) lines: a wave owns whole lines. Each lane holds K
+//    consecutive cells in registers; the one-cell neighbours it needs from the
+//    adjacent lanes (primitives, left face, flux) come by cross-lane shuffle.
+//    No LDS, no barrier, no halo re-read.
+//  * Y/Z (strided) lines: each lane marches along one column (coalesced over
+//    x across the wave) keeping a register window of the two-stage stencil.
+//    Columns may be split into S segments held by lanes of the same wave, for
+//    occupancy; a segment reads its 4-cell halos directly, and keeps its first
+//    4 outputs in a scratch buffer until the whole wave has finished reading,
+//    so the in-place update (and the periodic wrap) is race-free.
+#include "hllc_device.h"
+#include "imogen_internal.h"
+
+namespace imogen {
+
+// ---------------------------------------------------------------------------
+// cross-lane helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double shfl_d(double v, int src)
+{
+    return __shfl(v, src, 64);
+}
+
+__device__ __forceinline__ V5 shfl5(const V5 &v, int src)
+{
+    return V5{shfl_d(v.a, src), shfl_d(v.b, src), shfl_d(v.c, src), shfl_d(v.d, src),
+              shfl_d(v.e, src)};
+}
+
+__device__ __forceinline__ V5 sel5(bool c, const V5 &x, const V5 &y)
+{
+    return c ? x : y;
+}
+
+__device__ __forceinline__ V5 axpy_update(const V5 &q, double lam, const V5 &fr, const V5 &fl)
+{
+    // Qout = Qin - lambda*(F_{i+1/2} - F_{i-1/2})      (cudaFluidStep.cu:1510-1516)
+    return V5{q.a - lam * (fr.a - fl.a), q.b - lam * (fr.b - fl.b), q.c - lam * (fr.c - fl.c),
+              q.d - lam * (fr.d - fl.d), q.e - lam * (fr.e - fl.e)};
+}
+
+__device__ __forceinline__ V5 avg_update(const V5 &s, double lam, const V5 &fr, const V5 &fl)
+{
+    // Qout = .5*((Qin + Qout) - lambda*(F_{i+1/2} - F_{i-1/2}))   (cudaFluidStep.cu:1555-1561)
+    // s already holds (Qin + Qout).
+    return V5{.5 * (s.a - lam * (fr.a - fl.a)), .5 * (s.b - lam * (fr.b - fl.b)),
+              .5 * (s.c - lam * (fr.c - fl.c)), .5 * (s.d - lam * (fr.d - fl.d)),
+              .5 * (s.e - lam * (fr.e - fl.e))};
+}
+
+__device__ __forceinline__ V5 add5(const V5 &x, const V5 &y)
+{
+    return V5{x.a + y.a, x.b + y.b, x.c + y.c, x.d + y.d, x.e + y.e};
+}
+
+// Slab index holding (normal, tangential1, tangential2) momentum for a sweep
+// direction, in the order the reference kernels read them (:1199-1216).
+template <int DIR>
+struct MomSlabs;
+template <>
+struct MomSlabs<1> {
+    static constexpr int n = 2, t1 = 3, t2 = 4;
+};
+template <>
+struct MomSlabs<2> {
+    static constexpr int n = 3, t1 = 2, t2 = 4;
+};
+template <>
+struct MomSlabs<3> {
+    static constexpr int n = 4, t1 = 3, t2 = 2;
+};
+
+template <int DIR>
+__device__ __forceinline__ V5 load_cell(const double *__restrict__ p, int64_t pitch)
+{
+    using M = MomSlabs<DIR>;
+    return V5{p[0], p[pitch], p[M::n * pitch], p[M::t1 * pitch], p[M::t2 * pitch]};
+}
+
+template <int DIR>
+__device__ __forceinline__ void store_cell(double *__restrict__ p, int64_t pitch, const V5 &v)
+{
+    using M = MomSlabs<DIR>;
+    p[0] = v.a;
+    p[pitch] = v.b;
+    p[M::n * pitch] = v.c;
+    p[M::t1 * pitch] = v.d;
+    p[M::t2 * pitch] = v.e;
+}
+
+// ---------------------------------------------------------------------------
+// X sweep: lanes own K consecutive cells of a line
+// ---------------------------------------------------------------------------
+
+// Interface fluxes of the cells a lane owns. F[k] is the flux at k+1/2 for
+// k < cnt; Fm is the flux at -1/2 (the left lane's last interface).
+template <int K, int LIM>
+__device__ __forceinline__ void x_fluxes(const HydroConsts &hc, int rt, const V5 (&U)[K], int cnt,
+                                         int srcL, int srcR, V5 (&F)[K], V5 &Fm)
+{
+    V5 w[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) w[k] = to_prim(U[k]);
+
+    V5 wlast = w[0];
+#pragma unroll
+    for (int k = 1; k < K; k++) wlast = sel5(k == cnt - 1, w[k], wlast);
+    const V5 wl = shfl5(wlast, srcL);
+    const V5 wr = shfl5(w[0], srcR);
+
+    // backward differences d[k] = w[k] - w[k-1]; d[cnt] uses the right neighbour
+    V5 d[K + 1];
+    d[0] = w[0] - wl;
+#pragma unroll
+    for (int k = 1; k < K; k++) d[k] = sel5(k < cnt, w[k], wr) - w[k - 1];
+    d[K] = wr - w[K - 1];
+
+    V5 Lf[K], Rf[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) faces<LIM>(hc, w[k], d[k], d[k + 1], rt, Lf[k], Rf[k]);
+
+    const V5 LfR = shfl5(Lf[0], srcR);
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const V5 right = (k + 1 < K) ? sel5(k + 1 < cnt, Lf[(k + 1 < K) ? k + 1 : 0], LfR) : LfR;
+        F[k] = hllc_flux(hc, Rf[k], right);
+    }
+    V5 Flast = F[0];
+#pragma unroll
+    for (int k = 1; k < K; k++) Flast = sel5(k == cnt - 1, F[k], Flast);
+    Fm = shfl5(Flast, srcL);
+}
+
+template <int K, int LIM>
+__global__ __launch_bounds__(256) void hllc_sweep_x(double *__restrict__ q, int64_t pitch, int N,
+                                                    int64_t nlines, int nl, double lambda,
+                                                    HydroConsts hc, int rt)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int lpw = 64 / nl;
+    const int sub = lane / nl;
+    const int j = lane - sub * nl;
+    const int64_t line = wave * lpw + sub;
+    const bool active = (sub < lpw) && (line < nlines);
+    const int lbase = sub * nl;
+    const int srcL = (sub < lpw) ? lbase + (j == 0 ? nl - 1 : j - 1) : lane;
+    const int srcR = (sub < lpw) ? lbase + (j == nl - 1 ? 0 : j + 1) : lane;
+    const int x0 = j * K;
+    int cnt = N - x0;
+    cnt = cnt < K ? cnt : K;
+    if (!active) cnt = 0;
+
+    double *base = q + (active ? line * (int64_t)N + x0 : 0);
+
+    V5 Q[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        if (k < cnt) Q[k] = load_cell<1>(base + k, pitch);
+        else Q[k] = V5{1.0, 1.0, 0.0, 0.0, 0.0};
+    }
+
+    // stage A: U1 = Q - lambda dF(Q)
+    V5 F[K], Fm;
+    x_fluxes<K, LIM>(hc, rt, Q, cnt, srcL, srcR, F, Fm);
+    V5 U1[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) U1[k] = axpy_update(Q[k], lambda, F[k], k == 0 ? Fm : F[k > 0 ? k - 1 : 0]);
+
+    // stage B: Q = .5((U1 + Q) - lambda dF(U1))
+    x_fluxes<K, LIM>(hc, rt, U1, cnt, srcL, srcR, F, Fm);
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        if (k < cnt) {
+            const V5 o = avg_update(add5(U1[k], Q[k]), lambda, F[k], k == 0 ? Fm : F[k > 0 ? k - 1 : 0]);
+            store_cell<1>(base + k, pitch, o);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Y/Z sweep: register-window march along strided columns
+// ---------------------------------------------------------------------------
+template <int DIR, int LIM>
+__global__ __launch_bounds__(256) void hllc_sweep_march(double *__restrict__ q, int64_t pitch, int nx,
+                                                        int64_t sstride, int64_t ostride, int N,
+                                                        int S, int seglen, int wavesPerRow,
+                                                        double lambda, HydroConsts hc, int rt,
+                                                        double *__restrict__ defer)
+{
+    const int lane = threadIdx.x & 63;
+    const int wib = threadIdx.x >> 6;
+    const int W = 64 / S;
+    const int s = lane / W;
+    const int c = lane - s * W;
+    const int64_t wrow = (int64_t)blockIdx.x * (blockDim.x >> 6) + wib;  // wave index within this row
+    if (wrow >= wavesPerRow) return;
+    const int64_t x = wrow * W + c;
+    const bool colValid = (x < nx) && (s < S);
+    const int j0 = s * seglen;
+    int L = N - j0;
+    L = L < seglen ? L : seglen;
+    if (!colValid) L = 0;
+    const int steps = L + 8;  // loads j0-4 .. j0+L+3
+
+    double *col = q + (colValid ? x + (int64_t)blockIdx.y * ostride : 0);
+    const int64_t gtid = ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t nthr = (int64_t)gridDim.x * gridDim.y * blockDim.x;
+
+    // wrapped sweep index of the first load (m = j0 - 4)
+    int mw = j0 - 4;
+    if (mw < 0) mw += N;
+    if (mw < 0) mw += N;
+
+    const V5 one = V5{1.0, 1.0, 0.0, 0.0, 0.0};
+    const V5 zero = V5{0.0, 0.0, 0.0, 0.0, 0.0};
+    V5 Qm2 = one, Qm1 = one;
+    V5 wprev = one, dprev = zero, Rprev = one, FAprev = zero;
+    V5 Sm3 = one, Sm4 = one;
+    V5 w1prev = one, d1prev = zero, R1prev = one, FBprev = zero;
+
+    const int maxsteps = seglen + 8;
+    V5 Qnext = one;
+    if (steps > 0) Qnext = load_cell<DIR>(col + (int64_t)mw * sstride, pitch);
+
+    for (int t = 0; t < maxsteps; t++) {
+        const V5 Qcur = Qnext;
+        mw = (mw + 1 == N) ? 0 : mw + 1;
+        if (t + 1 < steps) Qnext = load_cell<DIR>(col + (int64_t)mw * sstride, pitch);
+
+        // stage A at cell m-2
+        const V5 w = to_prim(Qcur);
+        const V5 d = w - wprev;
+        V5 Lc, Rc;
+        faces<LIM>(hc, wprev, dprev, d, rt, Lc, Rc);  // faces of cell m-1
+        const V5 FA = hllc_flux(hc, Rprev, Lc);        // interface (m-2)+1/2
+        const V5 U1 = axpy_update(Qm2, lambda, FA, FAprev);
+        const V5 S2 = add5(U1, Qm2);
+
+        // stage B at cell m-4
+        const V5 w1 = to_prim(U1);
+        const V5 d1 = w1 - w1prev;
+        V5 L1c, R1c;
+        faces<LIM>(hc, w1prev, d1prev, d1, rt, L1c, R1c);  // faces of cell m-3
+        const V5 FB = hllc_flux(hc, R1prev, L1c);           // interface (m-4)+1/2
+        const V5 o = avg_update(Sm4, lambda, FB, FBprev);   // cell m-4
+
+        const int oi = t - 8;  // output index within the segment
+        if (oi >= 0 && oi < L) {
+            if (oi < 4) {
+                double *dp = defer + gtid;
+                dp[(oi * 5 + 0) * nthr] = o.a;
+                dp[(oi * 5 + 1) * nthr] = o.b;
+                dp[(oi * 5 + 2) * nthr] = o.c;
+                dp[(oi * 5 + 3) * nthr] = o.d;
+                dp[(oi * 5 + 4) * nthr] = o.e;
+            } else {
+                store_cell<DIR>(col + (int64_t)(j0 + oi) * sstride, pitch, o);
+            }
+        }
+
+        Qm2 = Qm1;
+        Qm1 = Qcur;
+        wprev = w;
+        dprev = d;
+        Rprev = Rc;
+        FAprev = FA;
+        Sm4 = Sm3;
+        Sm3 = S2;
+        w1prev = w1;
+        d1prev = d1;
+        R1prev = R1c;
+        FBprev = FB;
+    }
+
+    // every lane of the wave has finished reading: release the deferred outputs
+    const int nd = L < 4 ? L : 4;
+    for (int oi = 0; oi < nd; oi++) {
+        const double *dp = defer + gtid;
+        V5 o{dp[(oi * 5 + 0) * nthr], dp[(oi * 5 + 1) * nthr], dp[(oi * 5 + 2) * nthr],
+             dp[(oi * 5 + 3) * nthr], dp[(oi * 5 + 4) * nthr]};
+        store_cell<DIR>(col + (int64_t)(j0 + oi) * sstride, pitch, o);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+template <int LIM>
+static int launch_x(double *q, int64_t pitch, int N, int64_t nlines, double lambda, const HydroConsts &hc,
+                    int rt, hipStream_t s)
+{
+    int K;
+    if (N <= 64) K = 1;
+    else if (N <= 128) K = 2;
+    else if (N <= 192) K = 3;
+    else if (N <= 256) K = 4;
+    else if (N <= 384) K = 6;
+    else if (N <= 512) K = 8;
+    else if (N <= 768) K = 12;
+    else if (N <= 1024) K = 16;
+    else return IM_FAIL(ERROR_NOIMPLEMENT, "X sweep supports lines of at most 1024 cells (got %d)", N);
+    const int nl = (N + K - 1) / K;
+    const int lpw = 64 / nl;
+    const int64_t waves = (nlines + lpw - 1) / lpw;
+    const int64_t blocks = (waves + 3) / 4;
+    dim3 grid((unsigned)blocks), block(256);
+#define IM_XK(KK)                                                                                 \
+    case KK:                                                                                      \
+        hipLaunchKernelGGL((hllc_sweep_x<KK, LIM>), grid, block, 0, s, q, pitch, N, nlines, nl,   \
+                           lambda, hc, rt);                                                       \
+        break;
+    switch (K) {
+        IM_XK(1)
+        IM_XK(2)
+        IM_XK(3)
+        IM_XK(4)
+        IM_XK(6)
+        IM_XK(8)
+        IM_XK(12)
+        IM_XK(16)
+    }
+#undef IM_XK
+    return IM_LAUNCHED("hllc_sweep_x");
+}
+
+template <int LIM>
+static int launch_march(double *q, int64_t pitch, int nx, int ny, int nz, int dir, double lambda,
+                        const HydroConsts &hc, int rt, hipStream_t s)
+{
+    const int N = (dir == 2) ? ny : nz;
+    const int64_t sstride = (dir == 2) ? (int64_t)nx : (int64_t)nx * ny;
+    const int64_t ostride = (dir == 2) ? (int64_t)nx * ny : (int64_t)nx;
+    const int nother = (dir == 2) ? nz : ny;
+    const int64_t columns = (int64_t)nx * nother;
+    // Segments per column: aim for >= 2 waves per SIMD (256 CUs x 4 SIMDs).
+    int S = 1;
+    while (S < 4 && columns * S < 2048 * 64 && N >= 32 * S) S *= 2;
+    const int seglen = (N + S - 1) / S;
+    const int W = 64 / S;
+    const int wavesPerRow = (nx + W - 1) / W;
+    const int blocksPerRow = (wavesPerRow + 3) / 4;
+    dim3 grid((unsigned)blocksPerRow, (unsigned)nother), block(256);
+    const int64_t nthr = (int64_t)blocksPerRow * nother * 256;
+    double *defer = (double *)scratch((size_t)nthr * 20 * sizeof(double));
+    if (!defer) return IM_FAIL(ERROR_NOMEM, "no scratch for deferred outputs");
+    if (dir == 2)
+        hipLaunchKernelGGL((hllc_sweep_march<2, LIM>), grid, block, 0, s, q, pitch, nx, sstride, ostride,
+                           N, S, seglen, wavesPerRow, lambda, hc, rt, defer);
+    else
+        hipLaunchKernelGGL((hllc_sweep_march<3, LIM>), grid, block, 0, s, q, pitch, nx, sstride, ostride,
+                           N, S, seglen, wavesPerRow, lambda, hc, rt, defer);
+    return IM_LAUNCHED("hllc_sweep_march");
+}
+
+template <int LIM>
+static int launch_dir(double *q, int64_t pitch, int nx, int ny, int nz, int dir, double lambda,
+                      const HydroConsts &hc, int rt, hipStream_t s)
+{
+    const int slot = prof_begin(PROF_SWEEP_X + dir - 1, s);
+    int rc;
+    if (dir == 1) rc = launch_x<LIM>(q, pitch, nx, (int64_t)ny * nz, lambda, hc, rt, s);
+    else rc = launch_march<LIM>(q, pitch, nx, ny, nz, dir, lambda, hc, rt, s);
+    prof_end(slot, s);
+    return rc;
+}
+
+int launch_hllc_sweep(double *base, int64_t pitch, int nx, int ny, int nz, int dir, double lambda,
+                      double gamma, int limiter, hipStream_t s)
+{
+    if (dir < 1 || dir > 3) return IM_FAIL(ERROR_INVALID_ARGS, "sweep direction %d", dir);
+    const int N = (dir == 1) ? nx : (dir == 2 ? ny : nz);
+    if (N < 3) return SUCCESSFUL;  // cudaFluidStep.cu:301
+    HydroConsts hc{gamma, gamma - 1.0, gamma / (gamma - 1.0)};
+    switch (limiter) {
+    case LIMITER_OSPRE: return launch_dir<0>(base, pitch, nx, ny, nz, dir, lambda, hc, 0, s);
+    case LIMITER_VANLEER: return launch_dir<1>(base, pitch, nx, ny, nz, dir, lambda, hc, 1, s);
+    case LIMITER_MINMOD:
+    case LIMITER_SUPERBEE:
+    case LIMITER_VANALBADA:
+    case LIMITER_MC:
+    case LIMITER_ZERO: return launch_dir<-1>(base, pitch, nx, ny, nz, dir, lambda, hc, limiter, s);
+    default: return IM_FAIL(ERROR_INVALID_ARGS, "unknown slope limiter %d", limiter);
+    }
+}
+
+}  // namespace imogen

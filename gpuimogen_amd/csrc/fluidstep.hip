@@ -1,0 +1,235 @@
+// fluidstep.hip -- host side of the fluid step: the reference's compiled-core
+// entry points (performFluidUpdate_3D/_1D, setFluidBoundary, calculateSoundspeed,
+// computeLocalCFLTimestep) over the kernels of this library.
+//
+// The sweep order and per-direction skip rules follow gpuclass/flux.cu:27-155.
+// No array is transposed: each direction's kernel walks its own stride, so the
+// permutation bookkeeping of the reference (cudaArrayRotateB.cu:551-640)
+// always ends where it started (identity), which is what the caller observes.
+#include <cstring>
+
+#include "imogen_internal.h"
+
+using namespace imogen;
+
+namespace {
+
+// fluxcall table of gpuclass/flux.cu:46: direction swept at position n of sweep s
+const int kFluxcall[3][6] = {{1, 2, 1, 2, 3, 3}, {3, 1, 2, 3, 1, 2}, {2, 3, 3, 1, 2, 1}};
+
+// Validates a 5-array fluid canister (slab views of one 5-slab owner) and
+// returns its base pointer and slab pitch in doubles.
+int fluid_view(MGArray *fluid, double **base, int64_t *pitch)
+{
+    if (!fluid) return IM_FAIL(ERROR_NULL_POINTER, "null fluid");
+    if (fluid->nGPUs != 1)
+        return IM_FAIL(ERROR_NOIMPLEMENT, "only single-GPU fluids are supported (nGPUs = %d)", fluid->nGPUs);
+    if (fluid->permtag != 1 && fluid->permtag != 0)
+        return IM_FAIL(ERROR_NOIMPLEMENT, "fluid arrays must be in XYZ memory order (permtag %d)", fluid->permtag);
+    const int64_t p = fluid->slabPitch[0] / (int64_t)sizeof(double);
+    for (int i = 1; i < 5; i++) {
+        if (fluid[i].devicePtr[0] != fluid[0].devicePtr[0] + i * p)
+            return IM_FAIL(ERROR_INVALID_ARGS, "fluid array %d is not slab %d of the canister", i, i);
+    }
+    if (!fluid->devicePtr[0]) return IM_FAIL(ERROR_NULL_POINTER, "null device pointer");
+    *base = fluid->devicePtr[0];
+    *pitch = p;
+    return SUCCESSFUL;
+}
+
+int check_topology(const ParallelTopology *t)
+{
+    if (!t) return SUCCESSFUL;
+    for (int i = 0; i < 3; i++)
+        if (t->nproc[i] > 1)
+            return IM_FAIL(ERROR_NOIMPLEMENT, "multi-rank topologies are not built (nproc[%d] = %d)", i, t->nproc[i]);
+    return SUCCESSFUL;
+}
+
+// sas code of one array for one face (cudaStatics.cu:226-250)
+int sas_code(const MGArray *a, int direction, int side, int *code)
+{
+    const BCModeTypes m = a->boundaryConditions.mode[2 * (direction - 1) + side];
+    switch (m) {
+    case circular:
+    case stationary: *code = -1; return SUCCESSFUL;
+    case mirror: *code = (a->vectorComponent == direction) ? 1 : 0; return SUCCESSFUL;
+    case extrapConstant: *code = 2; return SUCCESSFUL;
+    case extrapLinear: *code = 3; return SUCCESSFUL;
+    case wall: return IM_FAIL(ERROR_INVALID_ARGS, "Wall BC is not implemented (as in the reference)");
+    default: return IM_FAIL(ERROR_NOIMPLEMENT, "boundary mode %d is not built", (int)m);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int setFluidBoundary(MGArray *fluid, GeometryParams *geo, int direction)
+{
+    (void)geo;
+    double *base;
+    int64_t pitch;
+    int rc = fluid_view(fluid, &base, &pitch);
+    if (rc) return rc;
+    if (direction < 1 || direction > 3)
+        return IM_FAIL(ERROR_INVALID_ARGS, "direction %d is not one of 1 (X) 2 (Y) 3 (Z)", direction);
+    int modes10[10];
+    for (int side = 0; side < 2; side++) {
+        const BCModeTypes bt = fluid[0].boundaryConditions.mode[2 * (direction - 1) + side];
+        if (bt == outflow || bt == freebalance)
+            return IM_FAIL(ERROR_NOIMPLEMENT, "outflow/freebalance boundaries are not built");
+        for (int i = 0; i < 5; i++) {
+            rc = sas_code(&fluid[i], direction, side, &modes10[2 * i + side]);
+            if (rc) return rc;
+        }
+    }
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    const int slot = prof_begin(PROF_BC, st->stream);
+    rc = launch_fluid_bc(base, pitch, fluid->dim[0], fluid->dim[1], fluid->dim[2], direction, modes10,
+                         fluid->haloSize > 0 ? fluid->haloSize : 4, st->stream);
+    prof_end(slot, st->stream);
+    return rc;
+}
+
+int setArrayBoundaryConditions(MGArray *phi, GeometryParams *geo, int direction, int side)
+{
+    (void)geo;
+    if (!phi) return IM_FAIL(ERROR_NULL_POINTER, "null array");
+    if (phi->nGPUs != 1) return IM_FAIL(ERROR_NOIMPLEMENT, "single-GPU arrays only");
+    if (direction == 0) return SUCCESSFUL;
+    if (direction < 1 || direction > 3 || side < 0 || side > 1)
+        return IM_FAIL(ERROR_INVALID_ARGS, "direction %d side %d", direction, side);
+    int code;
+    int rc = sas_code(phi, direction, side, &code);
+    if (rc) return rc;
+    int modes10[10];
+    for (int i = 0; i < 10; i++) modes10[i] = -1;
+    modes10[side] = code;  // slab 0 of a one-slab launch at phi's own pointer
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    // A single array is addressed as slab 0; the other 4 table entries are inert.
+    return launch_fluid_bc(phi->devicePtr[0], 0, phi->dim[0], phi->dim[1], phi->dim[2], direction, modes10,
+                           phi->haloSize > 0 ? phi->haloSize : 4, st->stream);
+}
+
+int performFluidUpdate_1D(MGArray *fluid, FluidStepParams params, ParallelTopology *topo, MGArray *tmpst)
+{
+    (void)tmpst;
+    double *base;
+    int64_t pitch;
+    int rc = fluid_view(fluid, &base, &pitch);
+    if (rc) return rc;
+    rc = check_topology(topo);
+    if (rc) return rc;
+    if (params.geometry.shape != SQUARE)
+        return IM_FAIL(ERROR_NOIMPLEMENT, "only SQUARE (cartesian) geometry is built");
+    if (params.onlyHydro != 1) return IM_FAIL(ERROR_NOIMPLEMENT, "MHD is not built (dead in the reference)");
+    const int dir = params.stepDirection;
+    if (dir < 1 || dir > 3) return IM_FAIL(ERROR_INVALID_ARGS, "stepDirection %d", dir);
+    const double lambda = params.dt / params.geometry.h[dir - 1];
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    switch (params.stepMethod) {
+    case METHOD_HLLC:
+        return launch_hllc_sweep(base, pitch, fluid->dim[0], fluid->dim[1], fluid->dim[2], dir, lambda,
+                                 params.thermoGamma, params.slopeLimiter, st->stream);
+    case METHOD_HLL:
+    case METHOD_XINJIN:
+        return IM_FAIL(ERROR_NOIMPLEMENT, "fluid method %d is not built yet", (int)params.stepMethod);
+    default: return IM_FAIL(ERROR_INVALID_ARGS, "unknown fluid method %d", (int)params.stepMethod);
+    }
+}
+
+int performFluidUpdate_3D(MGArray *fluid, ParallelTopology *parallelTopo, FluidStepParams fsp,
+                          MGArray *tempStorage)
+{
+    int rc;
+    FluidStepParams p = fsp;
+    if (fluid->dim[1] == 1 && fluid->dim[2] == 1) {  // flux.cu:57-66
+        p.stepDirection = 1;
+        rc = performFluidUpdate_1D(fluid, p, parallelTopo, tempStorage);
+        if (rc) return rc;
+        return setFluidBoundary(fluid, &fsp.geometry, 1);
+    }
+    int sweep = (fsp.stepNumber + 3 * (fsp.stepDirection > 0)) % 6;
+    if (sweep < 0) sweep += 6;
+    for (int n = 0; n < 3; n++) {
+        const int d = kFluxcall[n][sweep];
+        if (fluid->dim[d - 1] > 3) {  // flux.cu:102
+            p.stepDirection = d;
+            rc = performFluidUpdate_1D(fluid, p, parallelTopo, tempStorage);
+            if (rc) return rc;
+            rc = setFluidBoundary(fluid, &fsp.geometry, d);
+            if (rc) return rc;
+        }
+    }
+    return SUCCESSFUL;
+}
+
+int calculateSoundspeed(MGArray *fluid, MGArray *mag, MGArray *cs, double gamma)
+{
+    if (mag) return IM_FAIL(ERROR_NOIMPLEMENT, "MHD sound speed is not built");
+    double *base;
+    int64_t pitch;
+    int rc = fluid_view(fluid, &base, &pitch);
+    if (rc) return rc;
+    if (!cs || cs->nGPUs != 1 || cs->numel != fluid->numel)
+        return IM_FAIL(ERROR_INVALID_ARGS, "sound speed array does not match the fluid");
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    return launch_soundspeed(base, pitch, fluid->partNumel[0], gamma, cs->devicePtr[0], st->stream);
+}
+
+static int cfl_common(MGArray *fluid, MGArray *csound, GeometryParams *geom, int method, double gamma,
+                      double *globalResolution, double *tstep)
+{
+    double *base;
+    int64_t pitch;
+    int rc = fluid_view(fluid, &base, &pitch);
+    if (rc) return rc;
+    if (!geom || !globalResolution || !tstep) return IM_FAIL(ERROR_NULL_POINTER, "null argument");
+    if (geom->shape != SQUARE) return IM_FAIL(ERROR_NOIMPLEMENT, "only SQUARE geometry is built");
+    if (method < 1 || method > 3) return IM_FAIL(ERROR_INVALID_ARGS, "method %d", method);
+    int spacedim = 0;
+    if (globalResolution[1] > 1) spacedim = 1;
+    if (globalResolution[2] > 1) spacedim = 2;
+    const int dim = spacedim + 1;
+    double hmin = geom->h[0];
+    if (dim > 1 && geom->h[1] < hmin) hmin = geom->h[1];
+    if (dim > 2 && geom->h[2] < hmin) hmin = geom->h[2];
+    const int xj = (method == METHOD_XINJIN) && dim > 1;  // the XJ-1D table entry is the HLLC form
+    const int halve = (method == METHOD_HLL || method == METHOD_HLLC);
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    const int slot = prof_begin(PROF_CFL, st->stream);
+    rc = launch_cfl(base, pitch, fluid->partNumel[0], csound ? csound->devicePtr[0] : nullptr, gamma,
+                    csound ? 0 : 1, dim, xj, halve, geom->h[0], geom->h[1], geom->h[2], hmin, st->devScalars,
+                    st->stream);
+    prof_end(slot, st->stream);
+    if (rc) return rc;
+    rc = IM_HIP(hipMemcpyAsync(st->pinned, st->devScalars, sizeof(double), hipMemcpyDeviceToHost, st->stream));
+    if (rc) return rc;
+    rc = IM_HIP(hipStreamSynchronize(st->stream));
+    if (rc) return rc;
+    tstep[0] = st->pinned[0];
+    return SUCCESSFUL;
+}
+
+int computeLocalCFLTimestep(MGArray *fluid, MGArray *csound, GeometryParams *geom, int method,
+                            double *globalResolution, double *tstep)
+{
+    if (!csound) return IM_FAIL(ERROR_NULL_POINTER, "null sound speed array");
+    if (csound->nGPUs != 1 || csound->numel != fluid->numel)
+        return IM_FAIL(ERROR_INVALID_ARGS, "sound speed array does not match the fluid");
+    return cfl_common(fluid, csound, geom, method, 0.0, globalResolution, tstep);
+}
+
+int imogen_cflTimestepFused(MGArray *fluid, GeometryParams *geom, int method, double gamma,
+                            double *globalResolution, double *tstep)
+{
+    return cfl_common(fluid, nullptr, geom, method, gamma, globalResolution, tstep);
+}
+
+}  // extern "C"

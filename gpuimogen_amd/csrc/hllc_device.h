@@ -1,0 +1,184 @@
+// hllc_device.h -- per-cell arithmetic of the HLLC sweep (gfx950, FP64).
+//
+// Same formulas, in the same operation order, as the reference kernel
+// cukern_HLLC_2ndorder (gpuclass/cudaFluidStep.cu:1124-1716): primitive
+// recovery with RECONSTRUCT_VELOCITY (gpuclass/Makefile:5), half-slope
+// limiters (gpuclass/limiter_functions.h:50-117), Einfeldt speed bounds
+// (cudaFluidStep.cu:1728-1754), Batten et al. contact speed (:1357) and the
+// four-way HLLC flux selection (:1363-1431). The data movement around these
+// functions (which lane owns which cell, how neighbours are exchanged) is this
+// library's own; only the per-cell algebra is shared with the reference.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace imogen {
+
+// Variables of one cell or face, ordered (rho, e|P, vN, vT1, vT2): normal
+// velocity first, then the two tangential components in the order the
+// reference kernels read them (X: y,z; Y: x,z; Z: y,x).
+struct V5 {
+    double a, b, c, d, e;
+};
+
+struct HydroConsts {
+    double gamma;     // adiabatic index
+    double gm1;       // gamma - 1
+    double govergm1;  // gamma / (gamma - 1)
+};
+
+__device__ __forceinline__ V5 operator-(const V5 &x, const V5 &y)
+{
+    return V5{x.a - y.a, x.b - y.b, x.c - y.c, x.d - y.d, x.e - y.e};
+}
+
+// ---- half-slope limiters (limiter_functions.h) ----------------------------
+__device__ __forceinline__ double flux_minmod(double a, double b)
+{
+    if (a * b < 0) return 0.0;
+    return (fabs(a) > fabs(b)) ? b : a;
+}
+
+template <int LIM>
+__device__ __forceinline__ double slope_limit(double a, double b, int rt)
+{
+    const int which = (LIM >= 0) ? LIM : rt;
+    switch (which) {
+    case 0: {  // Ospre: .75 A B (A+B) / (A^2 + AB + B^2)
+        double r = a * b;
+        if (r > 0) {
+            double s = a + b;
+            return .75 * r * s / (s * s - r);
+        }
+        return 0.0;
+    }
+    case 1: {  // van Leer: A B / (A+B)
+        double r = a * b;
+        if (r > 0.0) return r / (a + b);
+        return 0.0;
+    }
+    case 2: {  // minmod, returned through float as the reference does (:70)
+        if (a * b < 0) return 0.0;
+        return (double)(float)((fabs(a) > fabs(b)) ? .5 * b : .5 * a);
+    }
+    case 3: {  // superbee (first branch without the 1/2, :80-86)
+        if (a * b < 0) return 0.0;
+        if (b < a) return flux_minmod(a, 2 * b);
+        return .5 * flux_minmod(2 * a, b);
+    }
+    case 4: {  // van Albada
+        double r = a * b;
+        if (r < 0) return 0;
+        return .5 * r * (a + b) / (a * a + b * b);
+    }
+    case 5: {  // monotonized central
+        if (a * b <= 0) return 0;
+        double r = b / a;
+        double s = .25 + .25 * r;
+        s = (s < r) ? s : r;
+        s = (s < 1) ? s : 1.0;
+        return s * a;
+    }
+    default:
+        return 0.0;
+    }
+}
+
+// Conserved (rho, E, pN, pT1, pT2) -> primitive (rho, e_int, vN, vT1, vT2),
+// cudaFluidStep.cu:1218-1234.
+__device__ __forceinline__ V5 to_prim(const V5 &q)
+{
+    double F = 1.0 / q.a;
+    double B = q.b - .5 * (q.c * q.c + q.d * q.d + q.e * q.e) * F;
+    return V5{q.a, B, q.c * F, q.d * F, q.e * F};
+}
+
+// Left/right face states of a cell from its primitive w and the backward
+// differences dl = w_i - w_{i-1}, dr = w_{i+1} - w_i. The energy slot holds
+// pressure (gm1 * e) on the faces (cudaFluidStep.cu:1277-1278).
+template <int LIM>
+__device__ __forceinline__ void faces(const HydroConsts &k, const V5 &w, const V5 &dl, const V5 &dr,
+                                      int rt, V5 &L, V5 &R)
+{
+    double s;
+    s = slope_limit<LIM>(dl.a, dr.a, rt);
+    L.a = w.a - s;
+    R.a = w.a + s;
+    s = slope_limit<LIM>(dl.b, dr.b, rt);
+    L.b = k.gm1 * (w.b - s);
+    R.b = k.gm1 * (w.b + s);
+    s = slope_limit<LIM>(dl.c, dr.c, rt);
+    L.c = w.c - s;
+    R.c = w.c + s;
+    s = slope_limit<LIM>(dl.d, dr.d, rt);
+    L.d = w.d - s;
+    R.d = w.d + s;
+    s = slope_limit<LIM>(dl.e, dr.e, rt);
+    L.e = w.e - s;
+    R.e = w.e + s;
+}
+
+// HLLC flux through the interface between face state L (right face of cell i)
+// and R (left face of cell i+1). Returns (mass, energy, pN, pT1, pT2) fluxes.
+// The four reference cases are evaluated without divergence: the upwind side
+// and its wave speed are selected first, then the plain and star fluxes of
+// that side are both formed and one is kept.
+__device__ __forceinline__ V5 hllc_flux(const HydroConsts &k, const V5 &L, const V5 &R)
+{
+    // velocity jump squared (cudaFluidStep.cu:1318-1322)
+    double C = L.c - R.c;
+    double D = L.d - R.d;
+    double E = L.e - R.e;
+    double dvsq = C * C + D * D + E * E;
+
+    // Einfeldt bounds (cudaFluidStep.cu:1728-1754)
+    double csql = k.gamma * L.b / L.a;
+    double csqr = k.gamma * R.b / R.a;
+    double Fa = sqrt(R.a / L.a);
+    double Fb = 1.0 / (1.0 + Fa);
+    double Croe = ((csql + Fa * csqr) + .5 * k.gm1 * dvsq * Fa * Fb) * Fb;
+    double Vroe = (L.c + Fa * R.c) * Fb;
+    Croe = Vroe - sqrt(Croe);
+    double sl = L.c - sqrt(csql);
+    double sr = R.c + sqrt(csqr);
+    sl = (Croe < sl) ? Croe : sl;
+    Croe = 2 * Vroe - Croe;
+    sr = (Croe > sr) ? Croe : sr;
+
+    // contact speed, Batten et al. eq. 34 (cudaFluidStep.cu:1352-1357)
+    double A = L.a * (sl - L.c);
+    double B = R.a * (sr - R.c);
+    double sm = (B * R.c - R.b - A * L.c + L.b) / (B - A);
+
+    // case selection (cudaFluidStep.cu:1363-1369)
+    const bool left = sm > 0;
+    const bool star = left ? !(sl > 0) : (sr > 0);
+    const V5 S = left ? L : R;
+    const double sk = left ? sl : sr;
+
+    double ksum = S.c * S.c + S.d * S.d + S.e * S.e;  // |v|^2 of the upwind face
+    V5 out;
+    if (!star) {
+        // cases 0/1 (cudaFluidStep.cu:1372-1383, 1416-1430)
+        double Ek = .5 * ksum * S.a;
+        double m = S.a * S.c;
+        out.a = m;
+        out.b = S.c * (Ek + k.govergm1 * S.b);
+        out.c = m * S.c + S.b;
+        out.d = S.d * m;
+        out.e = S.e * m;
+    } else {
+        // cases 2/3 (cudaFluidStep.cu:1384-1415)
+        double rb = S.a * (sk - S.c) / (sk - sm);  // rho * beta
+        double G = .5 * ksum;
+        double m = rb * sm;
+        double Cc = sk * (sm - S.c);
+        out.a = m;
+        out.b = rb * sm * (k.govergm1 * S.b / S.a + G + Cc);
+        out.c = S.b + rb * (Cc + sm * S.c);
+        out.d = S.d * m;
+        out.e = S.e * m;
+    }
+    return out;
+}
+
+}  // namespace imogen

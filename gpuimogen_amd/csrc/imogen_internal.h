@@ -1,0 +1,57 @@
+// imogen_internal.h -- shared host-side plumbing of libimogen_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/imogen_abi.h"
+
+namespace imogen {
+
+// Records a HIP failure (with where it happened) and returns ERROR_CUDA_BLEW_UP,
+// mirroring CHECK_CUDA_ERROR of the reference (gpuclass/cudaCommon.h:331-337).
+int check_hip(hipError_t e, const char *what, const char *file, int line);
+// Records an Imogen status (non-zero) with a message and passes it through.
+int fail(int status, const char *file, int line, const char *fmt, ...);
+
+#define IM_HIP(call) ::imogen::check_hip((call), #call, __FILE__, __LINE__)
+#define IM_LAUNCHED(name) ::imogen::check_hip(hipGetLastError(), name, __FILE__, __LINE__)
+#define IM_FAIL(status, ...) ::imogen::fail((status), __FILE__, __LINE__, __VA_ARGS__)
+
+// Per-device state: one stream all kernels go to, a growable scratch block
+// and a small pinned host buffer for scalar read-back.
+struct DeviceState {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    void *scratch = nullptr;
+    size_t scratchBytes = 0;
+    double *pinned = nullptr;   // 64 doubles of pinned host memory
+    double *devScalars = nullptr;  // 64 doubles of device memory for reductions
+};
+
+// State of the current device (initialised on first use).
+DeviceState *state();
+
+// Optional per-kernel timing with HIP events on the launch stream
+// (imogen_profile_enable / imogen_profile_read). prof_begin returns a slot
+// (or -1 when timing is off or the event pool is exhausted).
+enum ProfKernel { PROF_SWEEP_X = 0, PROF_SWEEP_Y = 1, PROF_SWEEP_Z = 2, PROF_BC = 3, PROF_CFL = 4,
+                  PROF_DRAG = 5, PROF_MLP = 6, PROF_NKERNELS = 8 };
+int prof_begin(int kernel, hipStream_t s);
+void prof_end(int slot, hipStream_t s);
+// Device scratch of at least `bytes`; contents are undefined on return.
+void *scratch(size_t bytes);
+
+// Launchers (fluid_hllc.hip, statics.hip, cfl.hip)
+int launch_hllc_sweep(double *base, int64_t pitch, int nx, int ny, int nz, int dir, double lambda,
+                      double gamma, int limiter, hipStream_t s);
+int launch_fluid_bc(double *base, int64_t pitch, int nx, int ny, int nz, int dir, const int *modes10,
+                    int depth, hipStream_t s);
+int launch_soundspeed(const double *base, int64_t pitch, int64_t n, double gamma, double *cs,
+                      hipStream_t s);
+int launch_cfl(const double *base, int64_t pitch, int64_t n, const double *cs, double gamma, int fused,
+               int dim, int xinjin, int halve, double hx, double hy, double hz, double hmin, double *devOut,
+               hipStream_t s);
+
+}  // namespace imogen

@@ -1,0 +1,132 @@
+// statics.hip -- ghost-cell boundary conditions of the fluid arrays.
+//
+// Restates the kernel table of setBoundarySAS / callBCKernel
+// (gpuclass/cudaStatics.cu:1086-1240) with its per-direction index
+// conventions (cudaStatics.cu:1290-1532), including the reference's quirks:
+//   * z-minus mirror reflects about cell 4.5 (index 2*depth+1-q), x/y about 4;
+//   * +y and +z 'const' copy the innermost ghost cell (n-depth), +x copies the
+//     last interior cell (n-depth-1).
+// One launch handles one face (side) of one direction for all 5 slabs, which
+// preserves the reference's side-0-then-side-1 order.
+#include "imogen_internal.h"
+
+namespace imogen {
+
+// sas codes per slab: -1 none, 0 symmetric, 1 antisymmetric, 2 constant, 3 linear
+struct SasTable {
+    int s[5];
+};
+
+__global__ void bc_face_x(double *__restrict__ q, int64_t pitch, int nx, int64_t rows, int side, int d,
+                          SasTable t)
+{
+    // one thread per (ghost cell, slab, row)
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = rows * 5 * d;
+    if (tid >= total) return;
+    const int g = (int)(tid % d);
+    const int slab = (int)((tid / d) % 5);
+    const int64_t row = tid / (5 * d);
+    const int sas = t.s[slab];
+    if (sas < 0) return;
+    double *r = q + slab * pitch + row * nx;
+    if (side == 0) {
+        switch (sas) {
+        case 0: r[d - 1 - g] = r[d + 1 + g]; break;            // cudaStatics.cu:1308-1313
+        case 1: r[d - 1 - g] = -r[d + 1 + g]; break;           // :1315-1320
+        case 2: r[g] = r[d]; break;                            // :1339-1344
+        case 3: r[g] = r[d] + (d - g) * (r[d] - r[d + 1]); break;  // :1372-1380
+        }
+    } else {
+        switch (sas) {
+        case 0: r[nx - d + g] = r[nx - d - 2 - g]; break;      // :1322-1327
+        case 1: r[nx - d + g] = -r[nx - d - 2 - g]; break;     // :1329-1334
+        case 2: r[nx - d + g] = r[nx - d - 1]; break;          // :1361-1366
+        }
+    }
+}
+
+// memdir 2 (y) and 3 (z): one thread per (face cell, slab), looping over depth
+__global__ void bc_face_yz(double *__restrict__ q, int64_t pitch, int nx, int64_t n1, int64_t n2,
+                           int64_t stride, int64_t ostride, int N, int zdir, int side, int d, SasTable t)
+{
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t face = (int64_t)nx * n2;
+    const int slab = (int)blockIdx.y;
+    if (tid >= face) return;
+    const int sas = t.s[slab];
+    if (sas < 0) return;
+    (void)n1;
+    const int64_t x = tid % nx;
+    const int64_t o = tid / nx;
+    double *c = q + slab * pitch + x + o * ostride;
+    if (side == 0) {
+        const int off = zdir ? 2 * d + 1 : 2 * d;  // z-minus mirror quirk (:1496-1500)
+        switch (sas) {
+        case 0:
+            for (int g = 0; g < d; g++) c[g * stride] = c[(off - g) * stride];
+            break;
+        case 1:
+            for (int g = 0; g < d; g++) c[g * stride] = -c[(off - g) * stride];
+            break;
+        case 2: {
+            const double f = c[d * stride];
+            for (int g = 0; g < d; g++) c[g * stride] = f;
+        } break;
+        }
+    } else {
+        switch (sas) {
+        case 0:
+            for (int g = 0; g < d; g++) c[(N - 1 - g) * stride] = c[(N - 1 - 2 * d + g) * stride];
+            break;
+        case 1:
+            for (int g = 0; g < d; g++) c[(N - 1 - g) * stride] = -c[(N - 1 - 2 * d + g) * stride];
+            break;
+        case 2: {
+            const double f = c[(N - d) * stride];  // innermost ghost (:1461-1466, :1524-1532)
+            for (int g = 0; g < d; g++) c[(N - d + g) * stride] = f;
+        } break;
+        }
+    }
+}
+
+int launch_fluid_bc(double *base, int64_t pitch, int nx, int ny, int nz, int dir, const int *modes10,
+                    int depth, hipStream_t s)
+{
+    const int N = (dir == 1) ? nx : (dir == 2 ? ny : nz);
+    for (int side = 0; side < 2; side++) {
+        SasTable t;
+        bool any = false;
+        for (int k = 0; k < 5; k++) {
+            t.s[k] = modes10[2 * k + side];
+            if (t.s[k] >= 0) {
+                any = true;
+                if ((t.s[k] == 0 || t.s[k] == 1) && N < 2 * depth + 2)
+                    return IM_FAIL(ERROR_INVALID_ARGS, "mirror BC needs >= %d cells (got %d)", 2 * depth + 2, N);
+                if (N < depth + 2)
+                    return IM_FAIL(ERROR_INVALID_ARGS, "BC needs >= %d cells (got %d)", depth + 2, N);
+                if (t.s[k] == 3 && (dir != 1 || side != 0))
+                    return IM_FAIL(ERROR_NOIMPLEMENT, "linear extrapolation BC exists only on the -x face");
+            }
+        }
+        if (!any) continue;
+        if (dir == 1) {
+            const int64_t rows = (int64_t)ny * nz;
+            const int64_t total = rows * 5 * depth;
+            hipLaunchKernelGGL(bc_face_x, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, base, pitch,
+                               nx, rows, side, depth, t);
+        } else {
+            const int64_t stride = (dir == 2) ? (int64_t)nx : (int64_t)nx * ny;
+            const int64_t ostride = (dir == 2) ? (int64_t)nx * ny : (int64_t)nx;
+            const int64_t n2 = (dir == 2) ? nz : ny;
+            const int64_t face = (int64_t)nx * n2;
+            hipLaunchKernelGGL(bc_face_yz, dim3((unsigned)((face + 255) / 256), 5), dim3(256), 0, s, base,
+                               pitch, nx, (int64_t)N, n2, stride, ostride, N, dir == 3 ? 1 : 0, side, depth, t);
+        }
+        int rc = IM_LAUNCHED("boundary condition kernel");
+        if (rc) return rc;
+    }
+    return SUCCESSFUL;
+}
+
+}  // namespace imogen

@@ -1,0 +1,51 @@
+"""Initial conditions of the benchmark and test problems (host numpy, FP64).
+
+States are shaped (5, nz, ny, nx) with slabs (rho, E, px, py, pz), x fastest.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def conserved(rho, vx, vy, vz, P, gamma):
+    q = np.empty((5,) + np.shape(rho))
+    q[0] = rho
+    q[1] = P / (gamma - 1) + 0.5 * rho * (vx * vx + vy * vy + vz * vz)
+    q[2] = rho * vx
+    q[3] = rho * vy
+    q[4] = rho * vz
+    return q
+
+
+def sedov3d(n=256, gamma=5.0 / 3.0, energy=1.0, p_background=1e-5, r2max=9):
+    """Sedov-Taylor blast on the unit cube, n^3 cells, dx = 1/n.
+
+    rho = 1, v = 0, P = p_background; the total energy `energy` is deposited as
+    internal energy, uniformly, in every cell whose integer offset (i,j,k) from
+    the centre cell satisfies i^2+j^2+k^2 <= r2max (123 cells for r2max = 9).
+    The centre cell is 0-based index floor(n/2 + 0.5) - 1 on each axis, the
+    reference's 1-based floor(N/2+0.5) convention
+    (experiment/SedovTaylorBlastWave/SedovTaylorBlastWaveInitializer.m:95).
+    Returns (state, h, deposit_cell_count)."""
+    dx = 1.0 / n
+    c = int(np.floor(n / 2 + 0.5)) - 1
+    q = np.zeros((5, n, n, n))
+    q[0] = 1.0
+    q[1] = p_background / (gamma - 1)
+    r = int(np.floor(np.sqrt(r2max)))
+    offs = [(i, j, k) for i in range(-r, r + 1) for j in range(-r, r + 1) for k in range(-r, r + 1)
+            if i * i + j * j + k * k <= r2max]
+    e_cell = energy / (len(offs) * dx ** 3)
+    for (i, j, k) in offs:
+        q[1, c + k, c + j, c + i] = e_cell
+    return q, (dx, dx, dx), len(offs)
+
+
+def sod1d(n=1024, gamma=1.4):
+    """Sod shock tube on [0,1]: (rho, v, P) = (1,0,1) | (0.125,0,0.1) at x = 0.5."""
+    dx = 1.0 / n
+    x = (np.arange(n) + 0.5) * dx
+    rho = np.where(x < 0.5, 1.0, 0.125)
+    P = np.where(x < 0.5, 1.0, 0.1)
+    z = np.zeros(n)
+    return conserved(rho, z, z, z, P, gamma).reshape(5, 1, 1, n), (dx, 1.0, 1.0)

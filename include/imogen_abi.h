@@ -1,0 +1,269 @@
+/*
+ * imogen_abi.h -- C ABI of libimogen_hip.so, the MI355X (gfx950) fluid step.
+ *
+ * Every entry point keeps the name, argument meaning, ownership rules and
+ * status-code convention of the reference's compiled core (the functions its
+ * MEX gateways call), so a MEX gateway, a cgo/ctypes binding or a C++ driver
+ * binds it the way the reference's own gateways bind the CUDA build. Each
+ * declaration cites the reference interface it replaces.
+ *
+ * Plain C: no HIP, torch or MATLAB types in any signature. Device pointers are
+ * carried inside MGArray / int64 tags exactly as in the reference.
+ */
+#ifndef IMOGEN_ABI_H
+#define IMOGEN_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes: gpuclass/imogenChecks.h:12-24 ----------------------- */
+#define SUCCESSFUL 0
+#define ERROR_INVALID_ARGS -100
+#define ERROR_CRASH -101
+#define ERROR_NULL_POINTER -102
+#define ERROR_GET_GPUTAG_FAILED -103
+#define ERROR_DESERIALIZE_GPUTAG_FAILED -104
+#define ERROR_NOIMPLEMENT -105
+#define ERROR_NOMEM -106
+#define ERROR_LIBFAILED -107
+#define ERROR_NOFILE -108
+#define ERROR_FILE_IO -109
+#define ERROR_CUDA_BLEW_UP -1000
+
+/* ---- array tag layout: gpuclass/cudaCommon.h:184-196 -------------------- */
+#define MAX_GPUS_USED 4
+#define GPU_TAG_LENGTH 11
+#define GPU_TAG_DIM0 0
+#define GPU_TAG_DIM1 1
+#define GPU_TAG_DIM2 2
+#define GPU_TAG_DIMSLAB 3
+#define GPU_TAG_HALO 4
+#define GPU_TAG_PARTDIR 5
+#define GPU_TAG_NGPUS 6
+#define GPU_TAG_EXTERIORHALO 7
+#define GPU_TAG_DIMPERMUTATION 8
+#define GPU_TAG_CIRCULARBITS 9
+#define GPU_TAG_VECTOR_COMPONENT 10
+#define GPU_TAG_MAXLEN (GPU_TAG_LENGTH + 2 * MAX_GPUS_USED)
+
+#define PARTITION_X 1
+#define PARTITION_Y 2
+#define PARTITION_Z 3
+
+/* ---- boundary modes: gpuclass/cudaCommon.h:49 --------------------------- */
+typedef enum BCModeTypes {
+    circular, mirror, wall, stationary, extrapConstant, extrapLinear, outflow, freebalance
+} BCModeTypes;
+
+/* gpuclass/cudaCommon.h:51-60 */
+typedef struct BCSettings {
+    BCModeTypes mode[6]; /* [-X, +X, -Y, +Y, -Z, +Z] */
+    void *externalData;  /* caller-owned (MATLAB class handle in the MEX build); unused here */
+    int extIndex;
+    double *staticCells[MAX_GPUS_USED];
+    int nStatics[4];
+} BCSettings;
+
+/* gpuclass/cudaCommon.h:66-118 */
+typedef struct MGArray {
+    int dim[3];                        /* size excluding partition halos */
+    int64_t numel;
+    int numSlabs;                      /* >0: owner of N slabs; <=0: view of slab -numSlabs */
+    int64_t slabPitch[MAX_GPUS_USED];  /* bytes per slab, rounded up to 256 */
+    int haloSize;
+    int partitionDir;
+    int addExteriorHalo;
+    int permtag;
+    int currentPermutation[3];
+    int nGPUs;
+    int deviceID[MAX_GPUS_USED];
+    double *devicePtr[MAX_GPUS_USED];
+    int partNumel[MAX_GPUS_USED];
+    int mpiCircularBoundaryBits;
+    BCSettings boundaryConditions;
+    int vectorComponent;               /* 0 scalar, 1/2/3 = x/y/z component */
+} MGArray;
+
+/* gpuclass/cudaCommon.h:122 */
+typedef enum geometryType_t { SQUARE, CYLINDRICAL, RZSQUARE, RZCYLINDRICAL } geometryType_t;
+
+/* gpuclass/cudaCommon.h:122-144 */
+typedef struct GeometryParams {
+    geometryType_t shape;
+    double h[3];
+    double x0, y0, z0;
+    double Rinner;
+    double frameRotateCenter[3];
+    double frameOmega;
+    int globalRez[3];
+    int localRez[3];
+    int gridAffine[3];
+    MGArray *XYVector;
+} GeometryParams;
+
+/* gpuclass/cudaCommon.h:146-158 */
+typedef struct ThermoDetails {
+    double gamma;
+    double m;
+    double Cisothermal; /* -1 = adiabatic */
+    double kBolt;
+    double mu0;
+    double muTindex;
+    double sigma0;
+    double sigmaTindex;
+} ThermoDetails;
+
+/* gpuclass/cudaFluidStep.h:6-8 */
+typedef enum FluidMethods { METHOD_HLL = 1, METHOD_HLLC = 2, METHOD_XINJIN = 3 } FluidMethods;
+
+/* Slope limiter of the MUSCL reconstruction. The reference selects it at
+ * compile time (SLOPEFUNC, gpuclass/cudaFluidStep.cu:127-133; default Ospre);
+ * here it is a run-time field of FluidStepParams, 0 = the reference default. */
+typedef enum SlopeLimiters {
+    LIMITER_OSPRE = 0, LIMITER_VANLEER = 1, LIMITER_MINMOD = 2, LIMITER_SUPERBEE = 3,
+    LIMITER_VANALBADA = 4, LIMITER_MC = 5, LIMITER_ZERO = 6
+} SlopeLimiters;
+
+/* gpuclass/cudaFluidStep.h:10-26, plus slopeLimiter appended (see above) */
+typedef struct FluidStepParams {
+    double dt;
+    double cflPrefactor;
+    int onlyHydro;
+    double thermoGamma;
+    double minimumRho;
+    double Cisothermal;
+    FluidMethods stepMethod;
+    int stepDirection;  /* performFluidUpdate_1D: 1/2/3 = X/Y/Z; _3D: +1 forward, -1 backward */
+    int stepNumber;
+    GeometryParams geometry;
+    int multifluidDragMethod;
+    int slopeLimiter;   /* SlopeLimiters */
+} FluidStepParams;
+
+/* mpi/mpi_common.h:14-29 without MPI types. One rank: nproc = {1,1,1}. */
+typedef struct ParallelTopology {
+    int ndim;
+    int comm;
+    int nproc[3];
+    int coord[3];
+    int neighbor_left[3];
+    int neighbor_right[3];
+    int dimcomm[3];
+} ParallelTopology;
+
+/* ======================================================================== */
+/* Array-tag runtime (gpuclass/cudaCommon.cu, gpuclass/cudaCommon.h:257-296) */
+/* ======================================================================== */
+
+/* cudaCommon.cu:50-91 */
+void calcPartitionExtent(MGArray *m, int P, int *sub);
+/* cudaCommon.cu:117-172 */
+int deserializeTagToMGArray(int64_t *tag, MGArray *mg);
+/* cudaCommon.cu:175-195 */
+void serializeMGArrayToTag(MGArray *mg, int64_t *tag);
+/* cudaCommon.cu:198-212 */
+void MGA_permtagToNums(int permtag, int *p);
+/* cudaCommon.cu:216-235 */
+int MGA_numsToPermtag(int *nums);
+/* cudaCommon.cu:237-246 */
+int MGA_dir2memdir(int *perm, int dir);
+/* cudaCommon.cu:256-335: *ret == NULL -> malloc'd array of N MGArrays */
+int MGA_allocArrays(MGArray **ret, int N, MGArray *skeleton);
+/* cudaCommon.cu:337-405 */
+int MGA_allocSlab(MGArray *skeleton, MGArray *nu, int Nslabs);
+/* cudaCommon.cu:407-468: frees owners (numSlabs > 0), ignores slab views */
+int MGA_delete(MGArray *victim);
+/* cudaCommon.cu:1867-2046: host array (dim[0]*dim[1]*dim[2] doubles) -> device */
+int MGA_uploadArrayToGPU(double *p, MGArray *g, int partitionTo);
+/* cudaCommon.cu:2048-2217: device -> host; *p == NULL -> malloc'd for the caller */
+int MGA_downloadArrayToCPU(MGArray *g, double **p, int partitionFrom);
+
+/* ======================================================================== */
+/* MEX-level tag primitives (gpuclass/GPU_*.cu) with host buffers and int64  */
+/* tags in place of mxArrays. Tags are GPU_TAG_LENGTH + 2 int64.             */
+/* ======================================================================== */
+
+/* GPU_upload.cu:20-130: tag = GPU_upload(host, devList, [halo partDir extHalo]) */
+int GPU_upload(const double *host, const int *dims3, int haloSize, int partitionDir, int64_t *tagOut);
+/* GPU_download.cu:20-84 */
+int GPU_download(const int64_t *tag, double *host);
+/* GPU_free.cu:15-43 */
+int GPU_free(const int64_t *tag);
+/* GPU_makeslab.cu:15-72: new owner with N slabs; slab 0 copied from the old array, old freed */
+int GPU_makeslab(const int64_t *tag, int N, int64_t *newTag);
+/* GPU_getslab.cu:15-47: view of slab k (numSlabs = -k) */
+int GPU_getslab(const int64_t *tag, int k, int64_t *viewTag);
+/* GPU_setslab.cu:15-81: upload host data into slab k and return its view */
+int GPU_setslab(const int64_t *tag, int k, const double *host, int64_t *viewTag);
+/* GPU_clone.cu:15-50 */
+int GPU_clone(const int64_t *tag, int64_t *newTag);
+/* GPU_copy.cu:15-50: dst <- src, identical shapes */
+int GPU_copy(const int64_t *dstTag, const int64_t *srcTag);
+/* GPU_ctrl.cu 'memory' query */
+int GPU_ctrl_memory(int device, size_t *freeBytes, size_t *totalBytes);
+
+/* ======================================================================== */
+/* Fluid step core                                                           */
+/* ======================================================================== */
+
+/* gpuclass/flux.h:17, flux.cu:27-155. fluid -> 5 MGArrays (rho E px py pz), the
+ * slab views of one 5-slab owner. One `fluidstep`: 3 directional SSP-RK2 sweeps
+ * in the order (stepNumber + 3*(stepDirection>0)) % 6, each followed by its
+ * boundary conditions. tempStorage may be NULL (scratch is library-cached). */
+int performFluidUpdate_3D(MGArray *fluid, ParallelTopology *parallelTopo, FluidStepParams fsp,
+                          MGArray *tempStorage);
+
+/* gpuclass/cudaFluidStep.h:56-58, cudaFluidStep.cu:233-663: one directional
+ * predictor+corrector along params.stepDirection (1/2/3 = X/Y/Z). */
+int performFluidUpdate_1D(MGArray *fluid, FluidStepParams params, ParallelTopology *topo,
+                          MGArray *tmpst);
+
+/* gpuclass/cudaStatics.h: setFluidBoundary (cudaStatics.cu:112-196) and
+ * setArrayBoundaryConditions (cudaStatics.cu:199-286). direction 1/2/3,
+ * side 0 = minus face, 1 = plus face. */
+int setFluidBoundary(MGArray *fluid, GeometryParams *geo, int direction);
+int setArrayBoundaryConditions(MGArray *phi, GeometryParams *geo, int direction, int side);
+
+/* gpuclass/cudaSoundspeed.h, cudaSoundspeed.cu:107-152 (mag must be NULL: hydro only) */
+int calculateSoundspeed(MGArray *fluid, MGArray *mag, MGArray *cs, double gamma);
+
+/* gpuclass/cflTimestep.h:11, cflTimestep.cu:107-243: tstep[0] = the CFL time
+ * step limit (already halved for HLL/HLLC). Synchronises the device. */
+int computeLocalCFLTimestep(MGArray *fluid, MGArray *csound, GeometryParams *geom, int method,
+                            double *globalResolution, double *tstep);
+
+/* Same result as calculateSoundspeed + computeLocalCFLTimestep, in one pass:
+ * the sound speed is formed in-line from gamma (no counterpart in the
+ * reference, which reads a separately computed sound-speed array). */
+int imogen_cflTimestepFused(MGArray *fluid, GeometryParams *geom, int method, double gamma,
+                            double *globalResolution, double *tstep);
+
+/* ======================================================================== */
+/* Runtime utilities (no counterpart in the reference: it used the legacy   */
+/* default stream; this library issues all work on one stream per device).  */
+/* ======================================================================== */
+
+/* The hipStream_t all kernels of this library are launched on (device 0). */
+void *imogen_stream(void);
+/* Blocks until all work of this library on the current device is done. */
+int imogen_synchronize(void);
+/* Per-kernel timing with HIP events on the library stream. enable(1) clears
+ * the record and starts timing every launch; read() returns the number of
+ * timed launches and their summed duration for a kernel id:
+ * 0/1/2 = X/Y/Z fluid sweep, 3 = boundary conditions, 4 = CFL reduction,
+ * 5 = drag source, 6 = learned drag. */
+int imogen_profile_enable(int on);
+int imogen_profile_read(int kernel, int64_t *count, double *totalMs);
+/* Text of the last error recorded by the library (never NULL). */
+const char *imogen_last_error(void);
+/* Library version string. */
+const char *imogen_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,607 @@
+/*
+ * imogen_oracle.c -- FP64 CPU restatement of the Imogen fluid step.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker of the HIP path; never shipped,
+ * never called by the product library). Built with -ffp-contract=off so every
+ * expression is evaluated exactly as written, in the reference's operation
+ * order.
+ *
+ * Each function cites the reference code it restates. Parity status of each
+ * part (what pins it) is listed in the test files under tests/.
+ */
+#include "imogen_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_OK 0
+#define ORC_EINVAL (-100)
+#define ORC_ENOIMPL (-105)
+#define ORC_ENOMEM (-106)
+
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+int orc_get_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* ------------------------------------------------------------------------ */
+/* Limiters: gpuclass/limiter_functions.h:11-119                              */
+/* ------------------------------------------------------------------------ */
+
+/* fluxLimiter_minmod, limiter_functions.h:22-27 */
+static double flux_minmod(double a, double b)
+{
+    if (a * b < 0) return 0.0;
+    if (fabs(a) > fabs(b)) return b;
+    return a;
+}
+
+/* fluxLimiter_Ospre, limiter_functions.h:37-43 (used by the Xin-Jin corrector) */
+static double flux_ospre(double a, double b)
+{
+    double r = a * b;
+    if (r <= 0.0) return 0.0;
+    return 1.5 * r * (a + b) / (a * a + r + b * b);
+}
+
+/* half-slope limiters, limiter_functions.h:50-117 */
+static double slope_ospre(double a, double b)
+{
+    double r = a * b;
+    if (r > 0) {
+        double s = a + b;
+        return .75 * r * s / (s * s - r);
+    }
+    return 0.0;
+}
+
+static double slope_vanleer(double a, double b)
+{
+    double r = a * b;
+    if (r > 0.0) return r / (a + b);
+    return 0;
+}
+
+/* slopeLimiter_minmod returns float in the reference (limiter_functions.h:70) */
+static double slope_minmod(double a, double b)
+{
+    if (a * b < 0) return (double)(float)0.0;
+    if (fabs(a) > fabs(b)) return (double)(float)(.5 * b);
+    return (double)(float)(.5 * a);
+}
+
+/* slopeLimiter_superbee, limiter_functions.h:80-86 (first branch has no 1/2) */
+static double slope_superbee(double a, double b)
+{
+    if (a * b < 0) return 0.0;
+    if (b < a) return flux_minmod(a, 2 * b);
+    return .5 * flux_minmod(2 * a, b);
+}
+
+static double slope_vanalbada(double a, double b)
+{
+    double r = a * b;
+    if (r < 0) return 0;
+    return .5 * r * (a + b) / (a * a + b * b);
+}
+
+static double slope_mc(double a, double b)
+{
+    if (a * b <= 0) return 0;
+    double r = b / a;
+    double s = .25 + .25 * r;
+    s = (s < r) ? s : r;
+    s = (s < 1) ? s : 1.0;
+    return s * a;
+}
+
+double orc_slope_limiter(int which, double a, double b)
+{
+    switch (which) {
+    case ORC_LIM_OSPRE: return slope_ospre(a, b);
+    case ORC_LIM_VANLEER: return slope_vanleer(a, b);
+    case ORC_LIM_MINMOD: return slope_minmod(a, b);
+    case ORC_LIM_SUPERBEE: return slope_superbee(a, b);
+    case ORC_LIM_VANALBADA: return slope_vanalbada(a, b);
+    case ORC_LIM_MC: return slope_mc(a, b);
+    default: return 0.0;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* HLLC sweep: gpuclass/cudaFluidStep.cu:1124-1716, Einfeldt bound 1728-1754 */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    double gamma, gm1, govergm1;
+    int lim;
+} thermo_t;
+
+/* computeEinfeldtBounds, cudaFluidStep.cu:1728-1754 */
+static void einfeldt(const thermo_t *t, double rhoL, double vL, double PL, double rhoR, double vR,
+                     double PR, double dvsq, double *sl, double *sr)
+{
+    double csql = t->gamma * PL / rhoL;
+    double csqr = t->gamma * PR / rhoR;
+    double Fa = sqrt(rhoR / rhoL);
+    double Fb = 1.0 / (1.0 + Fa);
+    double Croe = ((csql + Fa * csqr) + .5 * t->gm1 * dvsq * Fa * Fb) * Fb;
+    double Vroe = (vL + Fa * vR) * Fb;
+    Croe = Vroe - sqrt(Croe);
+    double vl = vL - sqrt(csql);
+    double vr = vR + sqrt(csqr);
+    vl = (Croe < vl) ? Croe : vl;
+    Croe = 2 * Vroe - Croe;
+    vr = (Croe > vr) ? Croe : vr;
+    *sl = vl;
+    *sr = vr;
+}
+
+/* HLLC flux at one interface. L = right face of cell i, R = left face of cell
+ * i+1, each (rho, P, vN, vT1, vT2). out = (mass, energy, pN, pT1, pT2) flux.
+ * cudaFluidStep.cu:1331-1431 */
+static void hllc_flux(const thermo_t *t, const double *L, const double *R, double *out)
+{
+    double C = L[2] - R[2];
+    double D = L[3] - R[3];
+    double E = L[4] - R[4];
+    C = C * C + D * D + E * E;
+    double sl, sr;
+    einfeldt(t, L[0], L[2], L[1], R[0], R[2], R[1], C, &sl, &sr);
+    C = sl;
+    D = sr;
+    double A = L[0] * (C - L[2]);
+    double B = R[0] * (D - R[2]);
+    E = (B * R[2] - R[1] - A * L[2] + L[1]) / (B - A);
+
+    int mode;
+    if (E > 0) mode = (C > 0) ? 0 : 2;
+    else mode = (D > 0) ? 3 : 1;
+
+    double F, G, H;
+    switch (mode) {
+    case 0:
+    case 1: {
+        const double *S = (mode == 0) ? L : R;
+        A = S[0];
+        B = S[2];
+        C = S[3];
+        D = S[4];
+        E = .5 * (B * B + C * C + D * D) * A;
+        F = S[1];
+        A *= B;
+        C *= A;
+        D *= A;
+        G = A * B + F;
+        B *= (E + t->govergm1 * F);
+        out[0] = A;
+        out[1] = B;
+        out[2] = G;
+        out[3] = C;
+        out[4] = D;
+    } break;
+    case 2:
+        A = L[0];
+        B = L[2];
+        D = A * (C - B) / (C - E);
+        F = L[3];
+        H = L[4];
+        G = .5 * (B * B + F * F + H * H);
+        A = D * E;
+        F *= A;
+        H *= A;
+        C *= (E - B);
+        B = L[1] + D * (C + E * B);
+        G = D * E * (t->govergm1 * L[1] / L[0] + G + C);
+        out[0] = A;
+        out[1] = G;
+        out[2] = B;
+        out[3] = F;
+        out[4] = H;
+        break;
+    default: /* 3 */
+        A = R[0];
+        B = R[2];
+        C = A * (D - B) / (D - E);
+        F = R[3];
+        H = R[4];
+        G = .5 * (B * B + F * F + H * H);
+        A = C * E;
+        F *= A;
+        H *= A;
+        D *= (E - B);
+        B = R[1] + C * (D + E * B);
+        G = C * E * (t->govergm1 * R[1] / R[0] + G + D);
+        out[0] = A;
+        out[1] = G;
+        out[2] = B;
+        out[3] = F;
+        out[4] = H;
+        break;
+    }
+}
+
+/* Interface fluxes fl[v][i] (at i+1/2) of one periodic line of n cells.
+ * in[v] are the 5 line arrays ordered (rho, E, pN, pT1, pT2). ws must hold
+ * 15*n doubles. cudaFluidStep.cu:1185-1431 (MUSCL on rho, e_int, v with
+ * RECONSTRUCT_VELOCITY, gpuclass/Makefile:5). */
+static void hllc_line_fluxes(const thermo_t *t, int n, double *const in[5], double *fl[5], double *ws)
+{
+    double *prim = ws;           /* 5*n */
+    double *lf = ws + 5 * n;     /* 5*n left faces  (rho, P, vN, vT1, vT2) */
+    double *rf = ws + 10 * n;    /* 5*n right faces */
+    int i, v;
+    for (i = 0; i < n; i++) {
+        double A = in[0][i], B = in[1][i], C = in[2][i], D = in[3][i], E = in[4][i];
+        double F = 1.0 / A;
+        B -= .5 * (C * C + D * D + E * E) * F;
+        C *= F;
+        D *= F;
+        E *= F;
+        prim[0 * n + i] = A;
+        prim[1 * n + i] = B;
+        prim[2 * n + i] = C;
+        prim[3 * n + i] = D;
+        prim[4 * n + i] = E;
+    }
+    for (v = 0; v < 5; v++) {
+        const double *p = prim + v * n;
+        for (i = 0; i < n; i++) {
+            int im = (i == 0) ? n - 1 : i - 1;
+            int ip = (i == n - 1) ? 0 : i + 1;
+            double dl = p[i] - p[im];
+            double dr = p[ip] - p[i];
+            double s = orc_slope_limiter(t->lim, dl, dr);
+            if (v == 1) { /* store pressure, not internal energy (cudaFluidStep.cu:1277-1278) */
+                lf[v * n + i] = t->gm1 * (p[i] - s);
+                rf[v * n + i] = t->gm1 * (p[i] + s);
+            } else {
+                lf[v * n + i] = p[i] - s;
+                rf[v * n + i] = p[i] + s;
+            }
+        }
+    }
+    for (i = 0; i < n; i++) {
+        int ip = (i == n - 1) ? 0 : i + 1;
+        double L[5], R[5], o[5];
+        for (v = 0; v < 5; v++) {
+            L[v] = rf[v * n + i];
+            R[v] = lf[v * n + ip];
+        }
+        hllc_flux(t, L, R, o);
+        for (v = 0; v < 5; v++) fl[v][i] = o[v];
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Line gather/scatter over the 3D array                                     */
+/* ------------------------------------------------------------------------ */
+
+/* Slab used for (normal, tangent1, tangent2) momentum per direction, as the
+ * reference kernels read them (cudaFluidStep.cu:1199-1216). */
+static const int mom_slab[4][3] = {{0, 0, 0}, {2, 3, 4}, {3, 2, 4}, {4, 3, 2}};
+
+typedef struct {
+    long stride;   /* element stride along the line */
+    int n;         /* line length */
+    long nlines;
+} lineinfo_t;
+
+static void line_geometry(const orc_grid *g, int dir, lineinfo_t *li)
+{
+    switch (dir) {
+    case 1: li->stride = 1; li->n = g->nx; li->nlines = (long)g->ny * g->nz; break;
+    case 2: li->stride = g->nx; li->n = g->ny; li->nlines = (long)g->nx * g->nz; break;
+    default: li->stride = (long)g->nx * g->ny; li->n = g->nz; li->nlines = (long)g->nx * g->ny; break;
+    }
+}
+
+static long line_base(const orc_grid *g, int dir, long l)
+{
+    long nx = g->nx, ny = g->ny;
+    switch (dir) {
+    case 1: return l * nx;                              /* l = y + ny*z */
+    case 2: return (l % nx) + (l / nx) * nx * ny;       /* l = x + nx*z */
+    default: return l;                                  /* l = x + nx*y */
+    }
+}
+
+static int slab_for(int dir, int v)
+{
+    if (v < 2) return v;
+    return mom_slab[dir][v - 2];
+}
+
+/* ------------------------------------------------------------------------ */
+/* Directional update                                                        */
+/* ------------------------------------------------------------------------ */
+
+int orc_fluid_update_1d(double *q, const orc_grid *g, int dir, int method, double lambda,
+                        const orc_fluid_params *p)
+{
+    if (dir < 1 || dir > 3) return ORC_EINVAL;
+    lineinfo_t li;
+    line_geometry(g, dir, &li);
+    if (li.n < 3) return ORC_OK; /* cudaFluidStep.cu:301 */
+    if (method != ORC_HLLC) return ORC_ENOIMPL;
+
+    thermo_t t;
+    t.gamma = p->gamma;
+    t.gm1 = p->gamma - 1.0;
+    t.govergm1 = p->gamma / (p->gamma - 1.0);
+    t.lim = p->limiter;
+
+    int n = li.n;
+    int failed = 0;
+#pragma omp parallel
+    {
+        double *buf = (double *)malloc(sizeof(double) * (size_t)n * 35);
+        if (!buf) {
+#pragma omp atomic write
+            failed = 1;
+        }
+        long l;
+#pragma omp for schedule(static)
+        for (l = 0; l < li.nlines; l++) {
+            if (!buf) continue;
+            double *Q[5], *U[5], *FL[5];
+            double *ws = buf + 15 * n;
+            int v, i;
+            for (v = 0; v < 5; v++) {
+                Q[v] = buf + v * n;
+                U[v] = buf + 5 * n + v * n;
+                FL[v] = buf + 10 * n + v * n;
+            }
+            /* ws occupies [15n, 30n); the remaining 5n hold the updated line */
+            double *O[5];
+            for (v = 0; v < 5; v++) O[v] = buf + 30 * n + v * n;
+            long base = line_base(g, dir, l);
+            for (v = 0; v < 5; v++) {
+                const double *src = q + (long)slab_for(dir, v) * g->pitch + base;
+                for (i = 0; i < n; i++) Q[v][i] = src[(long)i * li.stride];
+            }
+            /* SSP-RK2 stage A: U = Q - lambda dF(Q)   (TIMESCHEME_SSPRK_A, :1510-1516) */
+            hllc_line_fluxes(&t, n, Q, FL, ws);
+            for (v = 0; v < 5; v++)
+                for (i = 0; i < n; i++) {
+                    int im = (i == 0) ? n - 1 : i - 1;
+                    U[v][i] = Q[v][i] - lambda * (FL[v][i] - FL[v][im]);
+                }
+            /* stage B: Q = .5 (U + Q - lambda dF(U))   (TIMESCHEME_SSPRK_B, :1555-1561) */
+            hllc_line_fluxes(&t, n, U, FL, ws);
+            for (v = 0; v < 5; v++)
+                for (i = 0; i < n; i++) {
+                    int im = (i == 0) ? n - 1 : i - 1;
+                    O[v][i] = .5 * (U[v][i] + Q[v][i] - lambda * (FL[v][i] - FL[v][im]));
+                }
+            for (v = 0; v < 5; v++) {
+                double *dst = q + (long)slab_for(dir, v) * g->pitch + base;
+                for (i = 0; i < n; i++) dst[(long)i * li.stride] = O[v][i];
+            }
+        }
+        free(buf);
+    }
+    return failed ? ORC_ENOMEM : ORC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Boundary conditions: gpuclass/cudaStatics.cu:112-286, 1086-1532           */
+/* ------------------------------------------------------------------------ */
+
+/* Applies one array's BC on one face. memdir = 1,2,3; sas: 0 symmetric,
+ * 1 antisymmetric, 2 constant, 3 linear (x only). Mirrors callBCKernel's
+ * kernel table (cudaStatics.cu:1169-1240) including the per-direction index
+ * conventions of each kernel. */
+static int array_face_bc(double *phi, const orc_grid *g, int side, int memdir, int sas, int d)
+{
+    long nx = g->nx, ny = g->ny, nz = g->nz;
+    long i, j, k;
+    int q;
+    long n = (memdir == 1) ? nx : (memdir == 2 ? ny : nz);
+    if ((sas == 0 || sas == 1) && n < 2 * d + 2) return ORC_EINVAL;
+    if (n < d + 2) return ORC_EINVAL;
+    /* cukern_extrapolateLinearBdyXPlus (cudaStatics.cu:1406-1415) writes one
+     * cell past the end of each row and is marked "not done" in the reference
+     * (:1180); linear extrapolation exists only on the -x face here. */
+    if (sas == 3 && (memdir != 1 || side == 1)) return ORC_ENOIMPL;
+    if (memdir == 1) {
+        for (k = 0; k < nz; k++)
+            for (j = 0; j < ny; j++) {
+                double *r = phi + j * nx + k * nx * ny;
+                if (side == 0) {
+                    if (sas == 0) for (q = 0; q < d; q++) r[d - 1 - q] = r[d + 1 + q];
+                    else if (sas == 1) for (q = 0; q < d; q++) r[d - 1 - q] = -r[d + 1 + q];
+                    else if (sas == 2) for (q = 0; q < d; q++) r[q] = r[d];
+                    else {
+                        /* cukern_extrapolateLinearBdyXMinus (cudaStatics.cu:1372-1380) */
+                        double f0 = r[d], f1 = r[d + 1];
+                        for (q = 0; q < d; q++) r[q] = r[d] + (d - q) * (f0 - f1);
+                    }
+                } else {
+                    if (sas == 0) for (q = 0; q < d; q++) r[nx - d + q] = r[nx - d - 2 - q];
+                    else if (sas == 1) for (q = 0; q < d; q++) r[nx - d + q] = -r[nx - d - 2 - q];
+                    else for (q = 0; q < d; q++) r[nx - d + q] = r[nx - d - 1];
+                }
+            }
+        return ORC_OK;
+    }
+    if (sas == 3) return ORC_ENOIMPL; /* callBCKernel cases 19/27/35/43 */
+    if (memdir == 2) {
+        for (k = 0; k < nz; k++)
+            for (i = 0; i < nx; i++) {
+                double *c = phi + i + k * nx * ny;
+                if (side == 0) {
+                    if (sas == 0) for (q = 0; q < d; q++) c[nx * q] = c[nx * (2 * d - q)];
+                    else if (sas == 1) for (q = 0; q < d; q++) c[nx * q] = -c[nx * (2 * d - q)];
+                    else { double f = c[d * nx]; for (q = 0; q < d; q++) c[q * nx] = f; }
+                } else {
+                    if (sas == 0) for (q = 0; q < d; q++) c[nx * (ny - 1 - q)] = c[nx * (ny - 1 - 2 * d + q)];
+                    else if (sas == 1) for (q = 0; q < d; q++) c[nx * (ny - 1 - q)] = -c[nx * (ny - 1 - 2 * d + q)];
+                    else { /* copies the innermost ghost, not the last interior cell (cudaStatics.cu:1461-1466) */
+                        double f = c[(ny - d) * nx];
+                        for (q = 0; q < d; q++) c[(ny - d + q) * nx] = f;
+                    }
+                }
+            }
+        return ORC_OK;
+    }
+    /* memdir 3 */
+    {
+        long st = nx * ny;
+        for (j = 0; j < ny; j++)
+            for (i = 0; i < nx; i++) {
+                double *c = phi + i + nx * j;
+                if (side == 0) {
+                    /* z-minus mirror reflects about 4.5 (2*depth+MIRROR_OFFSET-q, :1496-1500) */
+                    if (sas == 0) for (q = 0; q < d; q++) c[q * st] = c[(2 * d + 1 - q) * st];
+                    else if (sas == 1) for (q = 0; q < d; q++) c[q * st] = -c[(2 * d + 1 - q) * st];
+                    else for (q = 0; q < d; q++) c[st * q] = c[st * d];
+                } else {
+                    if (sas == 0) for (q = 0; q < d; q++) c[st * (nz - 1 - q)] = c[st * (nz - 1 - 2 * d + q)];
+                    else if (sas == 1) for (q = 0; q < d; q++) c[st * (nz - 1 - q)] = -c[st * (nz - 1 - 2 * d + q)];
+                    else for (q = 0; q < d; q++) c[st * (nz - 1 - q)] = c[st * (nz - d)];
+                }
+            }
+    }
+    return ORC_OK;
+}
+
+int orc_set_fluid_boundary(double *q, const orc_grid *g, int dir, const int *modes, int depth)
+{
+    if (dir < 1 || dir > 3) return ORC_EINVAL;
+    int side, s;
+    for (side = 0; side < 2; side++) {
+        int bt = modes[2 * (dir - 1) + side];
+        if (bt == ORC_BC_OUTFLOW || bt == ORC_BC_FREEBALANCE || bt == ORC_BC_WALL) return ORC_ENOIMPL;
+        if (bt == ORC_BC_CIRCULAR || bt == ORC_BC_STATIONARY) continue;
+        for (s = 0; s < 5; s++) {
+            int sas;
+            int veccomp = (s < 2) ? 0 : s - 1; /* px,py,pz carry vectorComponent 1,2,3 */
+            if (bt == ORC_BC_MIRROR) sas = (veccomp == dir);
+            else if (bt == ORC_BC_CONST) sas = 2;
+            else sas = 3;
+            int rc = array_face_bc(q + (long)s * g->pitch, g, side, dir, sas, depth);
+            if (rc) return rc;
+        }
+    }
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* fluidstep: gpuclass/flux.cu:27-155                                        */
+/* ------------------------------------------------------------------------ */
+
+/* fluxcall table, flux.cu:46 */
+static const int fluxcall[3][6] = {{1, 2, 1, 2, 3, 3}, {3, 1, 2, 3, 1, 2}, {2, 3, 3, 1, 2, 1}};
+
+void orc_sweep_order(int sweep, int *dirs3)
+{
+    int n;
+    for (n = 0; n < 3; n++) dirs3[n] = fluxcall[n][((sweep % 6) + 6) % 6];
+}
+
+int orc_fluidstep(double *q, const orc_grid *g, const double *h, double dt, int stepDirection,
+                  int stepNumber, int method, const orc_fluid_params *p, const int *modes, int depth)
+{
+    int rc;
+    if (g->ny == 1 && g->nz == 1) {
+        rc = orc_fluid_update_1d(q, g, 1, method, dt / h[0], p);
+        if (rc) return rc;
+        return orc_set_fluid_boundary(q, g, 1, modes, depth);
+    }
+    int sweep = (stepNumber + 3 * (stepDirection > 0)) % 6;
+    int dims[3] = {g->nx, g->ny, g->nz};
+    int n;
+    for (n = 0; n < 3; n++) {
+        int d = fluxcall[n][sweep];
+        if (dims[d - 1] > 3) {
+            rc = orc_fluid_update_1d(q, g, d, method, dt / h[d - 1], p);
+            if (rc) return rc;
+            rc = orc_set_fluid_boundary(q, g, d, modes, depth);
+            if (rc) return rc;
+        }
+    }
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Sound speed and CFL                                                       */
+/* ------------------------------------------------------------------------ */
+
+void orc_soundspeed(const double *q, const orc_grid *g, double gamma, double *cs)
+{
+    long n = (long)g->nx * g->ny * g->nz, x;
+    double gg1 = gamma * (gamma - 1);
+    const double *rho = q, *E = q + g->pitch, *px = q + 2 * g->pitch, *py = q + 3 * g->pitch,
+                 *pz = q + 4 * g->pitch;
+#pragma omp parallel for schedule(static)
+    for (x = 0; x < n; x++) {
+        double rhoinv = 1 / rho[x];
+        double csq = gg1 * (E[x] - .5 * (px[x] * px[x] + py[x] * py[x] + pz[x] * pz[x]) * rhoinv) * rhoinv;
+        if (csq < 0.0) csq = 0.0;
+        cs[x] = sqrt(csq);
+    }
+}
+
+double orc_cfl_timestep(const double *q, const double *cs, const orc_grid *g, const double *h,
+                        int method, const int *globalRez)
+{
+    long n = (long)g->nx * g->ny * g->nz, x;
+    int spacedim = 0;
+    if (globalRez[1] > 1) spacedim = 1;
+    if (globalRez[2] > 1) spacedim = 2;
+    int dim = spacedim + 1;
+    const double *rho = q, *px = q + 2 * g->pitch, *py = q + 3 * g->pitch, *pz = q + 4 * g->pitch;
+    double tmin = 1e38;
+    /* the XJ 1D table entry is an HLLC-formula instance (cflTimestep.cu:183); same value */
+    int xj = (method == ORC_XINJIN) && dim > 1;
+    double hmin = h[0];
+    if (dim > 1 && h[1] < hmin) hmin = h[1];
+    if (dim > 2 && h[2] < hmin) hmin = h[2];
+#pragma omp parallel for reduction(min : tmin) schedule(static)
+    for (x = 0; x < n; x++) {
+        double u;
+        if (!xj) {
+            double pm;
+            if (dim == 1) pm = fabs(px[x]);
+            else {
+                double pp = px[x] * px[x];
+                pp = pp + py[x] * py[x];
+                if (dim > 2) pp = pp + pz[x] * pz[x];
+                pm = sqrt(pp);
+            }
+            u = pm / rho[x] + cs[x];
+            u = hmin / u;
+        } else {
+            double r = rho[x], v;
+            u = h[0] / (fabs(px[x]) / r + cs[x]);
+            v = h[1] / (fabs(py[x]) / r + cs[x]);
+            u = (u < v) ? u : v;
+            if (dim > 2) {
+                v = h[2] / (fabs(pz[x]) / r + cs[x]);
+                u = (u < v) ? u : v;
+            }
+        }
+        if (u < tmin) tmin = u;
+    }
+    if (tmin > 1e37) tmin = 1e37; /* per-block initial value, cflTimestep.cu:290 */
+    if (method == ORC_HLL || method == ORC_HLLC) tmin = tmin / 2;
+    return tmin;
+}

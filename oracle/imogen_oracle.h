@@ -1,0 +1,88 @@
+/*
+ * imogen_oracle.h -- CPU restatement (FP64, no FMA contraction) of the
+ * Imogen fluid-step arithmetic.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker for the HIP product
+ * path: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * may load it. The product library (libimogen_hip.so) never links it and has
+ * no CPU fallback.
+ *
+ * Array convention (identical to the reference's 5-slab fluid canister,
+ * save/uploadDataArrays.m:93-111): one allocation of 5 slabs
+ *   slab 0 = rho, 1 = E (total energy), 2 = px, 3 = py, 4 = pz
+ * each slab nx*ny*nz doubles with x fastest, slabs `pitch` doubles apart.
+ */
+#ifndef IMOGEN_ORACLE_H
+#define IMOGEN_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Slope limiter selector (gpuclass/limiter_functions.h). 0 is the
+ * reference's compiled-in default SLOPEFUNC = slopeLimiter_Ospre
+ * (gpuclass/cudaFluidStep.cu:132). */
+enum orc_limiter { ORC_LIM_OSPRE = 0, ORC_LIM_VANLEER = 1, ORC_LIM_MINMOD = 2,
+                   ORC_LIM_SUPERBEE = 3, ORC_LIM_VANALBADA = 4, ORC_LIM_MC = 5,
+                   ORC_LIM_ZERO = 6 };
+
+/* Fluid methods, gpuclass/cudaFluidStep.h:6-8 */
+enum orc_method { ORC_HLL = 1, ORC_HLLC = 2, ORC_XINJIN = 3 };
+
+/* BC modes, gpuclass/cudaCommon.h:49 */
+enum orc_bcmode { ORC_BC_CIRCULAR = 0, ORC_BC_MIRROR = 1, ORC_BC_WALL = 2, ORC_BC_STATIONARY = 3,
+                  ORC_BC_CONST = 4, ORC_BC_LINEAR = 5, ORC_BC_OUTFLOW = 6, ORC_BC_FREEBALANCE = 7 };
+
+typedef struct {
+    double gamma;     /* thermoGamma (already forced to 2 by the caller for isothermal fluids) */
+    double rhomin;    /* fluid MINMASS */
+    double ciso;      /* isothermal sound speed, -1 = adiabatic */
+    int limiter;      /* enum orc_limiter */
+} orc_fluid_params;
+
+typedef struct {
+    int nx, ny, nz;
+    long pitch;       /* doubles between slabs */
+} orc_grid;
+
+/* Thread count for the OpenMP loops over lines (<=0: leave default). */
+void orc_set_threads(int n);
+int  orc_get_threads(void);
+
+/* One directional SSP-RK2 update (predictor + corrector) of all lines along
+ * physical direction dir (1=x, 2=y, 3=z), periodic wrap along the line.
+ * lambda = dt/h[dir]. Follows performFluidUpdate_1D (gpuclass/cudaFluidStep.cu:233-663).
+ * Returns 0 or a negative status. */
+int orc_fluid_update_1d(double *q, const orc_grid *g, int dir, int method, double lambda,
+                        const orc_fluid_params *p);
+
+/* Boundary conditions for direction dir on both faces. modes[6] = [-x,+x,-y,+y,-z,+z],
+ * depth = halo depth (4). Follows setFluidBoundary (gpuclass/cudaStatics.cu:112-196). */
+int orc_set_fluid_boundary(double *q, const orc_grid *g, int dir, const int *modes, int depth);
+
+/* One `fluidstep` call: 3 directional sweeps in the order picked by
+ * (stepNumber + 3*(stepDirection>0)) % 6, each followed by its BC.
+ * Follows performFluidUpdate_3D (gpuclass/flux.cu:27-155). */
+int orc_fluidstep(double *q, const orc_grid *g, const double *h, double dt, int stepDirection,
+                  int stepNumber, int method, const orc_fluid_params *p, const int *modes, int depth);
+
+/* Returns the sweep order (3 physical directions) for a sweep index 0..5,
+ * derived from the fluxcall table of gpuclass/flux.cu:46. */
+void orc_sweep_order(int sweep, int *dirs3);
+
+/* Sound speed, gpuclass/cudaSoundspeed.cu:178-195 */
+void orc_soundspeed(const double *q, const orc_grid *g, double gamma, double *cs);
+
+/* CFL time step limit, gpuclass/cflTimestep.cu:107-377 (SQUARE geometry).
+ * globalRez selects the dimensionality as the reference does. Returns tau
+ * (already halved for HLL/HLLC). */
+double orc_cfl_timestep(const double *q, const double *cs, const orc_grid *g, const double *h,
+                        int method, const int *globalRez);
+
+/* Slope limiter evaluation (exported for the limiter tests). */
+double orc_slope_limiter(int which, double a, double b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

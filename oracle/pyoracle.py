@@ -1,0 +1,152 @@
+"""ctypes binding of oracle/liboracle.so (test infrastructure only).
+
+The functions take numpy float64 arrays shaped (5, nz, ny, nx) -- the
+5-slab fluid canister (rho, E, px, py, pz) with x fastest -- and update them
+in place, exactly as the CPU restatement in imogen_oracle.c does.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "liboracle.so")
+
+LIM_OSPRE, LIM_VANLEER, LIM_MINMOD, LIM_SUPERBEE, LIM_VANALBADA, LIM_MC, LIM_ZERO = range(7)
+HLL, HLLC, XINJIN = 1, 2, 3
+BC_CIRCULAR, BC_MIRROR, BC_WALL, BC_STATIONARY, BC_CONST, BC_LINEAR, BC_OUTFLOW, BC_FREEBALANCE = range(8)
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [("gamma", ctypes.c_double), ("rhomin", ctypes.c_double),
+                ("ciso", ctypes.c_double), ("limiter", ctypes.c_int)]
+
+
+class _Grid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("pitch", ctypes.c_long)]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            build()
+        L = ctypes.CDLL(LIBPATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.orc_fluid_update_1d.argtypes = [dp, ctypes.POINTER(_Grid), ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_double, ctypes.POINTER(_Params)]
+        L.orc_fluid_update_1d.restype = ctypes.c_int
+        L.orc_set_fluid_boundary.argtypes = [dp, ctypes.POINTER(_Grid), ctypes.c_int, ip, ctypes.c_int]
+        L.orc_set_fluid_boundary.restype = ctypes.c_int
+        L.orc_fluidstep.argtypes = [dp, ctypes.POINTER(_Grid), dp, ctypes.c_double, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_int, ctypes.POINTER(_Params), ip, ctypes.c_int]
+        L.orc_fluidstep.restype = ctypes.c_int
+        L.orc_sweep_order.argtypes = [ctypes.c_int, ip]
+        L.orc_soundspeed.argtypes = [dp, ctypes.POINTER(_Grid), ctypes.c_double, dp]
+        L.orc_cfl_timestep.argtypes = [dp, dp, ctypes.POINTER(_Grid), dp, ctypes.c_int, ip]
+        L.orc_cfl_timestep.restype = ctypes.c_double
+        L.orc_slope_limiter.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.orc_slope_limiter.restype = ctypes.c_double
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_get_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _grid(q):
+    if q.dtype != np.float64 or not q.flags.c_contiguous or q.ndim != 4 or q.shape[0] != 5:
+        raise ValueError("fluid must be C-contiguous float64 shaped (5, nz, ny, nx)")
+    _, nz, ny, nx = q.shape
+    return _Grid(nx, ny, nz, nx * ny * nz)
+
+
+def _params(gamma, rhomin=1e-8, ciso=-1.0, limiter=LIM_OSPRE):
+    return _Params(gamma, rhomin, ciso, limiter)
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
+
+
+def get_threads():
+    return lib().orc_get_threads()
+
+
+def fluid_update_1d(q, direction, lam, gamma, method=HLLC, limiter=LIM_OSPRE, rhomin=1e-8, ciso=-1.0):
+    g = _grid(q)
+    p = _params(gamma, rhomin, ciso, limiter)
+    rc = lib().orc_fluid_update_1d(_dptr(q), ctypes.byref(g), direction, method, lam, ctypes.byref(p))
+    if rc:
+        raise RuntimeError(f"orc_fluid_update_1d returned {rc}")
+
+
+def _modes(modes):
+    m = np.asarray(modes, dtype=np.int32)
+    if m.shape != (6,):
+        raise ValueError("modes must be 6 ints [-x,+x,-y,+y,-z,+z]")
+    return m
+
+
+def set_fluid_boundary(q, direction, modes, depth=4):
+    g = _grid(q)
+    m = _modes(modes)
+    rc = lib().orc_set_fluid_boundary(_dptr(q), ctypes.byref(g), direction,
+                                      m.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), depth)
+    if rc:
+        raise RuntimeError(f"orc_set_fluid_boundary returned {rc}")
+
+
+def fluidstep(q, h, dt, step_direction, step_number, gamma, method=HLLC, limiter=LIM_OSPRE,
+              modes=(BC_CIRCULAR,) * 6, depth=4, rhomin=1e-8, ciso=-1.0):
+    g = _grid(q)
+    p = _params(gamma, rhomin, ciso, limiter)
+    hh = np.ascontiguousarray(h, dtype=np.float64)
+    m = _modes(modes)
+    rc = lib().orc_fluidstep(_dptr(q), ctypes.byref(g), _dptr(hh), dt, step_direction, step_number,
+                             method, ctypes.byref(p), m.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), depth)
+    if rc:
+        raise RuntimeError(f"orc_fluidstep returned {rc}")
+
+
+def sweep_order(sweep):
+    out = (ctypes.c_int * 3)()
+    lib().orc_sweep_order(sweep, out)
+    return tuple(out)
+
+
+def soundspeed(q, gamma):
+    g = _grid(q)
+    cs = np.empty(q.shape[1:], dtype=np.float64)
+    lib().orc_soundspeed(_dptr(q), ctypes.byref(g), gamma, _dptr(cs))
+    return cs
+
+
+def cfl_timestep(q, cs, h, method, global_rez=None):
+    g = _grid(q)
+    hh = np.ascontiguousarray(h, dtype=np.float64)
+    if global_rez is None:
+        global_rez = (q.shape[3], q.shape[2], q.shape[1])
+    gr = np.asarray(global_rez, dtype=np.int32)
+    cs = np.ascontiguousarray(cs, dtype=np.float64)
+    return lib().orc_cfl_timestep(_dptr(q), _dptr(cs), ctypes.byref(g), _dptr(hh), method,
+                                  gr.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+
+
+def slope_limiter(which, a, b):
+    return lib().orc_slope_limiter(which, a, b)

@@ -1,0 +1,103 @@
+"""CPU checks of the C ABI: the library loads, exports every function declared
+in include/imogen_abi.h, the ctypes mirror matches the header's struct layout
+(compiled probe), and tag serialisation round-trips (no device calls)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from gpuimogen_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(abi.HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\*?\s*\b([A-Za-z_][A-Za-z0-9_]*)\s*\(",
+                       text, flags=re.M)
+    return sorted(set(n for n in names if n not in ("if", "for", "while", "defined")))
+
+
+def test_library_exports_every_declared_symbol():
+    L = abi.lib()
+    names = declared_functions()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    # and the ctypes signature table covers them all
+    assert set(names) <= set(abi.SIGNATURES), set(names) - set(abi.SIGNATURES)
+
+
+def test_struct_layout_matches_header(tmp_path):
+    fields = {name: [f[0] for f in cls._fields_] for name, cls in abi.STRUCTS.items()}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{abi.HEADER}"', "int main(void){"]
+    for name, fl in fields.items():
+        lines.append(f'printf("{name} size %zu\\n", sizeof({name}));')
+        for f in fl:
+            lines.append(f'printf("{name}.{f} %zu\\n", offsetof({name}, {f}));')
+    lines.append("return 0;}")
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-std=c11", "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = dict(line.rsplit(" ", 1) for line in out if line)
+    for name, cls in abi.STRUCTS.items():
+        assert int(got[f"{name} size"]) == ctypes.sizeof(cls), name
+        for f in fields[name]:
+            assert int(got[f"{name}.{f}"]) == getattr(cls, f).offset, f"{name}.{f}"
+
+
+def test_tag_round_trip_and_partition_extent():
+    L = abi.lib()
+    tag = np.zeros(abi.TAG_LEN_1GPU, dtype=np.int64)
+    tag[:11] = [40, 30, 20, 5, 4, 1, 1, 0, 1, 0, 2]
+    tag[11:] = [0, 0x7F0000001000]
+    m = abi.MGArray()
+    assert L.deserializeTagToMGArray(tag.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), ctypes.byref(m)) == 0
+    assert tuple(m.dim) == (40, 30, 20) and m.numel == 24000 and m.numSlabs == 5
+    assert m.slabPitch[0] == ((24000 * 8 + 255) // 256) * 256
+    assert m.vectorComponent == 2 and tuple(m.currentPermutation) == (1, 2, 3)
+    assert m.devicePtr[0] == 0x7F0000001000
+    back = np.zeros_like(tag)
+    L.serializeMGArrayToTag(ctypes.byref(m), back.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+    assert np.array_equal(back, tag)
+    sub = (ctypes.c_int * 6)()
+    L.calcPartitionExtent(ctypes.byref(m), 0, sub)
+    assert tuple(sub) == (0, 0, 0, 40, 30, 20)
+
+
+def test_permtag_maps_are_inverse():
+    L = abi.lib()
+    for t in range(1, 7):
+        p = (ctypes.c_int * 3)()
+        L.MGA_permtagToNums(t, p)
+        assert sorted(p) == [1, 2, 3]
+        assert L.MGA_numsToPermtag(p) == t
+        for d in (1, 2, 3):
+            assert p[L.MGA_dir2memdir(p, d) - 1] == d
+
+
+@pytest.mark.parametrize("bad", [[0, 1, 1], [4, -1, 1]])
+def test_malformed_tag_is_rejected(bad):
+    L = abi.lib()
+    tag = np.zeros(abi.TAG_LEN_1GPU, dtype=np.int64)
+    tag[:3] = bad
+    tag[6] = 1
+    m = abi.MGArray()
+    rc = L.deserializeTagToMGArray(tag.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), ctypes.byref(m))
+    assert rc == abi.ERROR_DESERIALIZE_GPUTAG_FAILED
+
+
+def test_too_many_gpus_rejected():
+    L = abi.lib()
+    tag = np.zeros(abi.GPU_TAG_LENGTH + 2 * 5, dtype=np.int64)
+    tag[:3] = [4, 4, 4]
+    tag[6] = 5
+    m = abi.MGArray()
+    rc = L.deserializeTagToMGArray(tag.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), ctypes.byref(m))
+    assert rc == abi.ERROR_DESERIALIZE_GPUTAG_FAILED
