@@ -129,7 +129,7 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--cfl", type=float, default=0.4)
     ap.add_argument("--limiter", choices=sorted(LIMITERS), default="vanleer")
-    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--cpu-iters", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", action="store_true", help="verify the final state is finite and positive")
     args = ap.parse_args()

@@ -5,13 +5,6 @@ this package is the host-side mirror of the reference's MATLAB call forms
 (gpuclass.py) and the ctypes binding (abi.py). There is no CPU fallback: if
 the library is missing or the GPU is absent, calls raise.
 """
-from . import abi  # noqa: F401
+from . import abi, gpuclass, problems  # noqa: F401
 
-__all__ = ["abi", "gpuclass"]
-
-
-def __getattr__(name):
-    if name == "gpuclass":
-        from . import gpuclass
-        return gpuclass
-    raise AttributeError(name)
+__all__ = ["abi", "gpuclass", "problems"]
