@@ -113,6 +113,26 @@ def run_cpu_baseline(args, q0, h):
                       f"{el:.2f} s"}
 
 
+def copy_probe(reps=10, ndoubles=1 << 27):
+    """Copies 1 GiB `reps` times with the library's copy kernel: the known-size
+    reference that calibrates the FETCH_SIZE / WRITE_SIZE counters."""
+    from gpuimogen_amd import abi, gpuclass as gc
+
+    a = gc.GPU_Type(dims=(ndoubles, 1, 1))
+    b = gc.GPU_Type(dims=(ndoubles, 1, 1))
+    gc.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        abi.call("GPU_copy", b._tp(), a._tp())
+    gc.synchronize()
+    el = time.perf_counter() - t0
+    nbytes = ndoubles * 8
+    print(json.dumps({"copy_probe": True, "bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes,
+                      "launches": reps, "GBs": 2 * nbytes * reps / el / 1e9}), flush=True)
+    a.free()
+    b.free()
+
+
 def load_traffic():
     p = os.path.join(ROOT, "profiles", "traffic_hllc_sweep_sedov256.json")
     if not os.path.exists(p):
@@ -132,7 +152,11 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", action="store_true", help="verify the final state is finite and positive")
+    ap.add_argument("--copy-probe", action="store_true", help="only run the 1 GiB copy-kernel calibration")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build has no "

@@ -98,6 +98,32 @@ void prof_end(int slot, hipStream_t s)
     if (hipEventRecord(g_prof[slot].b, s) != hipSuccess) g_prof[slot].kernel = -1;
 }
 
+// Plain device copy (16 B per lane per iteration, grid-stride). Used by the
+// GPU_copy / GPU_clone / GPU_makeslab primitives, and as the known-size kernel
+// that calibrates the HBM byte counters.
+__global__ void copy_kernel(double2 *__restrict__ dst, const double2 *__restrict__ src, int64_t n2,
+                            double *__restrict__ dtail, const double *__restrict__ stail, int tail)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride) dst[i] = src[i];
+    if (blockIdx.x == 0 && threadIdx.x < tail) dtail[threadIdx.x] = stail[threadIdx.x];
+}
+
+int device_copy(double *dst, const double *src, int64_t ndoubles, hipStream_t s)
+{
+    if (ndoubles <= 0) return SUCCESSFUL;
+    if (((uintptr_t)dst | (uintptr_t)src) & 15)
+        return IM_HIP(hipMemcpyAsync(dst, src, (size_t)ndoubles * sizeof(double), hipMemcpyDeviceToDevice, s));
+    const int64_t n2 = ndoubles / 2;
+    const int tail = (int)(ndoubles - 2 * n2);
+    int64_t blocks = (n2 + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(256), 0, s, (double2 *)dst, (const double2 *)src,
+                       n2, dst + 2 * n2, src + 2 * n2, tail);
+    return IM_LAUNCHED("copy_kernel");
+}
+
 static int64_t slab_pitch_bytes(int64_t numel)
 {
     int64_t b = numel * (int64_t)sizeof(double);
@@ -448,8 +474,7 @@ int GPU_makeslab(const int64_t *tag, int N, int64_t *newTag)
     rc = MGA_allocSlab(&m, &nu, N);
     if (rc) return rc;
     DeviceState *st = state();
-    rc = IM_HIP(hipMemcpyAsync(nu.devicePtr[0], m.devicePtr[0], (size_t)m.partNumel[0] * sizeof(double),
-                               hipMemcpyDeviceToDevice, st->stream));
+    rc = device_copy(nu.devicePtr[0], m.devicePtr[0], m.partNumel[0], st->stream);
     if (rc) return rc;
     if (m.numSlabs > 0) {
         rc = MGA_delete(&m);
@@ -492,8 +517,7 @@ int GPU_clone(const int64_t *tag, int64_t *newTag)
     rc = MGA_allocSlab(&m, &nu, 1);
     if (rc) return rc;
     DeviceState *st = state();
-    rc = IM_HIP(hipMemcpyAsync(nu.devicePtr[0], m.devicePtr[0], (size_t)m.partNumel[0] * sizeof(double),
-                               hipMemcpyDeviceToDevice, st->stream));
+    rc = device_copy(nu.devicePtr[0], m.devicePtr[0], m.partNumel[0], st->stream);
     if (rc) return rc;
     rc = IM_HIP(hipStreamSynchronize(st->stream));
     if (rc) return rc;
@@ -510,8 +534,7 @@ int GPU_copy(const int64_t *dstTag, const int64_t *srcTag)
     if (rc) return rc;
     if (d.numel != s.numel) return IM_FAIL(ERROR_INVALID_ARGS, "GPU_copy: shapes differ");
     DeviceState *st = state();
-    rc = IM_HIP(hipMemcpyAsync(d.devicePtr[0], s.devicePtr[0], (size_t)s.partNumel[0] * sizeof(double),
-                               hipMemcpyDeviceToDevice, st->stream));
+    rc = device_copy(d.devicePtr[0], s.devicePtr[0], s.partNumel[0], st->stream);
     if (rc) return rc;
     return IM_HIP(hipStreamSynchronize(st->stream));
 }
