@@ -141,7 +141,7 @@ __device__ __forceinline__ void x_fluxes(const HydroConsts &hc, int rt, const V5
     Fm = shfl5(Flast, srcL);
 }
 
-template <int K, int LIM>
+template <int K, int LIM, bool FULL>
 __global__ __launch_bounds__(256) void hllc_sweep_x(double *__restrict__ q, int64_t pitch, int N,
                                                     int64_t nlines, int nl, double lambda,
                                                     HydroConsts hc, int rt)
@@ -172,13 +172,15 @@ __global__ __launch_bounds__(256) void hllc_sweep_x(double *__restrict__ q, int6
 
     // stage A: U1 = Q - lambda dF(Q)
     V5 F[K], Fm;
-    x_fluxes<K, LIM>(hc, rt, Q, cnt, srcL, srcR, F, Fm);
+    // FULL: every line is exactly nl*K cells, so no lane holds a partial block
+    const int cntm = FULL ? K : cnt;
+    x_fluxes<K, LIM>(hc, rt, Q, cntm, srcL, srcR, F, Fm);
     V5 U1[K];
 #pragma unroll
     for (int k = 0; k < K; k++) U1[k] = axpy_update(Q[k], lambda, F[k], k == 0 ? Fm : F[k > 0 ? k - 1 : 0]);
 
     // stage B: Q = .5((U1 + Q) - lambda dF(U1))
-    x_fluxes<K, LIM>(hc, rt, U1, cnt, srcL, srcR, F, Fm);
+    x_fluxes<K, LIM>(hc, rt, U1, cntm, srcL, srcR, F, Fm);
 #pragma unroll
     for (int k = 0; k < K; k++) {
         if (k < cnt) {
@@ -224,71 +226,76 @@ __global__ __launch_bounds__(256) void hllc_sweep_march(double *__restrict__ q, 
 
     const V5 one = V5{1.0, 1.0, 0.0, 0.0, 0.0};
     const V5 zero = V5{0.0, 0.0, 0.0, 0.0, 0.0};
-    V5 Qm2 = one, Qm1 = one;
-    V5 wprev = one, dprev = zero, Rprev = one, FAprev = zero;
-    V5 Sm3 = one, Sm4 = one;
-    V5 w1prev = one, d1prev = zero, R1prev = one, FBprev = zero;
+    // Every window variable lives in two slots that trade roles on even and
+    // odd steps (prev <-> new), so the loop carries no register moves.
+    V5 QA = one, QB = one;  // Q_{m-2} in, Q_m out
+    V5 SA = one, SB = one;  // (U1+Q)_{m-4} in, (U1+Q)_{m-2} out
+    V5 wA = one, wB = one, dA = zero, dB = zero, RA = one, RB = one, FAA = zero, FAB = zero;
+    V5 w1A = one, w1B = one, d1A = zero, d1B = zero, R1A = one, R1B = one, FBA = zero, FBB = zero;
 
-    const int maxsteps = seglen + 8;
-    V5 Qnext = one;
-    if (steps > 0) Qnext = load_cell<DIR>(col + (int64_t)mw * sstride, pitch);
-
-    for (int t = 0; t < maxsteps; t++) {
-        const V5 Qcur = Qnext;
-        mw = (mw + 1 == N) ? 0 : mw + 1;
-        if (t + 1 < steps) Qnext = load_cell<DIR>(col + (int64_t)mw * sstride, pitch);
-
+    auto step = [&](int t, const V5 &Qcur, V5 &Qs, V5 &Ss, const V5 &wp, V5 &wn, const V5 &dp, V5 &dn,
+                    const V5 &Rp, V5 &Rn, const V5 &FAp, V5 &FAn, const V5 &w1p, V5 &w1n, const V5 &d1p,
+                    V5 &d1n, const V5 &R1p, V5 &R1n, const V5 &FBp, V5 &FBn) {
         // stage A at cell m-2
-        const V5 w = to_prim(Qcur);
-        const V5 d = w - wprev;
-        V5 Lc, Rc;
-        faces<LIM>(hc, wprev, dprev, d, rt, Lc, Rc);  // faces of cell m-1
-        const V5 FA = hllc_flux(hc, Rprev, Lc);        // interface (m-2)+1/2
-        const V5 U1 = axpy_update(Qm2, lambda, FA, FAprev);
-        const V5 S2 = add5(U1, Qm2);
+        wn = to_prim(Qcur);
+        dn = wn - wp;
+        V5 Lc;
+        faces<LIM>(hc, wp, dp, dn, rt, Lc, Rn);  // faces of cell m-1
+        FAn = hllc_flux(hc, Rp, Lc);              // interface (m-2)+1/2
+        const V5 U1 = axpy_update(Qs, lambda, FAn, FAp);
+        const V5 S2 = add5(U1, Qs);
+        Qs = Qcur;
 
         // stage B at cell m-4
-        const V5 w1 = to_prim(U1);
-        const V5 d1 = w1 - w1prev;
-        V5 L1c, R1c;
-        faces<LIM>(hc, w1prev, d1prev, d1, rt, L1c, R1c);  // faces of cell m-3
-        const V5 FB = hllc_flux(hc, R1prev, L1c);           // interface (m-4)+1/2
-        const V5 o = avg_update(Sm4, lambda, FB, FBprev);   // cell m-4
+        w1n = to_prim(U1);
+        d1n = w1n - w1p;
+        V5 L1c;
+        faces<LIM>(hc, w1p, d1p, d1n, rt, L1c, R1n);  // faces of cell m-3
+        FBn = hllc_flux(hc, R1p, L1c);                 // interface (m-4)+1/2
+        const V5 o = avg_update(Ss, lambda, FBn, FBp); // cell m-4
+        Ss = S2;
 
+        // Every step stores exactly 5 doubles: to the cell (outputs 4..L-1), to
+        // a deferred slot (outputs 0..3) or to this lane's dummy slot (no
+        // output yet / past the segment). With a fixed store count per step the
+        // compiler can wait for a prefetched load without waiting for stores
+        // (stores share the vector-memory counter on gfx9).
         const int oi = t - 8;  // output index within the segment
-        if (oi >= 0 && oi < L) {
-            if (oi < 4) {
-                double *dp = defer + gtid;
-                dp[(oi * 5 + 0) * nthr] = o.a;
-                dp[(oi * 5 + 1) * nthr] = o.b;
-                dp[(oi * 5 + 2) * nthr] = o.c;
-                dp[(oi * 5 + 3) * nthr] = o.d;
-                dp[(oi * 5 + 4) * nthr] = o.e;
-            } else {
-                store_cell<DIR>(col + (int64_t)(j0 + oi) * sstride, pitch, o);
-            }
-        }
+        const bool direct = (oi >= 4) && (oi < L);
+        const int slot = (oi >= 0 && oi < 4 && oi < L) ? oi : 4;
+        double *dst = direct ? col + (int64_t)(j0 + oi) * sstride : defer + gtid + (int64_t)slot * 5 * nthr;
+        store_cell<DIR>(dst, direct ? pitch : nthr, o);
+    };
 
-        Qm2 = Qm1;
-        Qm1 = Qcur;
-        wprev = w;
-        dprev = d;
-        Rprev = Rc;
-        FAprev = FA;
-        Sm4 = Sm3;
-        Sm3 = S2;
-        w1prev = w1;
-        d1prev = d1;
-        R1prev = R1c;
-        FBprev = FB;
+    const int maxsteps = seglen + 8;
+    // Prefetch slots for even / odd steps. The loads are unconditional (a lane
+    // past its segment, or of an inactive column, reads a valid cell it then
+    // ignores: mw always wraps into [0, N) and `col` of an inactive column is
+    // element 0), so the compiler neither copies slots nor waits for a load it
+    // has just issued.
+    V5 N0 = load_cell<DIR>(col + (int64_t)mw * sstride, pitch);
+    V5 N1;
+    // Enter the loop with the same vector-memory counter pattern as the
+    // back-edge (load group, then 5 stores) so the wait at the loop head can
+    // leave stores outstanding: the stores write the loaded cell into this
+    // lane's dummy slot, which also pins them after the load.
+    store_cell<DIR>(defer + gtid + (int64_t)4 * 5 * nthr, nthr, N0);
+
+    for (int t = 0; t < maxsteps; t += 2) {
+        mw = (mw + 1 == N) ? 0 : mw + 1;
+        N1 = load_cell<DIR>(col + (int64_t)mw * sstride, pitch);
+        step(t, N0, QA, SA, wA, wB, dA, dB, RA, RB, FAA, FAB, w1A, w1B, d1A, d1B, R1A, R1B, FBA, FBB);
+        if (t + 1 < maxsteps) {
+            mw = (mw + 1 == N) ? 0 : mw + 1;
+            N0 = load_cell<DIR>(col + (int64_t)mw * sstride, pitch);
+            step(t + 1, N1, QB, SB, wB, wA, dB, dA, RB, RA, FAB, FAA, w1B, w1A, d1B, d1A, R1B, R1A, FBB, FBA);
+        }
     }
 
     // every lane of the wave has finished reading: release the deferred outputs
     const int nd = L < 4 ? L : 4;
     for (int oi = 0; oi < nd; oi++) {
-        const double *dp = defer + gtid;
-        V5 o{dp[(oi * 5 + 0) * nthr], dp[(oi * 5 + 1) * nthr], dp[(oi * 5 + 2) * nthr],
-             dp[(oi * 5 + 3) * nthr], dp[(oi * 5 + 4) * nthr]};
+        const V5 o = load_cell<DIR>(defer + gtid + (int64_t)oi * 5 * nthr, nthr);
         store_cell<DIR>(col + (int64_t)(j0 + oi) * sstride, pitch, o);
     }
 }
@@ -315,10 +322,15 @@ static int launch_x(double *q, int64_t pitch, int N, int64_t nlines, double lamb
     const int64_t waves = (nlines + lpw - 1) / lpw;
     const int64_t blocks = (waves + 3) / 4;
     dim3 grid((unsigned)blocks), block(256);
-#define IM_XK(KK)                                                                                 \
-    case KK:                                                                                      \
-        hipLaunchKernelGGL((hllc_sweep_x<KK, LIM>), grid, block, 0, s, q, pitch, N, nlines, nl,   \
-                           lambda, hc, rt);                                                       \
+    const bool full = (nl * K == N);
+#define IM_XK(KK)                                                                                      \
+    case KK:                                                                                           \
+        if (full)                                                                                      \
+            hipLaunchKernelGGL((hllc_sweep_x<KK, LIM, true>), grid, block, 0, s, q, pitch, N, nlines, nl, \
+                               lambda, hc, rt);                                                        \
+        else                                                                                           \
+            hipLaunchKernelGGL((hllc_sweep_x<KK, LIM, false>), grid, block, 0, s, q, pitch, N, nlines, nl,\
+                               lambda, hc, rt);                                                        \
         break;
     switch (K) {
         IM_XK(1)
@@ -352,7 +364,8 @@ static int launch_march(double *q, int64_t pitch, int nx, int ny, int nz, int di
     const int blocksPerRow = (wavesPerRow + 3) / 4;
     dim3 grid((unsigned)blocksPerRow, (unsigned)nother), block(256);
     const int64_t nthr = (int64_t)blocksPerRow * nother * 256;
-    double *defer = (double *)scratch((size_t)nthr * 20 * sizeof(double));
+    // 4 deferred output slots + 1 dummy slot of 5 doubles per lane
+    double *defer = (double *)scratch((size_t)nthr * 25 * sizeof(double));
     if (!defer) return IM_FAIL(ERROR_NOMEM, "no scratch for deferred outputs");
     if (dir == 2)
         hipLaunchKernelGGL((hllc_sweep_march<2, LIM>), grid, block, 0, s, q, pitch, nx, sstride, ostride,

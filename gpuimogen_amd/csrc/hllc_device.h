@@ -31,6 +31,43 @@ __device__ __forceinline__ V5 operator-(const V5 &x, const V5 &y)
     return V5{x.a - y.a, x.b - y.b, x.c - y.c, x.d - y.d, x.e - y.e};
 }
 
+// ---- fast FP64 reciprocal, quotient and square root ------------------------
+// The IEEE-exact sequences the compiler emits for '/' and sqrt() handle
+// denormal and overflow scaling that the fluid variables never need; these
+// use the hardware estimates (v_rcp_f64, v_rsq_f64) with Newton/Goldschmidt
+// refinement to full double precision (within 1-2 ulp). Arguments are finite,
+// non-zero normals (rcp, quotient) or >= 0 normals (sqrt).
+__device__ __forceinline__ double rcp_nr(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+__device__ __forceinline__ double div_nr(double a, double b)
+{
+    const double r = rcp_nr(b);
+    const double q = a * r;
+    return fma(r, fma(-b, q, a), q);
+}
+
+__device__ __forceinline__ double sqrt_nr(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = 0.5 * y;
+    const double r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    g = fma(d, h, g);
+    return (x == 0.0) ? x : g;  // sqrt(+-0) = +-0 (rsq(0) = inf); x < 0 gives NaN via rsq
+}
+
 // ---- half-slope limiters (limiter_functions.h) ----------------------------
 __device__ __forceinline__ double flux_minmod(double a, double b)
 {
@@ -45,16 +82,14 @@ __device__ __forceinline__ double slope_limit(double a, double b, int rt)
     switch (which) {
     case 0: {  // Ospre: .75 A B (A+B) / (A^2 + AB + B^2)
         double r = a * b;
-        if (r > 0) {
-            double s = a + b;
-            return .75 * r * s / (s * s - r);
-        }
-        return 0.0;
+        double s = a + b;
+        double v = div_nr(.75 * r * s, s * s - r);
+        return (r > 0) ? v : 0.0;
     }
     case 1: {  // van Leer: A B / (A+B)
         double r = a * b;
-        if (r > 0.0) return r / (a + b);
-        return 0.0;
+        double v = div_nr(r, a + b);
+        return (r > 0.0) ? v : 0.0;
     }
     case 2: {  // minmod, returned through float as the reference does (:70)
         if (a * b < 0) return 0.0;
@@ -87,7 +122,7 @@ __device__ __forceinline__ double slope_limit(double a, double b, int rt)
 // cudaFluidStep.cu:1218-1234.
 __device__ __forceinline__ V5 to_prim(const V5 &q)
 {
-    double F = 1.0 / q.a;
+    double F = rcp_nr(q.a);
     double B = q.b - .5 * (q.c * q.c + q.d * q.d + q.e * q.e) * F;
     return V5{q.a, B, q.c * F, q.d * F, q.e * F};
 }
@@ -130,16 +165,19 @@ __device__ __forceinline__ V5 hllc_flux(const HydroConsts &k, const V5 &L, const
     double E = L.e - R.e;
     double dvsq = C * C + D * D + E * E;
 
-    // Einfeldt bounds (cudaFluidStep.cu:1728-1754)
-    double csql = k.gamma * L.b / L.a;
-    double csqr = k.gamma * R.b / R.a;
-    double Fa = sqrt(R.a / L.a);
-    double Fb = 1.0 / (1.0 + Fa);
+    // Einfeldt bounds (cudaFluidStep.cu:1728-1754). The two face densities'
+    // reciprocals serve both sound speeds, the Roe ratio and the star flux.
+    const double iL = rcp_nr(L.a);
+    const double iR = rcp_nr(R.a);
+    double csql = k.gamma * L.b * iL;
+    double csqr = k.gamma * R.b * iR;
+    double Fa = sqrt_nr(R.a * iL);
+    double Fb = rcp_nr(1.0 + Fa);
     double Croe = ((csql + Fa * csqr) + .5 * k.gm1 * dvsq * Fa * Fb) * Fb;
     double Vroe = (L.c + Fa * R.c) * Fb;
-    Croe = Vroe - sqrt(Croe);
-    double sl = L.c - sqrt(csql);
-    double sr = R.c + sqrt(csqr);
+    Croe = Vroe - sqrt_nr(Croe);
+    double sl = L.c - sqrt_nr(csql);
+    double sr = R.c + sqrt_nr(csqr);
     sl = (Croe < sl) ? Croe : sl;
     Croe = 2 * Vroe - Croe;
     sr = (Croe > sr) ? Croe : sr;
@@ -147,13 +185,14 @@ __device__ __forceinline__ V5 hllc_flux(const HydroConsts &k, const V5 &L, const
     // contact speed, Batten et al. eq. 34 (cudaFluidStep.cu:1352-1357)
     double A = L.a * (sl - L.c);
     double B = R.a * (sr - R.c);
-    double sm = (B * R.c - R.b - A * L.c + L.b) / (B - A);
+    double sm = div_nr(B * R.c - R.b - A * L.c + L.b, B - A);
 
     // case selection (cudaFluidStep.cu:1363-1369)
     const bool left = sm > 0;
     const bool star = left ? !(sl > 0) : (sr > 0);
     const V5 S = left ? L : R;
     const double sk = left ? sl : sr;
+    const double iS = left ? iL : iR;
 
     double ksum = S.c * S.c + S.d * S.d + S.e * S.e;  // |v|^2 of the upwind face
     V5 out;
@@ -168,12 +207,12 @@ __device__ __forceinline__ V5 hllc_flux(const HydroConsts &k, const V5 &L, const
         out.e = S.e * m;
     } else {
         // cases 2/3 (cudaFluidStep.cu:1384-1415)
-        double rb = S.a * (sk - S.c) / (sk - sm);  // rho * beta
+        double rb = div_nr(S.a * (sk - S.c), sk - sm);  // rho * beta
         double G = .5 * ksum;
         double m = rb * sm;
         double Cc = sk * (sm - S.c);
         out.a = m;
-        out.b = rb * sm * (k.govergm1 * S.b / S.a + G + Cc);
+        out.b = rb * sm * (k.govergm1 * S.b * iS + G + Cc);
         out.c = S.b + rb * (Cc + sm * S.c);
         out.d = S.d * m;
         out.e = S.e * m;
