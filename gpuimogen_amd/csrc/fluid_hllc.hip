@@ -141,52 +141,88 @@ __device__ __forceinline__ void x_fluxes(const HydroConsts &hc, int rt, const V5
     Fm = shfl5(Flast, srcL);
 }
 
+// Each wave loops over groups of lines (lpw lines per group, groups strided by
+// the number of waves in the grid). While one group is computed, the next
+// group's cells are already being loaded. FULL: every line is exactly nl*K
+// cells and nl divides 64, so every lane holds K cells and all memory
+// operations are unconditional (the compiler can then count them exactly and
+// wait only for the loads it needs); lanes past the last line read a clamped
+// valid line and write into a private dummy slot.
 template <int K, int LIM, bool FULL>
 __global__ __launch_bounds__(256) void hllc_sweep_x(double *__restrict__ q, int64_t pitch, int N,
                                                     int64_t nlines, int nl, double lambda,
-                                                    HydroConsts hc, int rt)
+                                                    HydroConsts hc, int rt, double *__restrict__ dummy)
 {
     const int lane = threadIdx.x & 63;
     const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
     const int lpw = 64 / nl;
     const int sub = lane / nl;
     const int j = lane - sub * nl;
-    const int64_t line = wave * lpw + sub;
-    const bool active = (sub < lpw) && (line < nlines);
+    const bool laneUsed = sub < lpw;
     const int lbase = sub * nl;
-    const int srcL = (sub < lpw) ? lbase + (j == 0 ? nl - 1 : j - 1) : lane;
-    const int srcR = (sub < lpw) ? lbase + (j == nl - 1 ? 0 : j + 1) : lane;
-    const int x0 = j * K;
-    int cnt = N - x0;
-    cnt = cnt < K ? cnt : K;
-    if (!active) cnt = 0;
+    const int srcL = laneUsed ? lbase + (j == 0 ? nl - 1 : j - 1) : lane;
+    const int srcR = laneUsed ? lbase + (j == nl - 1 ? 0 : j + 1) : lane;
+    const int x0 = laneUsed ? j * K : 0;
+    int cntL = N - x0;
+    cntL = cntL < K ? cntL : K;
+    if (!laneUsed) cntL = 0;
+    const int cnt = FULL ? K : cntL;
+    const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nthr = (int64_t)gridDim.x * blockDim.x;
+    double *mydummy = dummy + gtid;
+    const int64_t groupStride = nwaves * lpw;
 
-    double *base = q + (active ? line * (int64_t)N + x0 : 0);
-
-    V5 Q[K];
+    auto line_ptr = [&](int64_t ln) {
+        const int64_t c = ln < nlines ? ln : nlines - 1;
+        return q + c * (int64_t)N + x0;
+    };
+    auto load_group = [&](int64_t ln, V5 (&Qd)[K]) {
+        double *b = line_ptr(ln);
 #pragma unroll
-    for (int k = 0; k < K; k++) {
-        if (k < cnt) Q[k] = load_cell<1>(base + k, pitch);
-        else Q[k] = V5{1.0, 1.0, 0.0, 0.0, 0.0};
-    }
-
-    // stage A: U1 = Q - lambda dF(Q)
-    V5 F[K], Fm;
-    // FULL: every line is exactly nl*K cells, so no lane holds a partial block
-    const int cntm = FULL ? K : cnt;
-    x_fluxes<K, LIM>(hc, rt, Q, cntm, srcL, srcR, F, Fm);
-    V5 U1[K];
-#pragma unroll
-    for (int k = 0; k < K; k++) U1[k] = axpy_update(Q[k], lambda, F[k], k == 0 ? Fm : F[k > 0 ? k - 1 : 0]);
-
-    // stage B: Q = .5((U1 + Q) - lambda dF(U1))
-    x_fluxes<K, LIM>(hc, rt, U1, cntm, srcL, srcR, F, Fm);
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        if (k < cnt) {
-            const V5 o = avg_update(add5(U1[k], Q[k]), lambda, F[k], k == 0 ? Fm : F[k > 0 ? k - 1 : 0]);
-            store_cell<1>(base + k, pitch, o);
+        for (int k = 0; k < K; k++) {
+            if (k < cnt) Qd[k] = load_cell<1>(b + k, pitch);
+            else Qd[k] = V5{1.0, 1.0, 0.0, 0.0, 0.0};
         }
+    };
+
+    int64_t line = wave * lpw + (laneUsed ? sub : 0);
+    int64_t gbase = wave * lpw;  // first line of this wave's group (uniform)
+    if (gbase >= nlines) return;
+    V5 Q[K], Qn[K];
+    load_group(line, Q);
+
+    for (;;) {
+        const bool more = gbase + groupStride < nlines;  // uniform across the wave
+        load_group(line + groupStride, Qn);  // unconditional: clamped to a valid line when !more
+
+        // stage A: U1 = Q - lambda dF(Q)
+        V5 F[K], Fm;
+        x_fluxes<K, LIM>(hc, rt, Q, cnt, srcL, srcR, F, Fm);
+        V5 U1[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            U1[k] = axpy_update(Q[k], lambda, F[k], k == 0 ? Fm : F[k > 0 ? k - 1 : 0]);
+            Q[k] = add5(U1[k], Q[k]);  // Q now holds (U1 + Q)
+        }
+
+        // stage B: Q = .5((U1 + Q) - lambda dF(U1))
+        x_fluxes<K, LIM>(hc, rt, U1, cnt, srcL, srcR, F, Fm);
+        const bool own = laneUsed && line < nlines;
+        double *dst = own ? line_ptr(line) : mydummy;
+        const int64_t st = own ? pitch : nthr;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            if (k < cnt) {
+                const V5 o = avg_update(Q[k], lambda, F[k], k == 0 ? Fm : F[k > 0 ? k - 1 : 0]);
+                store_cell<1>(own ? dst + k : dst, st, o);
+            }
+        }
+        if (!more) break;
+#pragma unroll
+        for (int k = 0; k < K; k++) Q[k] = Qn[k];
+        line += groupStride;
+        gbase += groupStride;
     }
 }
 
@@ -213,7 +249,7 @@ __global__ __launch_bounds__(256) void hllc_sweep_march(double *__restrict__ q, 
     int L = N - j0;
     L = L < seglen ? L : seglen;
     if (!colValid) L = 0;
-    const int steps = L + 8;  // loads j0-4 .. j0+L+3
+    const int steps = L + 9;  // loads j0-4 .. j0+L+3, one more step drains stage B
 
     double *col = q + (colValid ? x + (int64_t)blockIdx.y * ostride : 0);
     const int64_t gtid = ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x;
@@ -228,46 +264,62 @@ __global__ __launch_bounds__(256) void hllc_sweep_march(double *__restrict__ q, 
     const V5 zero = V5{0.0, 0.0, 0.0, 0.0, 0.0};
     // Every window variable lives in two slots that trade roles on even and
     // odd steps (prev <-> new), so the loop carries no register moves.
-    V5 QA = one, QB = one;  // Q_{m-2} in, Q_m out
-    V5 SA = one, SB = one;  // (U1+Q)_{m-4} in, (U1+Q)_{m-2} out
+    V5 QA = one, QB = one;    // Q_{m-2} in, Q_m out
+    V5 U1A = one, U1B = one;  // U1 of the previous step in, this step's out
     V5 wA = one, wB = one, dA = zero, dB = zero, RA = one, RB = one, FAA = zero, FAB = zero;
     V5 w1A = one, w1B = one, d1A = zero, d1B = zero, R1A = one, R1B = one, FBA = zero, FBB = zero;
 
-    auto step = [&](int t, const V5 &Qcur, V5 &Qs, V5 &Ss, const V5 &wp, V5 &wn, const V5 &dp, V5 &dn,
-                    const V5 &Rp, V5 &Rn, const V5 &FAp, V5 &FAn, const V5 &w1p, V5 &w1n, const V5 &d1p,
-                    V5 &d1n, const V5 &R1p, V5 &R1n, const V5 &FBp, V5 &FBn) {
+    // Stage B runs one step behind stage A (it consumes the previous step's
+    // U1), so the two long dependency chains of a step are independent and
+    // interleave. (U1+Q) of a cell is needed three steps after it is formed:
+    // it waits in a 4-slot ring in LDS, private to the lane.
+    __shared__ double sring[4][4][5][64];  // [wave in block][slot][variable][lane]
+    double(*ring)[5][64] = sring[wib];
+
+    auto step = [&](int t, const V5 &Qcur, V5 &Qs, const V5 &U1p, V5 &U1n, const V5 &wp, V5 &wn,
+                    const V5 &dp, V5 &dn, const V5 &Rp, V5 &Rn, const V5 &FAp, V5 &FAn, const V5 &w1p,
+                    V5 &w1n, const V5 &d1p, V5 &d1n, const V5 &R1p, V5 &R1n, const V5 &FBp, V5 &FBn) {
         // stage A at cell m-2
         wn = to_prim(Qcur);
         dn = wn - wp;
         V5 Lc;
         faces<LIM>(hc, wp, dp, dn, rt, Lc, Rn);  // faces of cell m-1
         FAn = hllc_flux(hc, Rp, Lc);              // interface (m-2)+1/2
-        const V5 U1 = axpy_update(Qs, lambda, FAn, FAp);
-        const V5 S2 = add5(U1, Qs);
+        U1n = axpy_update(Qs, lambda, FAn, FAp);
+        {
+            const V5 S2 = add5(U1n, Qs);  // (U1+Q) of cell m-2, used at step m+3
+            double *r = &ring[t & 3][0][lane];
+            r[0] = S2.a;
+            r[64] = S2.b;
+            r[128] = S2.c;
+            r[192] = S2.d;
+            r[256] = S2.e;
+        }
         Qs = Qcur;
 
-        // stage B at cell m-4
-        w1n = to_prim(U1);
+        // stage B at cell m-5, from the previous step's U1 (cell m-3)
+        w1n = to_prim(U1p);
         d1n = w1n - w1p;
         V5 L1c;
-        faces<LIM>(hc, w1p, d1p, d1n, rt, L1c, R1n);  // faces of cell m-3
-        FBn = hllc_flux(hc, R1p, L1c);                 // interface (m-4)+1/2
-        const V5 o = avg_update(Ss, lambda, FBn, FBp); // cell m-4
-        Ss = S2;
+        faces<LIM>(hc, w1p, d1p, d1n, rt, L1c, R1n);  // faces of cell m-4
+        FBn = hllc_flux(hc, R1p, L1c);                 // interface (m-5)+1/2
+        const double *r = &ring[(t + 1) & 3][0][lane];  // slot of step t-3
+        const V5 S5{r[0], r[64], r[128], r[192], r[256]};
+        const V5 o = avg_update(S5, lambda, FBn, FBp);  // cell m-5
 
         // Every step stores exactly 5 doubles: to the cell (outputs 4..L-1), to
         // a deferred slot (outputs 0..3) or to this lane's dummy slot (no
         // output yet / past the segment). With a fixed store count per step the
         // compiler can wait for a prefetched load without waiting for stores
         // (stores share the vector-memory counter on gfx9).
-        const int oi = t - 8;  // output index within the segment
+        const int oi = t - 9;  // output index within the segment
         const bool direct = (oi >= 4) && (oi < L);
         const int slot = (oi >= 0 && oi < 4 && oi < L) ? oi : 4;
         double *dst = direct ? col + (int64_t)(j0 + oi) * sstride : defer + gtid + (int64_t)slot * 5 * nthr;
         store_cell<DIR>(dst, direct ? pitch : nthr, o);
     };
 
-    const int maxsteps = seglen + 8;
+    const int maxsteps = seglen + 9;
     // Prefetch slots for even / odd steps. The loads are unconditional (a lane
     // past its segment, or of an inactive column, reads a valid cell it then
     // ignores: mw always wraps into [0, N) and `col` of an inactive column is
@@ -284,11 +336,12 @@ __global__ __launch_bounds__(256) void hllc_sweep_march(double *__restrict__ q, 
     for (int t = 0; t < maxsteps; t += 2) {
         mw = (mw + 1 == N) ? 0 : mw + 1;
         N1 = load_cell<DIR>(col + (int64_t)mw * sstride, pitch);
-        step(t, N0, QA, SA, wA, wB, dA, dB, RA, RB, FAA, FAB, w1A, w1B, d1A, d1B, R1A, R1B, FBA, FBB);
+        step(t, N0, QA, U1A, U1B, wA, wB, dA, dB, RA, RB, FAA, FAB, w1A, w1B, d1A, d1B, R1A, R1B, FBA, FBB);
         if (t + 1 < maxsteps) {
             mw = (mw + 1 == N) ? 0 : mw + 1;
             N0 = load_cell<DIR>(col + (int64_t)mw * sstride, pitch);
-            step(t + 1, N1, QB, SB, wB, wA, dB, dA, RB, RA, FAB, FAA, w1B, w1A, d1B, d1A, R1B, R1A, FBB, FBA);
+            step(t + 1, N1, QB, U1B, U1A, wB, wA, dB, dA, RB, RA, FAB, FAA, w1B, w1A, d1B, d1A, R1B, R1A, FBB,
+                 FBA);
         }
     }
 
@@ -319,18 +372,23 @@ static int launch_x(double *q, int64_t pitch, int N, int64_t nlines, double lamb
     else return IM_FAIL(ERROR_NOIMPLEMENT, "X sweep supports lines of at most 1024 cells (got %d)", N);
     const int nl = (N + K - 1) / K;
     const int lpw = 64 / nl;
-    const int64_t waves = (nlines + lpw - 1) / lpw;
+    int64_t waves = (nlines + lpw - 1) / lpw;
+    // Resident-wave budget: 2 waves per SIMD on 1024 SIMDs; beyond it a wave
+    // takes several line groups and prefetches the next one.
+    if (waves > 2048) waves = 2048;
     const int64_t blocks = (waves + 3) / 4;
     dim3 grid((unsigned)blocks), block(256);
-    const bool full = (nl * K == N);
+    const bool full = (nl * K == N) && (64 % nl == 0);
+    double *dummy = (double *)scratch((size_t)blocks * 256 * 5 * sizeof(double));
+    if (!dummy) return IM_FAIL(ERROR_NOMEM, "no scratch for the X sweep");
 #define IM_XK(KK)                                                                                      \
     case KK:                                                                                           \
         if (full)                                                                                      \
             hipLaunchKernelGGL((hllc_sweep_x<KK, LIM, true>), grid, block, 0, s, q, pitch, N, nlines, nl, \
-                               lambda, hc, rt);                                                        \
+                               lambda, hc, rt, dummy);                                                 \
         else                                                                                           \
             hipLaunchKernelGGL((hllc_sweep_x<KK, LIM, false>), grid, block, 0, s, q, pitch, N, nlines, nl,\
-                               lambda, hc, rt);                                                        \
+                               lambda, hc, rt, dummy);                                                 \
         break;
     switch (K) {
         IM_XK(1)
