@@ -39,40 +39,85 @@ __device__ __forceinline__ double wave_min(double v)
 
 // dim = simulation dimension (1..3), xinjin selects the per-direction form.
 // fused != 0: sound speed computed in-line from gamma instead of read from cs.
+struct CflArgs {
+    double gg1, hx, hy, hz, hmin;
+    int fused, dim, xinjin;
+};
+
+// time-step limit of one cell (cflTimestep.cu:292-335)
+__device__ __forceinline__ double cfl_cell(const CflArgs &a, double rho, double E, double px, double py,
+                                           double pz, double cIn)
+{
+    const double c = a.fused ? cs_of(rho, E, px, py, pz, a.gg1) : cIn;
+    double u;
+    if (!a.xinjin) {
+        double pm;
+        if (a.dim == 1) {
+            pm = fabs(px);
+        } else {
+            double p = px * px;
+            p = p + py * py;
+            if (a.dim > 2) p = p + pz * pz;
+            pm = sqrt(p);
+        }
+        u = pm / rho + c;
+        u = a.hmin / u;
+    } else {
+        u = a.hx / (fabs(px) / rho + c);
+        double v = a.hy / (fabs(py) / rho + c);
+        u = (u < v) ? u : v;
+        if (a.dim > 2) {
+            v = a.hz / (fabs(pz) / rho + c);
+            u = (u < v) ? u : v;
+        }
+    }
+    return u;
+}
+
+// Two cells per lane per load (16-byte loads) and two such pairs in flight
+// per iteration. Requires an even slab pitch and 16-byte aligned slabs (the
+// 256-byte slab pitch guarantees both).
 __global__ __launch_bounds__(256) void cfl_pass1(const double *__restrict__ q, int64_t pitch, int64_t n,
-                                                 const double *__restrict__ cs, double gg1, int fused, int dim,
-                                                 int xinjin, double hx, double hy, double hz, double hmin,
+                                                 const double *__restrict__ cs, CflArgs a,
                                                  double *__restrict__ blockMin)
 {
     double m = 1e37;
-    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
-        const double rho = q[x];
-        const double px = q[2 * pitch + x];
-        const double py = q[3 * pitch + x];
-        const double pz = q[4 * pitch + x];
-        const double c = fused ? cs_of(rho, q[pitch + x], px, py, pz, gg1) : cs[x];
-        double u;
-        if (!xinjin) {
-            double pm;
-            if (dim == 1) {
-                pm = fabs(px);
-            } else {
-                double p = px * px;
-                p = p + py * py;
-                if (dim > 2) p = p + pz * pz;
-                pm = sqrt(p);
-            }
-            u = pm / rho + c;
-            u = hmin / u;
+    const int64_t nth = (int64_t)gridDim.x * blockDim.x;
+    const int64_t n2 = n / 2;
+    const double2 *q2 = (const double2 *)q;
+    const double2 *c2 = (const double2 *)cs;
+    const int64_t p2 = pitch / 2;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += 2 * nth) {
+        const int64_t i2 = (i + nth < n2) ? i + nth : i;
+        double2 r[2], e[2], x[2], y[2], z[2], c[2];
+        r[0] = q2[i];
+        e[0] = q2[p2 + i];
+        x[0] = q2[2 * p2 + i];
+        y[0] = q2[3 * p2 + i];
+        z[0] = q2[4 * p2 + i];
+        r[1] = q2[i2];
+        e[1] = q2[p2 + i2];
+        x[1] = q2[2 * p2 + i2];
+        y[1] = q2[3 * p2 + i2];
+        z[1] = q2[4 * p2 + i2];
+        if (!a.fused) {
+            c[0] = c2[i];
+            c[1] = c2[i2];
         } else {
-            u = hx / (fabs(px) / rho + c);
-            double v = hy / (fabs(py) / rho + c);
-            u = (u < v) ? u : v;
-            if (dim > 2) {
-                v = hz / (fabs(pz) / rho + c);
-                u = (u < v) ? u : v;
-            }
+            c[0] = c[1] = make_double2(0.0, 0.0);
         }
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            double u = cfl_cell(a, r[h].x, e[h].x, x[h].x, y[h].x, z[h].x, c[h].x);
+            m = (u < m) ? u : m;
+            u = cfl_cell(a, r[h].y, e[h].y, x[h].y, y[h].y, z[h].y, c[h].y);
+            m = (u < m) ? u : m;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) {
+        const int64_t t = n - 1;
+        const double u = cfl_cell(a, q[t], q[pitch + t], q[2 * pitch + t], q[3 * pitch + t], q[4 * pitch + t],
+                                  a.fused ? 0.0 : cs[t]);
         m = (u < m) ? u : m;
     }
     m = wave_min(m);
@@ -119,12 +164,14 @@ int launch_cfl(const double *base, int64_t pitch, int64_t n, const double *cs, d
                int dim, int xinjin, int halve, double hx, double hy, double hz, double hmin, double *devOut,
                hipStream_t s)
 {
-    int64_t blocks = (n + 255) / 256;
+    int64_t blocks = (n / 4 + 255) / 256;
     if (blocks > 2048) blocks = 2048;
     if (blocks < 1) blocks = 1;
+    if ((pitch & 1) || (((uintptr_t)base | (uintptr_t)cs) & 15))
+        return IM_FAIL(ERROR_INVALID_ARGS, "CFL needs 16-byte aligned slabs with an even pitch");
     double *blockMin = devOut + 8;
-    hipLaunchKernelGGL(cfl_pass1, dim3((unsigned)blocks), dim3(256), 0, s, base, pitch, n, cs,
-                       gamma * (gamma - 1), fused, dim, xinjin, hx, hy, hz, hmin, blockMin);
+    CflArgs a{gamma * (gamma - 1), hx, hy, hz, hmin, fused, dim, xinjin};
+    hipLaunchKernelGGL(cfl_pass1, dim3((unsigned)blocks), dim3(256), 0, s, base, pitch, n, cs, a, blockMin);
     int rc = IM_LAUNCHED("cfl_pass1");
     if (rc) return rc;
     hipLaunchKernelGGL(cfl_pass2, dim3(1), dim3(256), 0, s, blockMin, (int)blocks, halve ? 1.0 : 0.0, devOut);

@@ -6,28 +6,33 @@
 //   * z-minus mirror reflects about cell 4.5 (index 2*depth+1-q), x/y about 4;
 //   * +y and +z 'const' copy the innermost ghost cell (n-depth), +x copies the
 //     last interior cell (n-depth-1).
-// One launch handles one face (side) of one direction for all 5 slabs, which
-// preserves the reference's side-0-then-side-1 order.
+// One launch handles both faces of one direction for all 5 slabs when the
+// faces are disjoint; for lines shorter than 2*depth+2 the minus face runs
+// first, then the plus face, the reference's order.
 #include "imogen_internal.h"
 
 namespace imogen {
 
-// sas codes per slab: -1 none, 0 symmetric, 1 antisymmetric, 2 constant, 3 linear
+// sas codes per (slab, side) at [2*slab + side]: -1 none, 0 symmetric,
+// 1 antisymmetric, 2 constant, 3 linear
 struct SasTable {
-    int s[5];
+    int s[10];
 };
 
-__global__ void bc_face_x(double *__restrict__ q, int64_t pitch, int nx, int64_t rows, int side, int d,
-                          SasTable t)
+__global__ void bc_face_x(double *__restrict__ q, int64_t pitch, int nx, int64_t rows, int side0, int nsides,
+                          int d, SasTable t)
 {
-    // one thread per (ghost cell, slab, row)
+    // one thread per (ghost cell, slab, side, row)
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t total = rows * 5 * d;
+    const int64_t total = rows * 5 * d * nsides;
     if (tid >= total) return;
     const int g = (int)(tid % d);
-    const int slab = (int)((tid / d) % 5);
-    const int64_t row = tid / (5 * d);
-    const int sas = t.s[slab];
+    int64_t rest = tid / d;
+    const int slab = (int)(rest % 5);
+    rest /= 5;
+    const int side = side0 + (int)(rest % nsides);
+    const int64_t row = rest / nsides;
+    const int sas = t.s[2 * slab + side];
     if (sas < 0) return;
     double *r = q + slab * pitch + row * nx;
     if (side == 0) {
@@ -48,13 +53,14 @@ __global__ void bc_face_x(double *__restrict__ q, int64_t pitch, int nx, int64_t
 
 // memdir 2 (y) and 3 (z): one thread per (face cell, slab), looping over depth
 __global__ void bc_face_yz(double *__restrict__ q, int64_t pitch, int nx, int64_t n1, int64_t n2,
-                           int64_t stride, int64_t ostride, int N, int zdir, int side, int d, SasTable t)
+                           int64_t stride, int64_t ostride, int N, int zdir, int side0, int d, SasTable t)
 {
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t face = (int64_t)nx * n2;
-    const int slab = (int)blockIdx.y;
+    const int slab = (int)(blockIdx.y % 5);
+    const int side = side0 + (int)(blockIdx.y / 5);
     if (tid >= face) return;
-    const int sas = t.s[slab];
+    const int sas = t.s[2 * slab + side];
     if (sas < 0) return;
     (void)n1;
     const int64_t x = tid % nx;
@@ -94,37 +100,41 @@ int launch_fluid_bc(double *base, int64_t pitch, int nx, int ny, int nz, int dir
                     int depth, hipStream_t s)
 {
     const int N = (dir == 1) ? nx : (dir == 2 ? ny : nz);
+    SasTable t;
+    bool any[2] = {false, false};
+    for (int k = 0; k < 10; k++) {
+        const int side = k & 1;
+        t.s[k] = modes10[k];
+        if (t.s[k] < 0) continue;
+        any[side] = true;
+        if ((t.s[k] == 0 || t.s[k] == 1) && N < 2 * depth + 2)
+            return IM_FAIL(ERROR_INVALID_ARGS, "mirror BC needs >= %d cells (got %d)", 2 * depth + 2, N);
+        if (N < depth + 2) return IM_FAIL(ERROR_INVALID_ARGS, "BC needs >= %d cells (got %d)", depth + 2, N);
+        if (t.s[k] == 3 && (dir != 1 || side != 0))
+            return IM_FAIL(ERROR_NOIMPLEMENT, "linear extrapolation BC exists only on the -x face");
+    }
+    // Both faces in one launch when they cannot touch each other's cells
+    // (N >= 2*depth+2); otherwise side 0 then side 1, as the reference orders them.
+    const bool together = any[0] && any[1] && N >= 2 * depth + 2;
     for (int side = 0; side < 2; side++) {
-        SasTable t;
-        bool any = false;
-        for (int k = 0; k < 5; k++) {
-            t.s[k] = modes10[2 * k + side];
-            if (t.s[k] >= 0) {
-                any = true;
-                if ((t.s[k] == 0 || t.s[k] == 1) && N < 2 * depth + 2)
-                    return IM_FAIL(ERROR_INVALID_ARGS, "mirror BC needs >= %d cells (got %d)", 2 * depth + 2, N);
-                if (N < depth + 2)
-                    return IM_FAIL(ERROR_INVALID_ARGS, "BC needs >= %d cells (got %d)", depth + 2, N);
-                if (t.s[k] == 3 && (dir != 1 || side != 0))
-                    return IM_FAIL(ERROR_NOIMPLEMENT, "linear extrapolation BC exists only on the -x face");
-            }
-        }
-        if (!any) continue;
+        if (!any[side]) continue;
+        const int nsides = together ? 2 : 1;
         if (dir == 1) {
             const int64_t rows = (int64_t)ny * nz;
-            const int64_t total = rows * 5 * depth;
+            const int64_t total = rows * 5 * depth * nsides;
             hipLaunchKernelGGL(bc_face_x, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, base, pitch,
-                               nx, rows, side, depth, t);
+                               nx, rows, side, nsides, depth, t);
         } else {
             const int64_t stride = (dir == 2) ? (int64_t)nx : (int64_t)nx * ny;
             const int64_t ostride = (dir == 2) ? (int64_t)nx * ny : (int64_t)nx;
             const int64_t n2 = (dir == 2) ? nz : ny;
             const int64_t face = (int64_t)nx * n2;
-            hipLaunchKernelGGL(bc_face_yz, dim3((unsigned)((face + 255) / 256), 5), dim3(256), 0, s, base,
+            hipLaunchKernelGGL(bc_face_yz, dim3((unsigned)((face + 255) / 256), 5 * nsides), dim3(256), 0, s, base,
                                pitch, nx, (int64_t)N, n2, stride, ostride, N, dir == 3 ? 1 : 0, side, depth, t);
         }
         int rc = IM_LAUNCHED("boundary condition kernel");
         if (rc) return rc;
+        if (together) break;
     }
     return SUCCESSFUL;
 }
