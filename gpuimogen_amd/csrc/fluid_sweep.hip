@@ -1,9 +1,12 @@
-// fluid_hllc.hip -- fused HLLC SSP-RK2 directional sweeps for gfx950.
+// fluid_sweep.hip -- fused HLLC / HLL SSP-RK2 directional sweeps for gfx950.
 //
 // One kernel launch performs both SSP-RK2 stages of one directional sweep
-// (the reference runs them as two launches through a 5-slab temporary plus a
-// physical transpose, gpuclass/cudaFluidStep.cu:485-627 and
-// gpuclass/flux.cu:92-116). The fluid is updated in place in the reference's
+// (the reference runs them as two or more launches through a 5-slab temporary
+// plus a physical transpose, gpuclass/cudaFluidStep.cu:485-627 and :630-700,
+// gpuclass/flux.cu:92-116). For HLL the pressure solver's floors, which the
+// reference applies in place before each stage (:646-690), are applied to
+// each cell as it is read: Q before stage A, U1 before stage B; the floored
+// values feed both the fluxes and the SSP-RK2 combination. The fluid is updated in place in the reference's
 // 5-slab layout (rho, E, px, py, pz; x fastest), with the reference's periodic
 // wrap along the swept line (cudaFluidStep.cu:1159-1162).
 //
@@ -18,7 +21,7 @@
 //    occupancy; a segment reads its 4-cell halos directly, and keeps its first
 //    4 outputs in a scratch buffer until the whole wave has finished reading,
 //    so the in-place update (and the periodic wrap) is race-free.
-#include "hllc_device.h"
+#include "sweep_device.h"
 #include "imogen_internal.h"
 
 namespace imogen {
@@ -63,34 +66,17 @@ __device__ __forceinline__ V5 add5(const V5 &x, const V5 &y)
     return V5{x.a + y.a, x.b + y.b, x.c + y.c, x.d + y.d, x.e + y.e};
 }
 
-// Slab index holding (normal, tangential1, tangential2) momentum for a sweep
-// direction, in the order the reference kernels read them (:1199-1216).
-template <int DIR>
-struct MomSlabs;
-template <>
-struct MomSlabs<1> {
-    static constexpr int n = 2, t1 = 3, t2 = 4;
-};
-template <>
-struct MomSlabs<2> {
-    static constexpr int n = 3, t1 = 2, t2 = 4;
-};
-template <>
-struct MomSlabs<3> {
-    static constexpr int n = 4, t1 = 3, t2 = 2;
-};
-
-template <int DIR>
+template <class Meth, int DIR>
 __device__ __forceinline__ V5 load_cell(const double *__restrict__ p, int64_t pitch)
 {
-    using M = MomSlabs<DIR>;
+    using M = MomSlabs<Meth, DIR>;
     return V5{p[0], p[pitch], p[M::n * pitch], p[M::t1 * pitch], p[M::t2 * pitch]};
 }
 
-template <int DIR>
+template <class Meth, int DIR>
 __device__ __forceinline__ void store_cell(double *__restrict__ p, int64_t pitch, const V5 &v)
 {
-    using M = MomSlabs<DIR>;
+    using M = MomSlabs<Meth, DIR>;
     p[0] = v.a;
     p[pitch] = v.b;
     p[M::n * pitch] = v.c;
@@ -103,14 +89,18 @@ __device__ __forceinline__ void store_cell(double *__restrict__ p, int64_t pitch
 // ---------------------------------------------------------------------------
 
 // Interface fluxes of the cells a lane owns. F[k] is the flux at k+1/2 for
-// k < cnt; Fm is the flux at -1/2 (the left lane's last interface).
-template <int K, int LIM>
-__device__ __forceinline__ void x_fluxes(const HydroConsts &hc, int rt, const V5 (&U)[K], int cnt,
-                                         int srcL, int srcR, V5 (&F)[K], V5 &Fm)
+// k < cnt; Fm is the flux at -1/2 (the left lane's last interface). The
+// method's floors are applied to U in place first.
+template <class M, int K, int LIM>
+__device__ __forceinline__ void x_fluxes(const HydroConsts &hc, int rt, V5 (&U)[K], int cnt, int srcL,
+                                         int srcR, V5 (&F)[K], V5 &Fm)
 {
     V5 w[K];
 #pragma unroll
-    for (int k = 0; k < K; k++) w[k] = to_prim(U[k]);
+    for (int k = 0; k < K; k++) {
+        const double e = M::template floor<1>(hc, U[k]);
+        w[k] = M::prims(hc, U[k], e);
+    }
 
     V5 wlast = w[0];
 #pragma unroll
@@ -127,13 +117,13 @@ __device__ __forceinline__ void x_fluxes(const HydroConsts &hc, int rt, const V5
 
     V5 Lf[K], Rf[K];
 #pragma unroll
-    for (int k = 0; k < K; k++) faces<LIM>(hc, w[k], d[k], d[k + 1], rt, Lf[k], Rf[k]);
+    for (int k = 0; k < K; k++) M::template face<LIM>(hc, w[k], d[k], d[k + 1], rt, Lf[k], Rf[k]);
 
     const V5 LfR = shfl5(Lf[0], srcR);
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const V5 right = (k + 1 < K) ? sel5(k + 1 < cnt, Lf[(k + 1 < K) ? k + 1 : 0], LfR) : LfR;
-        F[k] = hllc_flux(hc, Rf[k], right);
+        F[k] = M::flux(hc, Rf[k], right);
     }
     V5 Flast = F[0];
 #pragma unroll
@@ -148,10 +138,10 @@ __device__ __forceinline__ void x_fluxes(const HydroConsts &hc, int rt, const V5
 // operations are unconditional (the compiler can then count them exactly and
 // wait only for the loads it needs); lanes past the last line read a clamped
 // valid line and write into a private dummy slot.
-template <int K, int LIM, bool FULL>
-__global__ __launch_bounds__(256) void hllc_sweep_x(double *__restrict__ q, int64_t pitch, int N,
-                                                    int64_t nlines, int nl, double lambda,
-                                                    HydroConsts hc, int rt, double *__restrict__ dummy)
+template <class M, int K, int LIM, bool FULL>
+__global__ __launch_bounds__(256) void sweep_x(double *__restrict__ q, int64_t pitch, int N, int64_t nlines,
+                                               int nl, double lambda, HydroConsts hc, int rt,
+                                               double *__restrict__ dummy)
 {
     const int lane = threadIdx.x & 63;
     const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -181,7 +171,7 @@ __global__ __launch_bounds__(256) void hllc_sweep_x(double *__restrict__ q, int6
         double *b = line_ptr(ln);
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            if (k < cnt) Qd[k] = load_cell<1>(b + k, pitch);
+            if (k < cnt) Qd[k] = load_cell<M, 1>(b + k, pitch);
             else Qd[k] = V5{1.0, 1.0, 0.0, 0.0, 0.0};
         }
     };
@@ -198,24 +188,21 @@ __global__ __launch_bounds__(256) void hllc_sweep_x(double *__restrict__ q, int6
 
         // stage A: U1 = Q - lambda dF(Q)
         V5 F[K], Fm;
-        x_fluxes<K, LIM>(hc, rt, Q, cnt, srcL, srcR, F, Fm);
+        x_fluxes<M, K, LIM>(hc, rt, Q, cnt, srcL, srcR, F, Fm);
         V5 U1[K];
 #pragma unroll
-        for (int k = 0; k < K; k++) {
-            U1[k] = axpy_update(Q[k], lambda, F[k], k == 0 ? Fm : F[k > 0 ? k - 1 : 0]);
-            Q[k] = add5(U1[k], Q[k]);  // Q now holds (U1 + Q)
-        }
+        for (int k = 0; k < K; k++) U1[k] = axpy_update(Q[k], lambda, F[k], k == 0 ? Fm : F[k > 0 ? k - 1 : 0]);
 
         // stage B: Q = .5((U1 + Q) - lambda dF(U1))
-        x_fluxes<K, LIM>(hc, rt, U1, cnt, srcL, srcR, F, Fm);
+        x_fluxes<M, K, LIM>(hc, rt, U1, cnt, srcL, srcR, F, Fm);
         const bool own = laneUsed && line < nlines;
         double *dst = own ? line_ptr(line) : mydummy;
         const int64_t st = own ? pitch : nthr;
 #pragma unroll
         for (int k = 0; k < K; k++) {
             if (k < cnt) {
-                const V5 o = avg_update(Q[k], lambda, F[k], k == 0 ? Fm : F[k > 0 ? k - 1 : 0]);
-                store_cell<1>(own ? dst + k : dst, st, o);
+                const V5 o = avg_update(add5(U1[k], Q[k]), lambda, F[k], k == 0 ? Fm : F[k > 0 ? k - 1 : 0]);
+                store_cell<M, 1>(own ? dst + k : dst, st, o);
             }
         }
         if (!more) break;
@@ -229,12 +216,11 @@ __global__ __launch_bounds__(256) void hllc_sweep_x(double *__restrict__ q, int6
 // ---------------------------------------------------------------------------
 // Y/Z sweep: register-window march along strided columns
 // ---------------------------------------------------------------------------
-template <int DIR, int LIM>
-__global__ __launch_bounds__(256) void hllc_sweep_march(double *__restrict__ q, int64_t pitch, int nx,
-                                                        int64_t sstride, int64_t ostride, int N,
-                                                        int S, int seglen, int wavesPerRow,
-                                                        double lambda, HydroConsts hc, int rt,
-                                                        double *__restrict__ defer)
+template <class M, int DIR, int LIM>
+__global__ __launch_bounds__(256) void sweep_march(double *__restrict__ q, int64_t pitch, int nx, int64_t sstride,
+                                                   int64_t ostride, int N, int S, int seglen, int wavesPerRow,
+                                                   double lambda, HydroConsts hc, int rt,
+                                                   double *__restrict__ defer)
 {
     const int lane = threadIdx.x & 63;
     const int wib = threadIdx.x >> 6;
@@ -249,7 +235,6 @@ __global__ __launch_bounds__(256) void hllc_sweep_march(double *__restrict__ q, 
     int L = N - j0;
     L = L < seglen ? L : seglen;
     if (!colValid) L = 0;
-    const int steps = L + 9;  // loads j0-4 .. j0+L+3, one more step drains stage B
 
     double *col = q + (colValid ? x + (int64_t)blockIdx.y * ostride : 0);
     const int64_t gtid = ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x;
@@ -265,7 +250,8 @@ __global__ __launch_bounds__(256) void hllc_sweep_march(double *__restrict__ q, 
     // Every window variable lives in two slots that trade roles on even and
     // odd steps (prev <-> new), so the loop carries no register moves.
     V5 QA = one, QB = one;    // Q_{m-2} in, Q_m out
-    V5 U1A = one, U1B = one;  // U1 of the previous step in, this step's out
+    V5 U1A = one, U1B = one;  // U1 (floored) of the previous step in, this step's out
+    double eA = 1.0, eB = 1.0;  // internal energy the floors left in U1A / U1B
     V5 wA = one, wB = one, dA = zero, dB = zero, RA = one, RB = one, FAA = zero, FAB = zero;
     V5 w1A = one, w1B = one, d1A = zero, d1B = zero, R1A = one, R1B = one, FBA = zero, FBB = zero;
 
@@ -276,16 +262,21 @@ __global__ __launch_bounds__(256) void hllc_sweep_march(double *__restrict__ q, 
     __shared__ double sring[4][4][5][64];  // [wave in block][slot][variable][lane]
     double(*ring)[5][64] = sring[wib];
 
-    auto step = [&](int t, const V5 &Qcur, V5 &Qs, const V5 &U1p, V5 &U1n, const V5 &wp, V5 &wn,
-                    const V5 &dp, V5 &dn, const V5 &Rp, V5 &Rn, const V5 &FAp, V5 &FAn, const V5 &w1p,
-                    V5 &w1n, const V5 &d1p, V5 &d1n, const V5 &R1p, V5 &R1n, const V5 &FBp, V5 &FBn) {
+    auto step = [&](int t, V5 &Qcur, V5 &Qs, const V5 &U1p, V5 &U1n, const double &ep, double &en,
+                    const V5 &wp, V5 &wn, const V5 &dp, V5 &dn, const V5 &Rp, V5 &Rn, const V5 &FAp, V5 &FAn,
+                    const V5 &w1p, V5 &w1n, const V5 &d1p, V5 &d1n, const V5 &R1p, V5 &R1n, const V5 &FBp,
+                    V5 &FBn) {
         // stage A at cell m-2
-        wn = to_prim(Qcur);
+        {
+            const double e = M::template floor<DIR>(hc, Qcur);
+            wn = M::prims(hc, Qcur, e);
+        }
         dn = wn - wp;
         V5 Lc;
-        faces<LIM>(hc, wp, dp, dn, rt, Lc, Rn);  // faces of cell m-1
-        FAn = hllc_flux(hc, Rp, Lc);              // interface (m-2)+1/2
+        M::template face<LIM>(hc, wp, dp, dn, rt, Lc, Rn);  // faces of cell m-1
+        FAn = M::flux(hc, Rp, Lc);                           // interface (m-2)+1/2
         U1n = axpy_update(Qs, lambda, FAn, FAp);
+        en = M::template floor<DIR>(hc, U1n);
         {
             const V5 S2 = add5(U1n, Qs);  // (U1+Q) of cell m-2, used at step m+3
             double *r = &ring[t & 3][0][lane];
@@ -298,11 +289,11 @@ __global__ __launch_bounds__(256) void hllc_sweep_march(double *__restrict__ q, 
         Qs = Qcur;
 
         // stage B at cell m-5, from the previous step's U1 (cell m-3)
-        w1n = to_prim(U1p);
+        w1n = M::prims(hc, U1p, ep);
         d1n = w1n - w1p;
         V5 L1c;
-        faces<LIM>(hc, w1p, d1p, d1n, rt, L1c, R1n);  // faces of cell m-4
-        FBn = hllc_flux(hc, R1p, L1c);                 // interface (m-5)+1/2
+        M::template face<LIM>(hc, w1p, d1p, d1n, rt, L1c, R1n);  // faces of cell m-4
+        FBn = M::flux(hc, R1p, L1c);                              // interface (m-5)+1/2
         const double *r = &ring[(t + 1) & 3][0][lane];  // slot of step t-3
         const V5 S5{r[0], r[64], r[128], r[192], r[256]};
         const V5 o = avg_update(S5, lambda, FBn, FBp);  // cell m-5
@@ -316,7 +307,7 @@ __global__ __launch_bounds__(256) void hllc_sweep_march(double *__restrict__ q, 
         const bool direct = (oi >= 4) && (oi < L);
         const int slot = (oi >= 0 && oi < 4 && oi < L) ? oi : 4;
         double *dst = direct ? col + (int64_t)(j0 + oi) * sstride : defer + gtid + (int64_t)slot * 5 * nthr;
-        store_cell<DIR>(dst, direct ? pitch : nthr, o);
+        store_cell<M, DIR>(dst, direct ? pitch : nthr, o);
     };
 
     const int maxsteps = seglen + 9;
@@ -325,38 +316,39 @@ __global__ __launch_bounds__(256) void hllc_sweep_march(double *__restrict__ q, 
     // ignores: mw always wraps into [0, N) and `col` of an inactive column is
     // element 0), so the compiler neither copies slots nor waits for a load it
     // has just issued.
-    V5 N0 = load_cell<DIR>(col + (int64_t)mw * sstride, pitch);
+    V5 N0 = load_cell<M, DIR>(col + (int64_t)mw * sstride, pitch);
     V5 N1;
     // Enter the loop with the same vector-memory counter pattern as the
     // back-edge (load group, then 5 stores) so the wait at the loop head can
     // leave stores outstanding: the stores write the loaded cell into this
     // lane's dummy slot, which also pins them after the load.
-    store_cell<DIR>(defer + gtid + (int64_t)4 * 5 * nthr, nthr, N0);
+    store_cell<M, DIR>(defer + gtid + (int64_t)4 * 5 * nthr, nthr, N0);
 
     for (int t = 0; t < maxsteps; t += 2) {
         mw = (mw + 1 == N) ? 0 : mw + 1;
-        N1 = load_cell<DIR>(col + (int64_t)mw * sstride, pitch);
-        step(t, N0, QA, U1A, U1B, wA, wB, dA, dB, RA, RB, FAA, FAB, w1A, w1B, d1A, d1B, R1A, R1B, FBA, FBB);
+        N1 = load_cell<M, DIR>(col + (int64_t)mw * sstride, pitch);
+        step(t, N0, QA, U1A, U1B, eA, eB, wA, wB, dA, dB, RA, RB, FAA, FAB, w1A, w1B, d1A, d1B, R1A, R1B, FBA,
+             FBB);
         if (t + 1 < maxsteps) {
             mw = (mw + 1 == N) ? 0 : mw + 1;
-            N0 = load_cell<DIR>(col + (int64_t)mw * sstride, pitch);
-            step(t + 1, N1, QB, U1B, U1A, wB, wA, dB, dA, RB, RA, FAB, FAA, w1B, w1A, d1B, d1A, R1B, R1A, FBB,
-                 FBA);
+            N0 = load_cell<M, DIR>(col + (int64_t)mw * sstride, pitch);
+            step(t + 1, N1, QB, U1B, U1A, eB, eA, wB, wA, dB, dA, RB, RA, FAB, FAA, w1B, w1A, d1B, d1A, R1B, R1A,
+                 FBB, FBA);
         }
     }
 
     // every lane of the wave has finished reading: release the deferred outputs
     const int nd = L < 4 ? L : 4;
     for (int oi = 0; oi < nd; oi++) {
-        const V5 o = load_cell<DIR>(defer + gtid + (int64_t)oi * 5 * nthr, nthr);
-        store_cell<DIR>(col + (int64_t)(j0 + oi) * sstride, pitch, o);
+        const V5 o = load_cell<M, DIR>(defer + gtid + (int64_t)oi * 5 * nthr, nthr);
+        store_cell<M, DIR>(col + (int64_t)(j0 + oi) * sstride, pitch, o);
     }
 }
 
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
-template <int LIM>
+template <class M, int LIM>
 static int launch_x(double *q, int64_t pitch, int N, int64_t nlines, double lambda, const HydroConsts &hc,
                     int rt, hipStream_t s)
 {
@@ -381,14 +373,14 @@ static int launch_x(double *q, int64_t pitch, int N, int64_t nlines, double lamb
     const bool full = (nl * K == N) && (64 % nl == 0);
     double *dummy = (double *)scratch((size_t)blocks * 256 * 5 * sizeof(double));
     if (!dummy) return IM_FAIL(ERROR_NOMEM, "no scratch for the X sweep");
-#define IM_XK(KK)                                                                                      \
-    case KK:                                                                                           \
-        if (full)                                                                                      \
-            hipLaunchKernelGGL((hllc_sweep_x<KK, LIM, true>), grid, block, 0, s, q, pitch, N, nlines, nl, \
-                               lambda, hc, rt, dummy);                                                 \
-        else                                                                                           \
-            hipLaunchKernelGGL((hllc_sweep_x<KK, LIM, false>), grid, block, 0, s, q, pitch, N, nlines, nl,\
-                               lambda, hc, rt, dummy);                                                 \
+#define IM_XK(KK)                                                                                       \
+    case KK:                                                                                            \
+        if (full)                                                                                       \
+            hipLaunchKernelGGL((sweep_x<M, KK, LIM, true>), grid, block, 0, s, q, pitch, N, nlines, nl,  \
+                               lambda, hc, rt, dummy);                                                  \
+        else                                                                                            \
+            hipLaunchKernelGGL((sweep_x<M, KK, LIM, false>), grid, block, 0, s, q, pitch, N, nlines, nl, \
+                               lambda, hc, rt, dummy);                                                  \
         break;
     switch (K) {
         IM_XK(1)
@@ -401,10 +393,10 @@ static int launch_x(double *q, int64_t pitch, int N, int64_t nlines, double lamb
         IM_XK(16)
     }
 #undef IM_XK
-    return IM_LAUNCHED("hllc_sweep_x");
+    return IM_LAUNCHED("sweep_x");
 }
 
-template <int LIM>
+template <class M, int LIM>
 static int launch_march(double *q, int64_t pitch, int nx, int ny, int nz, int dir, double lambda,
                         const HydroConsts &hc, int rt, hipStream_t s)
 {
@@ -426,43 +418,59 @@ static int launch_march(double *q, int64_t pitch, int nx, int ny, int nz, int di
     double *defer = (double *)scratch((size_t)nthr * 25 * sizeof(double));
     if (!defer) return IM_FAIL(ERROR_NOMEM, "no scratch for deferred outputs");
     if (dir == 2)
-        hipLaunchKernelGGL((hllc_sweep_march<2, LIM>), grid, block, 0, s, q, pitch, nx, sstride, ostride,
-                           N, S, seglen, wavesPerRow, lambda, hc, rt, defer);
+        hipLaunchKernelGGL((sweep_march<M, 2, LIM>), grid, block, 0, s, q, pitch, nx, sstride, ostride, N, S,
+                           seglen, wavesPerRow, lambda, hc, rt, defer);
     else
-        hipLaunchKernelGGL((hllc_sweep_march<3, LIM>), grid, block, 0, s, q, pitch, nx, sstride, ostride,
-                           N, S, seglen, wavesPerRow, lambda, hc, rt, defer);
-    return IM_LAUNCHED("hllc_sweep_march");
+        hipLaunchKernelGGL((sweep_march<M, 3, LIM>), grid, block, 0, s, q, pitch, nx, sstride, ostride, N, S,
+                           seglen, wavesPerRow, lambda, hc, rt, defer);
+    return IM_LAUNCHED("sweep_march");
 }
 
-template <int LIM>
+template <class M, int LIM>
 static int launch_dir(double *q, int64_t pitch, int nx, int ny, int nz, int dir, double lambda,
                       const HydroConsts &hc, int rt, hipStream_t s)
 {
     const int slot = prof_begin(PROF_SWEEP_X + dir - 1, s);
     int rc;
-    if (dir == 1) rc = launch_x<LIM>(q, pitch, nx, (int64_t)ny * nz, lambda, hc, rt, s);
-    else rc = launch_march<LIM>(q, pitch, nx, ny, nz, dir, lambda, hc, rt, s);
+    if (dir == 1) rc = launch_x<M, LIM>(q, pitch, nx, (int64_t)ny * nz, lambda, hc, rt, s);
+    else rc = launch_march<M, LIM>(q, pitch, nx, ny, nz, dir, lambda, hc, rt, s);
     prof_end(slot, s);
     return rc;
 }
 
-int launch_hllc_sweep(double *base, int64_t pitch, int nx, int ny, int nz, int dir, double lambda,
-                      double gamma, int limiter, hipStream_t s)
+template <class M>
+static int launch_method(double *base, int64_t pitch, int nx, int ny, int nz, int dir, double lambda,
+                         const HydroConsts &hc, int limiter, hipStream_t s)
 {
-    if (dir < 1 || dir > 3) return IM_FAIL(ERROR_INVALID_ARGS, "sweep direction %d", dir);
-    const int N = (dir == 1) ? nx : (dir == 2 ? ny : nz);
-    if (N < 3) return SUCCESSFUL;  // cudaFluidStep.cu:301
-    HydroConsts hc{gamma, gamma - 1.0, gamma / (gamma - 1.0)};
     switch (limiter) {
-    case LIMITER_OSPRE: return launch_dir<0>(base, pitch, nx, ny, nz, dir, lambda, hc, 0, s);
-    case LIMITER_VANLEER: return launch_dir<1>(base, pitch, nx, ny, nz, dir, lambda, hc, 1, s);
+    case LIMITER_OSPRE: return launch_dir<M, 0>(base, pitch, nx, ny, nz, dir, lambda, hc, 0, s);
+    case LIMITER_VANLEER: return launch_dir<M, 1>(base, pitch, nx, ny, nz, dir, lambda, hc, 1, s);
     case LIMITER_MINMOD:
     case LIMITER_SUPERBEE:
     case LIMITER_VANALBADA:
     case LIMITER_MC:
-    case LIMITER_ZERO: return launch_dir<-1>(base, pitch, nx, ny, nz, dir, lambda, hc, limiter, s);
+    case LIMITER_ZERO: return launch_dir<M, -1>(base, pitch, nx, ny, nz, dir, lambda, hc, limiter, s);
     default: return IM_FAIL(ERROR_INVALID_ARGS, "unknown slope limiter %d", limiter);
     }
+}
+
+int launch_fluid_sweep(double *base, int64_t pitch, int nx, int ny, int nz, int dir, int method, double lambda,
+                       double gamma, double rhomin, double ciso, int limiter, hipStream_t s)
+{
+    if (dir < 1 || dir > 3) return IM_FAIL(ERROR_INVALID_ARGS, "sweep direction %d", dir);
+    const int N = (dir == 1) ? nx : (dir == 2 ? ny : nz);
+    if (N < 3) return SUCCESSFUL;  // cudaFluidStep.cu:301
+    HydroConsts hc;
+    hc.gamma = gamma;
+    hc.gm1 = gamma - 1.0;
+    hc.govergm1 = gamma / (gamma - 1.0);
+    hc.rhomin = rhomin;
+    hc.emin = 1e-5 / (gamma - 1.0);  // cudaFluidStep.cu:298
+    hc.ciso = ciso;
+    hc.cisosq = ciso * ciso;
+    if (method == METHOD_HLLC) return launch_method<Hllc>(base, pitch, nx, ny, nz, dir, lambda, hc, limiter, s);
+    if (method == METHOD_HLL) return launch_method<Hll>(base, pitch, nx, ny, nz, dir, lambda, hc, limiter, s);
+    return IM_FAIL(ERROR_INVALID_ARGS, "sweep kernel for method %d", method);
 }
 
 }  // namespace imogen

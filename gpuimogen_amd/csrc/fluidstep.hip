@@ -133,11 +133,13 @@ int performFluidUpdate_1D(MGArray *fluid, FluidStepParams params, ParallelTopolo
     if (!st) return ERROR_CUDA_BLEW_UP;
     switch (params.stepMethod) {
     case METHOD_HLLC:
-        return launch_hllc_sweep(base, pitch, fluid->dim[0], fluid->dim[1], fluid->dim[2], dir, lambda,
-                                 params.thermoGamma, params.slopeLimiter, st->stream);
     case METHOD_HLL:
+        return launch_fluid_sweep(base, pitch, fluid->dim[0], fluid->dim[1], fluid->dim[2], dir,
+                                  (int)params.stepMethod, lambda, params.thermoGamma, params.minimumRho,
+                                  params.Cisothermal, params.slopeLimiter, st->stream);
     case METHOD_XINJIN:
-        return IM_FAIL(ERROR_NOIMPLEMENT, "fluid method %d is not built yet", (int)params.stepMethod);
+        return launch_xinjin_sweep(base, pitch, fluid->dim[0], fluid->dim[1], fluid->dim[2], dir, lambda,
+                                   params.thermoGamma, params.minimumRho, st->stream);
     default: return IM_FAIL(ERROR_INVALID_ARGS, "unknown fluid method %d", (int)params.stepMethod);
     }
 }

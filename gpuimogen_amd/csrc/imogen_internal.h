@@ -43,9 +43,13 @@ void prof_end(int slot, hipStream_t s);
 // Device scratch of at least `bytes`; contents are undefined on return.
 void *scratch(size_t bytes);
 
-// Launchers (fluid_hllc.hip, statics.hip, cfl.hip)
-int launch_hllc_sweep(double *base, int64_t pitch, int nx, int ny, int nz, int dir, double lambda,
-                      double gamma, int limiter, hipStream_t s);
+// Launchers (fluid_sweep.hip, xinjin.hip, statics.hip, cfl.hip)
+// method: METHOD_HLLC or METHOD_HLL. rhomin, ciso: HLL floors and isothermal
+// sound speed (-1: adiabatic).
+int launch_fluid_sweep(double *base, int64_t pitch, int nx, int ny, int nz, int dir, int method, double lambda,
+                       double gamma, double rhomin, double ciso, int limiter, hipStream_t s);
+int launch_xinjin_sweep(double *base, int64_t pitch, int nx, int ny, int nz, int dir, double lambda,
+                        double gamma, double rhomin, hipStream_t s);
 int launch_fluid_bc(double *base, int64_t pitch, int nx, int ny, int nz, int dir, const int *modes10,
                     int depth, hipStream_t s);
 int launch_soundspeed(const double *base, int64_t pitch, int64_t n, double gamma, double *cs,

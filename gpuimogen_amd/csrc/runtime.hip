@@ -364,7 +364,7 @@ int MGA_delete(MGArray *victim)
     for (int j = 0; j < victim->nGPUs; j++) {
         if (victim->devicePtr[j]) {
             DeviceState *st = state();
-            if (st) hipStreamSynchronize(st->stream);
+            if (st) (void)hipStreamSynchronize(st->stream);
             rc = IM_HIP(hipFree(victim->devicePtr[j]));
             victim->devicePtr[j] = nullptr;
         }
@@ -422,7 +422,7 @@ static void init_array(MGArray *m, const int *dims3, int halo, int partDir)
     m->currentPermutation[1] = 2;
     m->currentPermutation[2] = 3;
     int dev = 0;
-    hipGetDevice(&dev);
+    (void)hipGetDevice(&dev);
     m->deviceID[0] = dev;
     for (int i = 1; i < MAX_GPUS_USED; i++) m->deviceID[i] = -1;
     m->partNumel[0] = (int)m->numel;
@@ -542,13 +542,13 @@ int GPU_copy(const int64_t *dstTag, const int64_t *srcTag)
 int GPU_ctrl_memory(int device, size_t *freeBytes, size_t *totalBytes)
 {
     int cur = 0;
-    hipGetDevice(&cur);
+    (void)hipGetDevice(&cur);
     if (device != cur) {
         int rc = IM_HIP(hipSetDevice(device));
         if (rc) return rc;
     }
     int rc = IM_HIP(hipMemGetInfo(freeBytes, totalBytes));
-    if (device != cur) hipSetDevice(cur);
+    if (device != cur) (void)hipSetDevice(cur);
     return rc;
 }
 

@@ -291,6 +291,262 @@ static void hllc_line_fluxes(const thermo_t *t, int n, double *const in[5], doub
 }
 
 /* ------------------------------------------------------------------------ */
+/* HLL sweep: gpuclass/cudaFluidStep.cu:1801-2422, pressure solver 2813-2864 */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    double gamma, gm1, govergm1, rhomin, emin, ciso, cisosq;
+    int lim;
+} hllconst_t;
+
+/* cukern_PressureSolverHydro with CFD_ENFORCE_MINVALS (cudaFluidStep.cu:2813-2864):
+ * floors rho and e in place on the physical-order cell c = (rho, E, px, py, pz)
+ * and returns the gas pressure. */
+static double hll_floor_pressure(const hllconst_t *t, double *c)
+{
+    double rho = c[0], E, z, momsq, P;
+    if (rho < t->rhomin) {
+        P = t->rhomin / rho;
+        rho = c[0] = t->rhomin;
+        E = c[1] = P * c[1];
+        z = c[2] = P * c[2];
+        momsq = z * z;
+        z = c[3] = P * c[3];
+        momsq += z * z;
+        z = c[4] = P * c[4];
+        momsq += z * z;
+    } else {
+        E = c[1];
+        z = c[2];
+        momsq = z * z;
+        z = c[3];
+        momsq += z * z;
+        z = c[4];
+        momsq += z * z;
+    }
+    P = E - .5 * momsq / rho;
+    if (P < rho * t->emin) {
+        P = t->emin * rho;
+        c[1] = (.5 * momsq / rho) + P;
+    }
+    return P * t->gm1;
+}
+
+/* computeEinfeldtBounds_isothermal (cudaFluidStep.cu:1779-1799) */
+static void einfeldt_iso(double rhoL, double vL, double rhoR, double vR, double cs, double *sl, double *sr)
+{
+    double Fa = sqrt(rhoR / rhoL);
+    double Fb = 1.0 / (1.0 + Fa);
+    double Vroe = (vL + Fa * vR) * Fb;
+    cs = Vroe - cs;
+    double vl = vL - cs;
+    double vr = vR + cs;
+    vl = (cs < vl) ? cs : vl;
+    cs = 2 * Vroe - cs;
+    vr = (cs > vr) ? cs : vr;
+    *sl = vl;
+    *sr = vr;
+}
+
+/* HLL flux between left state L and right state R, each (rho, vN, P, vT1, vT2)
+ * as the reference stores them (A..E). out = (mass, energy, pN, pT1, pT2).
+ * cudaFluidStep.cu:2124-2196 */
+static void hll_flux(const hllconst_t *t, const double *L, const double *R, double *out)
+{
+    double Ale = L[0], Ble = L[1], Cle = L[2], Dle = L[3], Ele = L[4];
+    double Are = R[0], Bre = R[1], Cre = R[2], Dre = R[3], Ere = R[4];
+    double Utilde = (Bre - Ble);
+    double Sleft = (Dre - Dle);
+    double Sright = (Ere - Ele);
+    double sl, sr;
+    if (t->ciso == -1) {
+        thermo_t th;
+        th.gamma = t->gamma;
+        th.gm1 = t->gm1;
+        th.govergm1 = t->govergm1;
+        th.lim = t->lim;
+        einfeldt(&th, Ale, Ble, Cle, Are, Bre, Cre, Utilde * Utilde + Sleft * Sleft + Sright * Sright, &sl, &sr);
+    } else {
+        /* the reference passes C_iso^2 where the sound speed is expected */
+        einfeldt_iso(Ale, Ble, Are, Bre, t->cisosq, &sl, &sr);
+    }
+    Sleft = sl;
+    Sright = sr;
+    double Atilde = 1 / (Sright - Sleft);
+    double Fa = Ale * Ble;
+    double Fb = Are * Bre;
+    int mode = 1; /* HLL */
+    if (Sleft >= 0) mode = 0;
+    if (Sright <= 0) mode = 2;
+    switch (mode) {
+    case 0:
+        out[0] = Fa;
+        if (t->ciso == -1) {
+            out[1] = Ble * (t->govergm1 * Cle + .5 * Ale * (Ble * Ble + Dle * Dle + Ele * Ele));
+            out[2] = Fa * Ble + Cle;
+        } else {
+            out[1] = Ble * (Cle + .5 * Ale * (Ble * Ble + Dle * Dle + Ele * Ele));
+            out[2] = Fa * Ble + t->cisosq * Ale;
+        }
+        out[3] = Fa * Dle;
+        out[4] = Fa * Ele;
+        break;
+    case 1:
+        out[0] = (Sright * (Ble - Sleft) * Ale - Sleft * (Bre - Sright) * Are) * Atilde;
+        if (t->ciso == -1)
+            out[2] = (Sright * (Ble - Sleft) * Fa - Sleft * (Bre - Sright) * Fb + Sright * Cle - Sleft * Cre) * Atilde;
+        else
+            out[2] = (Sright * (Ble - Sleft) * Fa - Sleft * (Bre - Sright) * Fb +
+                      t->cisosq * (Sright * Ale - Sleft * Are)) * Atilde;
+        out[3] = (Sright * (Ble - Sleft) * Ale * Dle - Sleft * (Bre - Sright) * Are * Dre) * Atilde;
+        out[4] = (Sright * (Ble - Sleft) * Ale * Ele - Sleft * (Bre - Sright) * Are * Ere) * Atilde;
+        if (t->ciso == -1) {
+            Fa = .5 * Ale * (Ble * Ble + Dle * Dle + Ele * Ele);
+            Fb = .5 * Are * (Bre * Bre + Dre * Dre + Ere * Ere);
+            double e = (Sright * (Ble - Sleft) * Fa - Sleft * (Bre - Sright) * Fb);
+            out[1] = (e + (t->govergm1 * (Sright * Cle * Ble - Sleft * Cre * Bre) +
+                           Sleft * Sright * (Cre - Cle) / t->gm1)) * Atilde;
+        } else {
+            /* the isothermal energy flux lacks the 1/(Sr-Sl) factor in the reference (:2185-2189) */
+            Fa = .5 * Ale * (Ble * Ble + Dle * Dle + Ele * Ele) + Cle;
+            Fb = .5 * Are * (Bre * Bre + Dre * Dre + Ere * Ere) + Cre;
+            out[1] = (Sright * (Ble - Sleft) * Fa - Sleft * (Bre - Sright) * Fb);
+        }
+        break;
+    default:
+        out[0] = Fb;
+        if (t->ciso == -1) {
+            out[1] = Bre * (t->govergm1 * Cre + .5 * Are * (Bre * Bre + Dre * Dre + Ere * Ere));
+            out[2] = Fb * Bre + Cre;
+        } else {
+            out[1] = Bre * (Cre + .5 * Are * (Bre * Bre + Dre * Dre + Ere * Ere));
+            out[2] = Fb * Bre + t->ciso * Are; /* C_iso, not C_iso^2, on this side (:2191) */
+        }
+        out[3] = Fb * Dre;
+        out[4] = Fb * Ere;
+        break;
+    }
+}
+
+/* Fluxes at i+1/2 of a periodic line for HLL. in[v] holds the line in HLL's
+ * component order (rho, E, pN, pT1, pT2) after the floors; P its pressure. */
+static void hll_line_fluxes(const hllconst_t *t, int n, double *const in[5], const double *P, double *fl[5],
+                            double *ws)
+{
+    double *prim = ws;        /* 5n: rho, vN, P, vT1, vT2 */
+    double *lf = ws + 5 * n;
+    double *rf = ws + 10 * n;
+    int i, v;
+    for (i = 0; i < n; i++) {
+        double A = in[0][i];
+        prim[0 * n + i] = A;
+        prim[1 * n + i] = in[2][i] / A;
+        prim[2 * n + i] = P[i];
+        prim[3 * n + i] = in[3][i] / A;
+        prim[4 * n + i] = in[4][i] / A;
+    }
+    for (v = 0; v < 5; v++) {
+        const double *p = prim + v * n;
+        for (i = 0; i < n; i++) {
+            int im = (i == 0) ? n - 1 : i - 1;
+            int ip = (i == n - 1) ? 0 : i + 1;
+            double s = orc_slope_limiter(t->lim, p[i] - p[im], p[ip] - p[i]);
+            lf[v * n + i] = p[i] - s;
+            rf[v * n + i] = p[i] + s;
+        }
+    }
+    for (i = 0; i < n; i++) {
+        int ip = (i == n - 1) ? 0 : i + 1;
+        double L[5], R[5], o[5];
+        for (v = 0; v < 5; v++) {
+            L[v] = rf[v * n + i];
+            R[v] = lf[v * n + ip];
+        }
+        hll_flux(t, L, R, o);
+        for (v = 0; v < 5; v++) fl[v][i] = o[v];
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Xin-Jin relaxing TVD: cudaFluidStep.cu:2427-2654, freeze 2868-2941        */
+/* ------------------------------------------------------------------------ */
+
+/* Per-line freeze speed max(|p_x|/rho + sqrt(gamma P/rho)), always with the
+ * physical x momentum (cudaFluidStep.cu:2887-2896). c: physical-order line. */
+static double xj_freeze(double gamma, double gm1, int n, double *const c[5])
+{
+    double cmax = 0.0;
+    int i;
+    for (i = 0; i < n; i++) {
+        double invrho = 1.0 / c[0][i];
+        double psq = c[3][i];
+        double px = c[4][i];
+        psq = psq * psq + px * px;
+        px = c[2][i];
+        psq += px * px;
+        double P = (c[1][i] - .5 * psq * invrho) * gm1;
+        double loc = fabs(px) * invrho + sqrt(gamma * P * invrho);
+        cmax = (loc > cmax) ? loc : cmax;
+    }
+    return cmax;
+}
+
+/* One Xin-Jin stage on a periodic line (physical slab order). predict:
+ * out = floor(Q - lam dF(Q)); correct: out = floor(base - lam dF_tvd(Q)).
+ * nslab = slab index of the normal momentum (2,3,4). ws holds >= 4n doubles. */
+static void xj_stage(const hllconst_t *t, int n, int nslab, double cf, double lam, int correct,
+                     double *const Q[5], double *const base[5], double *out[5], double *ws)
+{
+    double *wl = ws, *wr = ws + n, *dl = ws + 2 * n, *dr = ws + 3 * n;
+    int i, v;
+    double *pr = (double *)malloc(sizeof(double) * 5 * (size_t)n);
+    for (v = 0; v < 5; v++) {
+        for (i = 0; i < n; i++) {
+            double rho = Q[0][i];
+            double P = t->gm1 * (Q[1][i] - .5 * (Q[4][i] * Q[4][i] + Q[3][i] * Q[3][i] + Q[2][i] * Q[2][i]) / rho);
+            double vx = Q[nslab][i] / rho;
+            double w;
+            if (v == 0) w = Q[nslab][i];
+            else if (v == 1) w = vx * (Q[1][i] + P);
+            else if (v == nslab) w = vx * Q[v][i] + P;
+            else w = vx * Q[v][i];
+            wl[i] = (cf * Q[v][i] - w);
+            wr[i] = (cf * Q[v][i] + w);
+        }
+        if (correct) {
+            for (i = 0; i < n; i++) {
+                int im = (i == 0) ? n - 1 : i - 1;
+                dl[i] = 0.5 * (wl[i] - wl[im]);
+                dr[i] = 0.5 * (wr[i] - wr[im]);
+            }
+            for (i = 0; i < n; i++) {
+                int ip = (i == n - 1) ? 0 : i + 1;
+                wl[i] -= flux_ospre(dl[i], dl[ip]);
+                wr[i] += flux_ospre(dr[i], dr[ip]);
+            }
+        }
+        for (i = 0; i < n; i++) {
+            int im = (i == 0) ? n - 1 : i - 1;
+            int ip = (i == n - 1) ? 0 : i + 1;
+            double b = correct ? base[v][i] : Q[v][i];
+            pr[v * n + i] = b - lam * (wr[i] - wr[im] - wl[ip] + wl[i]);
+        }
+    }
+    for (i = 0; i < n; i++) {
+        double p0 = pr[i], p1 = pr[n + i], p2 = pr[2 * n + i], p3 = pr[3 * n + i], p4 = pr[4 * n + i];
+        p0 = (p0 < t->rhomin) ? t->rhomin : p0;
+        double w = .5 * (p2 * p2 + p3 * p3 + p4 * p4) / p0;
+        if ((p1 - w) < p0 * t->emin) p1 = w + p0 * t->emin;
+        out[0][i] = p0;
+        out[1][i] = p1;
+        out[2][i] = p2;
+        out[3][i] = p3;
+        out[4][i] = p4;
+    }
+    free(pr);
+}
+
+/* ------------------------------------------------------------------------ */
 /* Line gather/scatter over the 3D array                                     */
 /* ------------------------------------------------------------------------ */
 
@@ -333,23 +589,113 @@ static int slab_for(int dir, int v)
 /* Directional update                                                        */
 /* ------------------------------------------------------------------------ */
 
-int orc_fluid_update_1d(double *q, const orc_grid *g, int dir, int method, double lambda,
-                        const orc_fluid_params *p)
+static int update_1d(double *q, const orc_grid *g, int dir, int method, double lambda,
+                     const orc_fluid_params *p, int predict_only)
 {
     if (dir < 1 || dir > 3) return ORC_EINVAL;
     lineinfo_t li;
     line_geometry(g, dir, &li);
     if (li.n < 3) return ORC_OK; /* cudaFluidStep.cu:301 */
-    if (method != ORC_HLLC) return ORC_ENOIMPL;
+    if (method != ORC_HLLC && method != ORC_HLL && method != ORC_XINJIN) return ORC_EINVAL;
 
     thermo_t t;
     t.gamma = p->gamma;
     t.gm1 = p->gamma - 1.0;
     t.govergm1 = p->gamma / (p->gamma - 1.0);
     t.lim = p->limiter;
+    hllconst_t hc;
+    hc.gamma = p->gamma;
+    hc.gm1 = p->gamma - 1.0;
+    hc.govergm1 = p->gamma / (p->gamma - 1.0);
+    hc.rhomin = p->rhomin;
+    hc.emin = 1e-5 / (p->gamma - 1); /* cudaFluidStep.cu:298 */
+    hc.ciso = p->ciso;
+    hc.cisosq = p->ciso * p->ciso;
+    hc.lim = p->limiter;
 
     int n = li.n;
     int failed = 0;
+    if (method != ORC_HLLC) {
+        /* HLL momentum order (cudaFluidStep.cu:1869-1885): X (px,py,pz), Y (py,px,pz), Z (pz,px,py) */
+        static const int hll_slab[4][3] = {{0, 0, 0}, {2, 3, 4}, {3, 2, 4}, {4, 2, 3}};
+#pragma omp parallel
+        {
+            double *buf = (double *)malloc(sizeof(double) * (size_t)n * 48);
+            if (!buf) {
+#pragma omp atomic write
+                failed = 1;
+            }
+            long l;
+#pragma omp for schedule(static)
+            for (l = 0; l < li.nlines; l++) {
+                if (!buf) continue;
+                double *C[5], *U[5], *FL[5], *H[5], *O[5];
+                double *P = buf + 25 * n, *ws = buf + 26 * n;
+                int v, i;
+                for (v = 0; v < 5; v++) {
+                    C[v] = buf + v * n;
+                    U[v] = buf + 5 * n + v * n;
+                    FL[v] = buf + 10 * n + v * n;
+                    H[v] = buf + 15 * n + v * n;
+                    O[v] = buf + 20 * n + v * n;
+                }
+                long base = line_base(g, dir, l);
+                for (v = 0; v < 5; v++) {
+                    const double *src = q + (long)v * g->pitch + base;
+                    for (i = 0; i < n; i++) C[v][i] = src[(long)i * li.stride];
+                }
+                if (method == ORC_XINJIN) {
+                    /* freeze(Q); predict 1/4 lambda; freeze(U1); correct 1/2 lambda (cudaFluidStep.cu:357-475) */
+                    double cf = xj_freeze(hc.gamma, hc.gm1, n, C);
+                    xj_stage(&hc, n, dir + 1, cf, 0.25 * lambda, 0, C, C, predict_only ? O : U, ws);
+                    if (!predict_only) {
+                        cf = xj_freeze(hc.gamma, hc.gm1, n, U);
+                        xj_stage(&hc, n, dir + 1, cf, 0.5 * lambda, 1, U, C, O, ws);
+                    }
+                } else {
+                    const int *ms = hll_slab[dir];
+                    double c[5];
+                    /* floors + pressure of Q (in place), HLL component order */
+                    for (i = 0; i < n; i++) {
+                        for (v = 0; v < 5; v++) c[v] = C[v][i];
+                        P[i] = hll_floor_pressure(&hc, c);
+                        for (v = 0; v < 5; v++) C[v][i] = c[v];
+                    }
+                    double *Hq[5] = {C[0], C[1], C[ms[0]], C[ms[1]], C[ms[2]]};
+                    hll_line_fluxes(&hc, n, Hq, P, FL, ws);
+                    for (v = 0; v < 5; v++)
+                        for (i = 0; i < n; i++) {
+                            int im = (i == 0) ? n - 1 : i - 1;
+                            H[v][i] = Hq[v][i] - lambda * (FL[v][i] - FL[v][im]);
+                        }
+                    /* U1 back to physical order, floors + pressure of U1 */
+                    for (i = 0; i < n; i++) {
+                        c[0] = H[0][i];
+                        c[1] = H[1][i];
+                        c[ms[0]] = H[2][i];
+                        c[ms[1]] = H[3][i];
+                        c[ms[2]] = H[4][i];
+                        P[i] = hll_floor_pressure(&hc, c);
+                        for (v = 0; v < 5; v++) U[v][i] = c[v];
+                    }
+                    double *Hu[5] = {U[0], U[1], U[ms[0]], U[ms[1]], U[ms[2]]};
+                    hll_line_fluxes(&hc, n, Hu, P, FL, ws);
+                    double *Ho[5] = {O[0], O[1], O[ms[0]], O[ms[1]], O[ms[2]]};
+                    for (v = 0; v < 5; v++)
+                        for (i = 0; i < n; i++) {
+                            int im = (i == 0) ? n - 1 : i - 1;
+                            Ho[v][i] = .5 * (Hq[v][i] + Hu[v][i] - lambda * (FL[v][i] - FL[v][im]));
+                        }
+                }
+                for (v = 0; v < 5; v++) {
+                    double *dst = q + (long)v * g->pitch + base;
+                    for (i = 0; i < n; i++) dst[(long)i * li.stride] = O[v][i];
+                }
+            }
+            free(buf);
+        }
+        return failed ? ORC_ENOMEM : ORC_OK;
+    }
 #pragma omp parallel
     {
         double *buf = (double *)malloc(sizeof(double) * (size_t)n * 35);
@@ -399,6 +745,17 @@ int orc_fluid_update_1d(double *q, const orc_grid *g, int dir, int method, doubl
         free(buf);
     }
     return failed ? ORC_ENOMEM : ORC_OK;
+}
+
+int orc_fluid_update_1d(double *q, const orc_grid *g, int dir, int method, double lambda,
+                        const orc_fluid_params *p)
+{
+    return update_1d(q, g, dir, method, lambda, p, 0);
+}
+
+int orc_xj_predict_1d(double *q, const orc_grid *g, int dir, double lambda, const orc_fluid_params *p)
+{
+    return update_1d(q, g, dir, ORC_XINJIN, lambda, p, 1);
 }
 
 /* ------------------------------------------------------------------------ */

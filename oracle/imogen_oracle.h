@@ -56,6 +56,11 @@ int  orc_get_threads(void);
 int orc_fluid_update_1d(double *q, const orc_grid *g, int dir, int method, double lambda,
                         const orc_fluid_params *p);
 
+/* Xin-Jin first stage only: freezing speed of Q, then the predictor
+ * U = floor(Q - lambda/4 dF(Q)) (gpuclass/cudaFluidStep.cu:380-424), written
+ * into q. Exported so the tests can pin the predictor on its own. */
+int orc_xj_predict_1d(double *q, const orc_grid *g, int dir, double lambda, const orc_fluid_params *p);
+
 /* Boundary conditions for direction dir on both faces. modes[6] = [-x,+x,-y,+y,-z,+z],
  * depth = halo depth (4). Follows setFluidBoundary (gpuclass/cudaStatics.cu:112-196). */
 int orc_set_fluid_boundary(double *q, const orc_grid *g, int dir, const int *modes, int depth);

@@ -48,6 +48,9 @@ def lib():
         L.orc_fluid_update_1d.argtypes = [dp, ctypes.POINTER(_Grid), ctypes.c_int, ctypes.c_int,
                                           ctypes.c_double, ctypes.POINTER(_Params)]
         L.orc_fluid_update_1d.restype = ctypes.c_int
+        L.orc_xj_predict_1d.argtypes = [dp, ctypes.POINTER(_Grid), ctypes.c_int, ctypes.c_double,
+                                        ctypes.POINTER(_Params)]
+        L.orc_xj_predict_1d.restype = ctypes.c_int
         L.orc_set_fluid_boundary.argtypes = [dp, ctypes.POINTER(_Grid), ctypes.c_int, ip, ctypes.c_int]
         L.orc_set_fluid_boundary.restype = ctypes.c_int
         L.orc_fluidstep.argtypes = [dp, ctypes.POINTER(_Grid), dp, ctypes.c_double, ctypes.c_int,
@@ -94,6 +97,15 @@ def fluid_update_1d(q, direction, lam, gamma, method=HLLC, limiter=LIM_OSPRE, rh
     rc = lib().orc_fluid_update_1d(_dptr(q), ctypes.byref(g), direction, method, lam, ctypes.byref(p))
     if rc:
         raise RuntimeError(f"orc_fluid_update_1d returned {rc}")
+
+
+def xj_predict_1d(q, direction, lam, gamma, rhomin=1e-8):
+    """Xin-Jin predictor only (freeze + U = floor(Q - lam/4 dF(Q)))."""
+    g = _grid(q)
+    p = _params(gamma, rhomin, -1.0, LIM_OSPRE)
+    rc = lib().orc_xj_predict_1d(_dptr(q), ctypes.byref(g), direction, lam, ctypes.byref(p))
+    if rc:
+        raise RuntimeError(f"orc_xj_predict_1d returned {rc}")
 
 
 def _modes(modes):

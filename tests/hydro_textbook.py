@@ -71,3 +71,46 @@ def first_order_heun_line(U, lam, g):
 
     U1 = U - lam * dF(U)
     return 0.5 * (U1 + U - lam * dF(U1))
+
+
+def einfeldt_speeds(UL, UR, g):
+    """Einfeldt's HLLE bounds from Roe averages of velocity and enthalpy."""
+    rl, vl, pl = prim(UL, g)
+    rr, vr, pr = prim(UR, g)
+    HL = (UL[1] + pl) / rl
+    HR = (UR[1] + pr) / rr
+    sl_, sr_ = np.sqrt(rl), np.sqrt(rr)
+    vt = (sl_ * vl + sr_ * vr) / (sl_ + sr_)
+    Ht = (sl_ * HL + sr_ * HR) / (sl_ + sr_)
+    ct = np.sqrt((g - 1) * (Ht - 0.5 * (vt ** 2).sum(axis=0)))
+    SL = np.minimum(vl[0] - np.sqrt(g * pl / rl), vt[0] - ct)
+    SR = np.maximum(vr[0] + np.sqrt(g * pr / rr), vt[0] + ct)
+    return SL, SR
+
+
+def hlle_textbook(UL, UR, g):
+    """HLL flux (Toro eq. 10.21) with Einfeldt speeds."""
+    SL, SR = einfeldt_speeds(UL, UR, g)
+    FL = phys_flux(UL, g)
+    FR = phys_flux(UR, g)
+    Fm = (SR * FL - SL * FR + SL * SR * (UR - UL)) / (SR - SL)
+    return np.where(SL >= 0, FL, np.where(SR <= 0, FR, Fm))
+
+
+def first_order_heun_line_with(flux, U, lam, g):
+    """SSP-RK2 with piecewise-constant states and the given two-state flux."""
+    def dF(V):
+        F = flux(V, np.roll(V, -1, axis=1), g)
+        return F - np.roll(F, 1, axis=1)
+
+    U1 = U - lam * dF(U)
+    return 0.5 * (U1 + U - lam * dF(U1))
+
+
+def rusanov_step(U, lam, g, c):
+    """One forward-Euler step with the local Lax-Friedrichs (Rusanov) flux
+    G_{i+1/2} = (F_i + F_{i+1})/2 - c (U_{i+1} - U_i)/2 and a fixed speed c."""
+    F = phys_flux(U, g)
+    Up, Fp = np.roll(U, -1, axis=1), np.roll(F, -1, axis=1)
+    G = 0.5 * (F + Fp) - 0.5 * c * (Up - U)
+    return U - lam * (G - np.roll(G, 1, axis=1))
