@@ -41,6 +41,10 @@ METHOD_HLL, METHOD_HLLC, METHOD_XINJIN = 1, 2, 3
 # SlopeLimiters
 (LIMITER_OSPRE, LIMITER_VANLEER, LIMITER_MINMOD, LIMITER_SUPERBEE, LIMITER_VANALBADA, LIMITER_MC,
  LIMITER_ZERO) = range(7)
+# DragMethods
+(DRAG_EMP, DRAG_RK4, DRAG_ETDRK1, DRAG_ETDRK2, DRAG_LOGTRAP2, DRAG_LOGTRAP3, DRAG_EXACT,
+ DRAG_LEARNED) = range(8)
+IMOGEN_MLP_NPARAMS = 4739
 
 c_int, c_double, c_int64, c_void_p = ctypes.c_int, ctypes.c_double, ctypes.c_int64, ctypes.c_void_p
 
@@ -85,15 +89,22 @@ class ParallelTopology(ctypes.Structure):
                 ("neighbor_left", c_int * 3), ("neighbor_right", c_int * 3), ("dimcomm", c_int * 3)]
 
 
+class DragModelNorm(ctypes.Structure):
+    _fields_ = [("fmean", ctypes.c_float * 5), ("fstd", ctypes.c_float * 5), ("tmean", ctypes.c_float * 3),
+                ("tstd", ctypes.c_float * 3), ("flo", ctypes.c_float * 5), ("fhi", ctypes.c_float * 5),
+                ("residTol", ctypes.c_float)]
+
+
 STRUCTS = {"BCSettings": BCSettings, "MGArray": MGArray, "GeometryParams": GeometryParams,
            "ThermoDetails": ThermoDetails, "FluidStepParams": FluidStepParams,
-           "ParallelTopology": ParallelTopology}
+           "ParallelTopology": ParallelTopology, "DragModelNorm": DragModelNorm}
 
 _P = ctypes.POINTER
 _DP = _P(c_double)
 _I64P = _P(c_int64)
 _IP = _P(c_int)
 _MGP = _P(MGArray)
+_FP = _P(ctypes.c_float)
 
 # name -> (restype, argtypes)
 SIGNATURES = {
@@ -124,6 +135,23 @@ SIGNATURES = {
     "calculateSoundspeed": (c_int, [_MGP, _MGP, _MGP, c_double]),
     "computeLocalCFLTimestep": (c_int, [_MGP, _MGP, _P(GeometryParams), c_int, _DP, _DP]),
     "imogen_cflTimestepFused": (c_int, [_MGP, _P(GeometryParams), c_int, c_double, _DP, _DP]),
+    "sourcefunction_2FluidDrag": (c_int, [_MGP, _MGP, _P(GeometryParams), _P(ThermoDetails),
+                                          _P(ThermoDetails), c_double, c_int]),
+    "imogen_drag_set_model": (c_int, [_FP, _P(DragModelNorm)]),
+    "imogen_drag_learned_stats": (c_int, [_I64P, _I64P]),
+    "imogen_drag_training_samples": (c_int, [_P(ThermoDetails), _P(ThermoDetails), c_int64, c_void_p, c_void_p,
+                                             c_void_p]),
+    "imogen_mlp_forward": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),
+    "imogen_mlp_backward": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "imogen_mlp_adam": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, ctypes.c_float, ctypes.c_float,
+                                ctypes.c_float, ctypes.c_float, c_int]),
+    "imogen_mlp_infer_bf16": (c_int, [c_int64, c_void_p, c_void_p]),
+    "imogen_mlp_mse_grad": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "imogen_gather_rows": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_void_p]),
+    "imogen_device_alloc": (c_int, [ctypes.c_size_t, _P(c_void_p)]),
+    "imogen_device_free": (c_int, [c_void_p]),
+    "imogen_memcpy_htod": (c_int, [c_void_p, c_void_p, ctypes.c_size_t]),
+    "imogen_memcpy_dtoh": (c_int, [c_void_p, c_void_p, ctypes.c_size_t]),
     "imogen_profile_enable": (c_int, [c_int]),
     "imogen_profile_read": (c_int, [c_int, _I64P, _DP]),
     "imogen_stream": (c_void_p, []),

@@ -1,0 +1,174 @@
+// drag_device.h -- per-cell arithmetic of the gas-dust drag source (FP64).
+//
+// Drag rate: cukern_GasDustDrag_GeneralLinearCore + allregimeCdrag
+// (gpuclass/cudaSource2FluidDrag.cu:819-890). Powers of the normalised
+// temperature share one logarithm: x^a = exp(a ln x) (within a few ulp of pow).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/imogen_abi.h"
+
+namespace imogen {
+
+// absolute tolerance on ln|dv| of the adaptive reference integrator
+constexpr double kDragExactTol = 1e-12;
+
+struct DragConsts {
+    double visc0, viscpow, lampow, alpha, beta, delta, epsilon, gm1;
+    double tol;
+};
+
+struct DragCell {
+    double rg, rd;   // gas and dust density
+    double s0, u0;   // |dv| and gas specific internal energy
+};
+
+__device__ __forceinline__ double pow_from_log(double lx, double a)
+{
+    return (a == 0.0) ? 1.0 : exp(a * lx);
+}
+
+// drag rate k(|dv|, u); no drag below |dv| = 1e-9 (:853-859)
+__device__ __forceinline__ double drag_k(const DragConsts &c, double rhog, double rhod, double magdv, double u)
+{
+    if (magdv < 1e-9) return 0.0;
+    const double lT = log(c.alpha * u);
+    const double Re = c.delta * rhog * magdv * pow_from_log(lT, c.viscpow);
+    const double Kn = c.beta * pow_from_log(lT, c.lampow) / rhog;
+    const double cunningham = 1 + Kn * (1.142 + 0.558 * exp(-0.999 / Kn));
+    const double Cd = (3 / Re + .45 * exp(-.319 * log(Re)) + .0509 * Re / (8710 + Re)) / cunningham;
+    return Cd * magdv * (rhog + rhod) * c.epsilon;
+}
+
+// |dv| and u_0 of a cell (cukern_prepareForERK_h0, :1883-1908)
+__device__ __forceinline__ DragCell drag_load(const double *g, const double *d, int64_t p)
+{
+    DragCell s;
+    s.rg = g[0];
+    s.rd = d[0];
+    const double E = g[p], gx = g[2 * p], gy = g[3 * p], gz = g[4 * p];
+    const double dx = d[2 * p], dy = d[3 * p], dz = d[4 * p];
+    double dv = gx / s.rg - dx / s.rd;
+    double dvsq = dv * dv;
+    dv = gy / s.rg - dy / s.rd;
+    dvsq += dv * dv;
+    dv = gz / s.rg - dz / s.rd;
+    dvsq += dv * dv;
+    s.s0 = sqrt(dvsq);
+    s.u0 = (E - .5 * (gx * gx + gy * gy + gz * gz) / s.rg) / s.rg;
+    return s;
+}
+
+// New momenta about the barycentric velocity, dv scaled by f, and the gas
+// energy change conserving total energy (:1632-1676).
+__device__ __forceinline__ void drag_apply(double *g, double *d, int64_t p, const DragCell &s, double f)
+{
+    const double mu = s.rd / (s.rg + s.rd) * s.rg;
+    const double rhoginv = 1.0 / s.rg;
+    double pdustsq = 0.0;
+#pragma unroll
+    for (int c = 2; c <= 4; c++) {
+        const double pg = g[c * p], pd = d[c * p];
+        pdustsq -= pd * pd;
+        const double vstick = (pg + pd) / (s.rg + s.rd);
+        const double dv_t = (pg * rhoginv - pd / s.rd) * f;
+        g[c * p] = s.rg * vstick + dv_t * mu;
+        const double q = s.rd * vstick - dv_t * mu;
+        d[c * p] = q;
+        pdustsq += q * q;
+    }
+    g[p] -= .5 * pdustsq / s.rd;
+}
+
+// path-averaged drag rate of LogTrap2 / LogTrap3 (cukern_LogTrapSolve, :1546-1630)
+__device__ __forceinline__ double drag_logtrap_rate(const DragConsts &c, double rhog, double rhod, double dv0,
+                                                    double u0, double t, int order)
+{
+    const double mu = rhod / (rhog + rhod);
+    double tmp0 = dv0, tmp1 = u0;
+    double ksum = drag_k(c, rhog, rhod, tmp0, tmp1);
+    double k = ksum;
+    double dv_t = dv0 * exp(-t * k);
+    double duint = -.5 * dv0 * dv0 * mu * expm1(-2 * k * t);
+    tmp0 = sqrt(dv_t * dv_t);
+    tmp1 += duint;
+    ksum += drag_k(c, rhog, rhod, tmp0, tmp1);
+    if (order == 3) {
+        tmp1 -= duint;
+        k = .25 * ksum;
+        dv_t = dv0 * exp(-t * k);
+        duint = -0.5 * dv0 * dv0 * mu * expm1(-2 * k * t);
+        tmp0 = sqrt(dv_t * dv_t);
+        tmp1 += duint;
+        const double khalf = drag_k(c, rhog, rhod, tmp0, tmp1);
+        return 0.16666666666666666667 * ksum + 0.66666666666666666667 * khalf;
+    }
+    return .5 * ksum;
+}
+
+// d ln|dv| / dt along the relaxation path, with the gas heated by the
+// dissipated relative kinetic energy
+__device__ __forceinline__ double drag_exact_rhs(const DragConsts &c, double rhog, double rhod, double s0sq,
+                                                 double u0, double h, double y)
+{
+    const double s = exp(y);
+    return -drag_k(c, rhog, rhod, s, u0 + h * (s0sq - s * s));
+}
+
+// |dv(t)|/|dv(0)|: adaptive Dormand-Prince 5(4) (J. Comput. Appl. Math. 6
+// (1980) 19) on y = ln|dv|, absolute tolerance c.tol on y, first trial step t.
+// *steps receives the number of steps tried (negative if the budget ran out).
+__device__ __forceinline__ double drag_exact_ratio(const DragConsts &c, double rhog, double rhod, double s0,
+                                                   double u0, double t, int *steps)
+{
+    *steps = 0;
+    if (s0 < 1e-9 || t <= 0) return 1.0;
+    const double hc = .5 * rhod / (rhog + rhod);
+    const double s0sq = s0 * s0;
+    const double y0 = log(s0);
+    double y = y0, tt = 0.0, h = t;
+    double k1 = drag_exact_rhs(c, rhog, rhod, s0sq, u0, hc, y);
+    int n = 0;
+    while (tt < t) {
+        if (n >= 4096) {
+            *steps = -n;
+            return exp(y - y0);
+        }
+        if (h > t - tt) h = t - tt;
+        const double k2 = drag_exact_rhs(c, rhog, rhod, s0sq, u0, hc, y + h * (1.0 / 5) * k1);
+        const double k3 = drag_exact_rhs(c, rhog, rhod, s0sq, u0, hc, y + h * ((3.0 / 40) * k1 + (9.0 / 40) * k2));
+        const double k4 = drag_exact_rhs(c, rhog, rhod, s0sq, u0, hc,
+                                         y + h * ((44.0 / 45) * k1 - (56.0 / 15) * k2 + (32.0 / 9) * k3));
+        const double k5 = drag_exact_rhs(c, rhog, rhod, s0sq, u0, hc,
+                                         y + h * ((19372.0 / 6561) * k1 - (25360.0 / 2187) * k2 +
+                                                  (64448.0 / 6561) * k3 - (212.0 / 729) * k4));
+        const double k6 = drag_exact_rhs(c, rhog, rhod, s0sq, u0, hc,
+                                         y + h * ((9017.0 / 3168) * k1 - (355.0 / 33) * k2 + (46732.0 / 5247) * k3 +
+                                                  (49.0 / 176) * k4 - (5103.0 / 18656) * k5));
+        const double yn = y + h * ((35.0 / 384) * k1 + (500.0 / 1113) * k3 + (125.0 / 192) * k4 -
+                                   (2187.0 / 6784) * k5 + (11.0 / 84) * k6);
+        const double k7 = drag_exact_rhs(c, rhog, rhod, s0sq, u0, hc, yn);
+        double err = h * ((71.0 / 57600) * k1 - (71.0 / 16695) * k3 + (71.0 / 1920) * k4 -
+                          (17253.0 / 339200) * k5 + (22.0 / 525) * k6 - (1.0 / 40) * k7);
+        err = fabs(err) / c.tol;
+        n++;
+        if (err <= 1.0) {
+            tt += h;
+            y = yn;
+            k1 = k7;
+        }
+        double f = (err > 0) ? 0.9 * pow(err, -0.2) : 5.0;
+        f = (f < 0.2) ? 0.2 : ((f > 5.0) ? 5.0 : f);
+        h *= f;
+    }
+    *steps = n;
+    return exp(y - y0);
+}
+
+DragConsts drag_consts(const ThermoDetails *tg, const ThermoDetails *td);
+int launch_drag(double *g, double *d, int64_t pitch, int64_t n, const DragConsts &c, double dt, int method,
+                unsigned long long *work, hipStream_t s);
+int launch_learned_drag(double *g, double *d, int64_t pitch, int64_t n, const DragConsts &c, double dt,
+                        hipStream_t s);
+
+}  // namespace imogen

@@ -12,14 +12,7 @@
 
 using namespace imogen;
 
-namespace {
-
-// fluxcall table of gpuclass/flux.cu:46: direction swept at position n of sweep s
-const int kFluxcall[3][6] = {{1, 2, 1, 2, 3, 3}, {3, 1, 2, 3, 1, 2}, {2, 3, 3, 1, 2, 1}};
-
-// Validates a 5-array fluid canister (slab views of one 5-slab owner) and
-// returns its base pointer and slab pitch in doubles.
-int fluid_view(MGArray *fluid, double **base, int64_t *pitch)
+int imogen::fluid_view(MGArray *fluid, double **base, int64_t *pitch)
 {
     if (!fluid) return IM_FAIL(ERROR_NULL_POINTER, "null fluid");
     if (fluid->nGPUs != 1)
@@ -36,6 +29,11 @@ int fluid_view(MGArray *fluid, double **base, int64_t *pitch)
     *pitch = p;
     return SUCCESSFUL;
 }
+
+namespace {
+
+// fluxcall table of gpuclass/flux.cu:46: direction swept at position n of sweep s
+const int kFluxcall[3][6] = {{1, 2, 1, 2, 3, 3}, {3, 1, 2, 3, 1, 2}, {2, 3, 3, 1, 2, 1}};
 
 int check_topology(const ParallelTopology *t)
 {

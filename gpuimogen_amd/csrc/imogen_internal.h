@@ -43,7 +43,11 @@ void prof_end(int slot, hipStream_t s);
 // Device scratch of at least `bytes`; contents are undefined on return.
 void *scratch(size_t bytes);
 
-// Launchers (fluid_sweep.hip, xinjin.hip, statics.hip, cfl.hip)
+// Validates a 5-array fluid canister (slab views of one 5-slab owner, XYZ
+// order, one GPU) and returns its base pointer and slab pitch in doubles.
+int fluid_view(MGArray *fluid, double **base, int64_t *pitch);
+
+// Launchers (fluid_sweep.hip, xinjin.hip, statics.hip, cfl.hip, drag.hip)
 // method: METHOD_HLLC or METHOD_HLL. rhomin, ciso: HLL floors and isothermal
 // sound speed (-1: adiabatic).
 int launch_fluid_sweep(double *base, int64_t pitch, int nx, int ny, int nz, int dir, int method, double lambda,

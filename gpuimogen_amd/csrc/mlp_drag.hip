@@ -1,0 +1,753 @@
+// mlp_drag.hip -- learned drag surrogate: BF16 MFMA inference fused into the
+// drag source with a per-cell residual check and exact fallback, plus the
+// FP32 kernels that train the network on the GPU.
+//
+// Network 5 -> 64 -> 64 -> 3 with GELU (tanh form). Inference keeps the cells
+// of a wave on the N side of v_mfma_f32_16x16x16_bf16 (16 cells per tile) and
+// the hidden units on M/K, so a layer's D tile is, lane for lane, the B
+// operand of the next layer: no data moves between layers, the FP32
+// accumulators only pass through GELU and a BF16 rounding. Biases enter as the
+// accumulator's initial value.
+//
+// Per cell the surrogate replaces the adaptive integrator of the relaxation
+// problem (drag_device.h). With k0 = k(|dv_0|, u_0), tau = dt k0 and the
+// analytic initial slope g = d ln k / d tau at tau = 0, the rates along the
+// exact path are written k/k0 = 1 + tau (c g + tau r) with c = 1/2 for the
+// mean rate kbar and the midpoint rate and c = 1 for the end rate; the network
+// predicts the three second-order remainders r (standardised). Cells whose
+// features leave the training box use r = 0. The cell is accepted when
+//   |dt (k(end state implied by kbar) - predicted end rate)|   and
+//   |dt (kbar - Simpson(k0, predicted mid rate, true end rate))|
+// are both below the model's residual tolerance, and its features lie inside
+// the training box; otherwise that cell runs the adaptive integrator.
+#include <cstring>
+#include <vector>
+
+#include "drag_device.h"
+#include "imogen_internal.h"
+
+namespace imogen {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kIn = 5, kHid = 64, kOut = 3;
+constexpr int kOffW1 = 0, kOffB1 = kIn * kHid, kOffW2 = kOffB1 + kHid, kOffB2 = kOffW2 + kHid * kHid,
+              kOffW3 = kOffB2 + kHid, kOffB3 = kOffW3 + kHid * kOut, kNParams = kOffB3 + kOut;
+static_assert(kNParams == IMOGEN_MLP_NPARAMS, "parameter layout");
+
+// Device copy of a model: BF16 transposed weights in MFMA A-operand order
+// (row = output unit, padded), FP32 biases (padded), and the standardisation.
+struct MlpDev {
+    const unsigned short *w1t;  // [64][16]
+    const unsigned short *w2t;  // [64][64]
+    const unsigned short *w3t;  // [16][64]
+    const float *b1, *b2, *b3;  // [64], [64], [16]
+    DragModelNorm norm;
+};
+
+__device__ __forceinline__ unsigned short f2bf(float f)
+{
+    // round to nearest even
+    unsigned u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+__device__ __forceinline__ float bf2f(unsigned short b)
+{
+    return __uint_as_float((unsigned)b << 16);
+}
+
+// GELU, tanh form: 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3))), with
+// tanh(a) = 1 - 2 / (exp(2a) + 1)
+__device__ __forceinline__ float gelu_f(float x)
+{
+    const float a = 0.7978845608028654f * (x + 0.044715f * x * x * x);
+    const float t = 1.0f - 2.0f / (__expf(2.0f * a) + 1.0f);
+    return 0.5f * x * (1.0f + t);
+}
+
+__device__ __forceinline__ float gelu_grad(float x)
+{
+    const float a = 0.7978845608028654f * (x + 0.044715f * x * x * x);
+    const float t = 1.0f - 2.0f / (__expf(2.0f * a) + 1.0f);
+    return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * 0.7978845608028654f * (1.0f + 3.0f * 0.044715f * x * x);
+}
+
+__device__ __forceinline__ bf16x4 load_frag(const unsigned short *w, int ld, int row, int col)
+{
+    const uint2 v = *reinterpret_cast<const uint2 *>(w + (int64_t)row * ld + col);
+    bf16x4 r;
+    r[0] = (short)(v.x & 0xffff);
+    r[1] = (short)(v.x >> 16);
+    r[2] = (short)(v.y & 0xffff);
+    r[3] = (short)(v.y >> 16);
+    return r;
+}
+
+__device__ __forceinline__ bf16x4 act_to_b(const f32x4 &d)
+{
+    bf16x4 r;
+    r[0] = (short)f2bf(gelu_f(d[0]));
+    r[1] = (short)f2bf(gelu_f(d[1]));
+    r[2] = (short)f2bf(gelu_f(d[2]));
+    r[3] = (short)f2bf(gelu_f(d[3]));
+    return r;
+}
+
+// Network outputs for the 64 cells of a wave: every lane passes its own
+// standardised features f[5] and receives its own outputs. Must be called by
+// all 64 lanes together.
+__device__ __forceinline__ void mlp_wave(const MlpDev &m, const float (&f)[5], float (&o)[3])
+{
+    const int lane = threadIdx.x & 63;
+    const int q = lane >> 4;   // which group of 4 K (or M) rows this lane holds
+    const int col = lane & 15; // which of the 16 cells of a tile
+    float res[4][3];
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+        // B operand of layer 1: features 4q..4q+3 of cell 16g + col (K padded to 16)
+        const int src = 16 * g + col;
+        float x[5];
+#pragma unroll
+        for (int j = 0; j < 5; j++) x[j] = __shfl(f[j], src, 64);
+        bf16x4 xb;
+        xb[0] = (short)f2bf(q == 0 ? x[0] : (q == 1 ? x[4] : 0.0f));
+        xb[1] = (short)f2bf(q == 0 ? x[1] : 0.0f);
+        xb[2] = (short)f2bf(q == 0 ? x[2] : 0.0f);
+        xb[3] = (short)f2bf(q == 0 ? x[3] : 0.0f);
+        // layer 1: 4 tiles of 16 hidden units
+        bf16x4 h1[4];
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) {
+            const int u = 16 * mt + 4 * q;
+            f32x4 acc = {m.b1[u], m.b1[u + 1], m.b1[u + 2], m.b1[u + 3]};
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(load_frag(m.w1t, 16, 16 * mt + col, 4 * q), xb, acc, 0, 0,
+                                                           0);
+            h1[mt] = act_to_b(acc);
+        }
+        // layer 2
+        bf16x4 h2[4];
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) {
+            const int u = 16 * mt + 4 * q;
+            f32x4 acc = {m.b2[u], m.b2[u + 1], m.b2[u + 2], m.b2[u + 3]};
+#pragma unroll
+            for (int kt = 0; kt < 4; kt++)
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(load_frag(m.w2t, 64, 16 * mt + col, 16 * kt + 4 * q),
+                                                               h1[kt], acc, 0, 0, 0);
+            h2[mt] = act_to_b(acc);
+        }
+        // layer 3: one tile, outputs 0..2 in rows 0..2 (lanes 0..15)
+        f32x4 acc = {m.b3[4 * q], m.b3[4 * q + 1], m.b3[4 * q + 2], m.b3[4 * q + 3]};
+#pragma unroll
+        for (int kt = 0; kt < 4; kt++)
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(load_frag(m.w3t, 64, col, 16 * kt + 4 * q), h2[kt], acc,
+                                                           0, 0, 0);
+        res[g][0] = acc[0];
+        res[g][1] = acc[1];
+        res[g][2] = acc[2];
+    }
+    // cell `lane` = column lane%16 of group lane/16, held by lane lane%16
+    const int g = lane >> 4;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const float v0 = __shfl(res[0][j], col, 64);
+        const float v1 = __shfl(res[1][j], col, 64);
+        const float v2 = __shfl(res[2][j], col, 64);
+        const float v3 = __shfl(res[3][j], col, 64);
+        o[j] = (g == 0) ? v0 : ((g == 1) ? v1 : ((g == 2) ? v2 : v3));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// features of a cell
+// ---------------------------------------------------------------------------
+struct DragRate {
+    double k, Re, Kn;
+    double g;  // d ln k / d tau at tau = 0 along the relaxation path
+};
+
+// k0, Re0, Kn0 and the initial slope. With k = C_d(Re, Kn) |dv| (rho_g +
+// rho_d) eps, Re ~ |dv| T^viscpow, Kn ~ T^lampow, T ~ u, and along the path
+// d ln|dv|/d tau = -1, d ln u/d tau = (rho_d/(rho_g+rho_d)) |dv_0|^2/u_0:
+//   g = -(1 + a) + (a viscpow + b lampow) rho_d |dv_0|^2 / ((rho_g+rho_d) u_0)
+// with a = d ln N/d ln Re, b = -d ln Cu/d ln Kn for C_d = N(Re)/Cu(Kn).
+__device__ __forceinline__ DragRate drag_k_parts(const DragConsts &c, double rhog, double rhod, double magdv,
+                                                 double u)
+{
+    DragRate r;
+    const double lT = log(c.alpha * u);
+    r.Re = c.delta * rhog * magdv * pow_from_log(lT, c.viscpow);
+    r.Kn = c.beta * pow_from_log(lT, c.lampow) / rhog;
+    const double E = exp(-0.999 / r.Kn);
+    const double cunningham = 1 + r.Kn * (1.142 + 0.558 * E);
+    const double p319 = exp(-.319 * log(r.Re));
+    const double N = 3 / r.Re + .45 * p319 + .0509 * r.Re / (8710 + r.Re);
+    const double Cd = N / cunningham;
+    r.k = Cd * magdv * (rhog + rhod) * c.epsilon;
+    const double q = 8710 + r.Re;
+    const double a = (-3 / r.Re - .319 * .45 * p319 + .0509 * 8710 * r.Re / (q * q)) / N;
+    const double b = -(1.142 * r.Kn + 0.558 * E * (r.Kn + 0.999)) / cunningham;
+    r.g = -(1.0 + a) + (a * c.viscpow + b * c.lampow) * rhod * magdv * magdv / ((rhog + rhod) * u);
+    return r;
+}
+
+// raw features: ln(dt k0), ln(rho_d/rho_g), ln(|dv|/c_s), ln Re0, ln Kn0
+__device__ __forceinline__ void drag_features(const DragConsts &c, const DragCell &s, const DragRate &r, double dt,
+                                              float (&f)[5])
+{
+    const double cs = sqrt((c.gm1 + 1.0) * c.gm1 * s.u0);
+    f[0] = __logf((float)(dt * r.k));
+    f[1] = __logf((float)(s.rd / s.rg));
+    f[2] = __logf((float)(s.s0 / cs));
+    f[3] = __logf((float)r.Re);
+    f[4] = __logf((float)r.Kn);
+}
+
+// ---------------------------------------------------------------------------
+// learned drag source
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void learned_drag_kernel(double *__restrict__ g, double *__restrict__ d,
+                                                           int64_t pitch, int64_t n, DragConsts c, double dt,
+                                                           MlpDev m, unsigned long long *__restrict__ counts)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    unsigned long long fallbacks = 0;
+    for (int64_t base = wave * 64; base < n; base += nwaves * 64) {
+        const int64_t i = base + lane;
+        const bool live = i < n;
+        const int64_t ic = live ? i : n - 1;  // lanes past the end work on a valid cell and store nothing
+        const DragCell s = drag_load(g + ic, d + ic, pitch);
+        const bool moving = s.s0 >= 1e-9;
+        DragRate r0 = {0.0, 1.0, 1.0, 0.0};
+        if (moving) r0 = drag_k_parts(c, s.rg, s.rd, s.s0, s.u0);
+        float f[5];
+        drag_features(c, s, r0, dt, f);
+        bool inside = moving;
+        float z[5];
+#pragma unroll
+        for (int j = 0; j < 5; j++) {
+            inside = inside && (f[j] >= m.norm.flo[j]) && (f[j] <= m.norm.fhi[j]);
+            z[j] = inside ? (f[j] - m.norm.fmean[j]) / m.norm.fstd[j] : 0.0f;
+        }
+        float o[3];
+        mlp_wave(m, z, o);
+        double fac = 1.0;
+        if (moving) {
+            const double tau = dt * r0.k;
+            const double q0 = inside ? (double)(o[0] * m.norm.tstd[0] + m.norm.tmean[0]) : 0.0;
+            const double q1 = inside ? (double)(o[1] * m.norm.tstd[1] + m.norm.tmean[1]) : 0.0;
+            const double q2 = inside ? (double)(o[2] * m.norm.tstd[2] + m.norm.tmean[2]) : 0.0;
+            const double kbar = r0.k * (1.0 + tau * (.5 * r0.g + tau * q0));
+            const double kmid = r0.k * (1.0 + tau * (.5 * r0.g + tau * q1));
+            const double kend = r0.k * (1.0 + tau * (r0.g + tau * q2));
+            // end state implied by kbar; its true rate checks the prediction
+            const double s1 = s.s0 * exp(-kbar * dt);
+            const double u1 = s.u0 + .5 * s.rd / (s.rg + s.rd) * (s.s0 * s.s0 - s1 * s1);
+            const double k1 = drag_k(c, s.rg, s.rd, s1, u1);
+            const double ksimp = (r0.k + 4.0 * kmid + k1) * (1.0 / 6.0);
+            const double e1 = fabs(dt * (k1 - kend));
+            const double e2 = fabs(dt * (kbar - ksimp));
+            const bool ok = (e1 <= m.norm.residTol) && (e2 <= m.norm.residTol) && kbar > 0.0;
+            if (ok) {
+                fac = exp(-kbar * dt);
+            } else {
+                int steps;
+                fac = drag_exact_ratio(c, s.rg, s.rd, s.s0, s.u0, dt, &steps);
+                fallbacks += live ? 1 : 0;
+            }
+        }
+        if (live) drag_apply(g + i, d + i, pitch, s, fac);
+    }
+    if (counts) {
+        for (int off = 32; off > 0; off >>= 1) fallbacks += __shfl_xor(fallbacks, off, 64);
+        if (lane == 0 && fallbacks) atomicAdd(counts + 1, fallbacks);
+    }
+}
+
+// standalone BF16 inference (tests)
+__global__ __launch_bounds__(256) void mlp_infer_kernel(int64_t batch, const float *__restrict__ x,
+                                                        float *__restrict__ y, MlpDev m)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t base = wave * 64; base < batch; base += nwaves * 64) {
+        const int64_t i = base + lane;
+        const int64_t ic = i < batch ? i : batch - 1;
+        float f[5], o[3];
+#pragma unroll
+        for (int j = 0; j < 5; j++) f[j] = x[ic * 5 + j];
+        mlp_wave(m, f, o);
+        if (i < batch) {
+#pragma unroll
+            for (int j = 0; j < 3; j++) y[i * 3 + j] = o[j];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// training set from the exact integrator
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void drag_samples_kernel(int64_t n, const double *__restrict__ cells, DragConsts c,
+                                                           float *__restrict__ feat, float *__restrict__ targ)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        DragCell s;
+        s.rg = cells[i * 5 + 0];
+        s.rd = cells[i * 5 + 1];
+        s.s0 = cells[i * 5 + 2];
+        s.u0 = cells[i * 5 + 3];
+        const double tau_in = cells[i * 5 + 4];
+        float f[5];
+        float t[3];
+        const DragRate r0 = (s.s0 < 1e-9) ? DragRate{0.0, 1.0, 1.0, 0.0} : drag_k_parts(c, s.rg, s.rd, s.s0, s.u0);
+        const double dt = (r0.k > 0.0) ? tau_in / r0.k : 0.0;
+        if (s.s0 < 1e-9 || !(dt > 0)) {
+#pragma unroll
+            for (int j = 0; j < 5; j++) f[j] = __int_as_float(0x7fc00000);
+            t[0] = t[1] = t[2] = __int_as_float(0x7fc00000);
+        } else {
+            drag_features(c, s, r0, dt, f);
+            int st;
+            const double hc = .5 * s.rd / (s.rg + s.rd);
+            const double rend = drag_exact_ratio(c, s.rg, s.rd, s.s0, s.u0, dt, &st);
+            const double rmid = drag_exact_ratio(c, s.rg, s.rd, s.s0, s.u0, .5 * dt, &st);
+            const double smid = s.s0 * rmid, send = s.s0 * rend;
+            const double kmid = drag_k(c, s.rg, s.rd, smid, s.u0 + hc * (s.s0 * s.s0 - smid * smid));
+            const double kend = drag_k(c, s.rg, s.rd, send, s.u0 + hc * (s.s0 * s.s0 - send * send));
+            const double kbar = -log(rend) / dt;
+            const double tau = dt * r0.k;
+            t[0] = (float)(((kbar / r0.k - 1.0) / tau - .5 * r0.g) / tau);
+            t[1] = (float)(((kmid / r0.k - 1.0) / tau - .5 * r0.g) / tau);
+            t[2] = (float)(((kend / r0.k - 1.0) / tau - r0.g) / tau);
+        }
+#pragma unroll
+        for (int j = 0; j < 5; j++) feat[i * 5 + j] = f[j];
+#pragma unroll
+        for (int j = 0; j < 3; j++) targ[i * 3 + j] = t[j];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 training kernels: 64 rows per block of 256 threads
+// ---------------------------------------------------------------------------
+constexpr int kRows = 64;
+
+__global__ __launch_bounds__(256) void mlp_fwd_kernel(const float *__restrict__ P, int64_t batch,
+                                                      const float *__restrict__ x, float *__restrict__ act,
+                                                      float *__restrict__ y)
+{
+    __shared__ float W2[kHid][kHid];
+    __shared__ float H[kRows][kHid + 1];
+    const int t = threadIdx.x;
+    const int64_t r0 = (int64_t)blockIdx.x * kRows;
+    for (int k = t; k < kHid * kHid; k += 256) W2[k / kHid][k % kHid] = P[kOffW2 + k];
+    const int r = t >> 2;          // row within the block
+    const int j0 = (t & 3) * 16;   // 16 hidden units per thread
+    const int64_t row = r0 + r;
+    const bool live = row < batch;
+    float xv[kIn];
+#pragma unroll
+    for (int k = 0; k < kIn; k++) xv[k] = live ? x[row * kIn + k] : 0.0f;
+    float *a = act + (live ? row : 0) * 256;
+#pragma unroll
+    for (int jj = 0; jj < 16; jj++) {
+        const int j = j0 + jj;
+        float z = P[kOffB1 + j];
+#pragma unroll
+        for (int k = 0; k < kIn; k++) z += xv[k] * P[kOffW1 + k * kHid + j];
+        const float h = gelu_f(z);
+        H[r][j] = h;
+        if (live) {
+            a[j] = z;
+            a[64 + j] = h;
+        }
+    }
+    __syncthreads();
+    float z2[16];
+#pragma unroll
+    for (int jj = 0; jj < 16; jj++) z2[jj] = P[kOffB2 + j0 + jj];
+    for (int k = 0; k < kHid; k++) {
+        const float h = H[r][k];
+#pragma unroll
+        for (int jj = 0; jj < 16; jj++) z2[jj] += h * W2[k][j0 + jj];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int jj = 0; jj < 16; jj++) {
+        const float h = gelu_f(z2[jj]);
+        H[r][j0 + jj] = h;
+        if (live) {
+            a[128 + j0 + jj] = z2[jj];
+            a[192 + j0 + jj] = h;
+        }
+    }
+    __syncthreads();
+    if (t < kRows * kOut) {
+        const int rr = t / kOut, o = t % kOut;
+        const int64_t rw = r0 + rr;
+        float v = P[kOffB3 + o];
+        for (int k = 0; k < kHid; k++) v += H[rr][k] * P[kOffW3 + k * kOut + o];
+        if (rw < batch) y[rw * kOut + o] = v;
+    }
+}
+
+// Per-block gradient partials (deterministic); reduced by mlp_grad_reduce.
+__global__ __launch_bounds__(256) void mlp_bwd_kernel(const float *__restrict__ P, int64_t batch,
+                                                      const float *__restrict__ x, const float *__restrict__ act,
+                                                      const float *__restrict__ dy, float *__restrict__ part)
+{
+    __shared__ float W2[kHid][kHid];
+    __shared__ float DZ2[kRows][kHid + 1];
+    __shared__ float DZ1[kRows][kHid + 1];
+    __shared__ float DY[kRows][kOut];
+    const int t = threadIdx.x;
+    const int64_t r0 = (int64_t)blockIdx.x * kRows;
+    for (int k = t; k < kHid * kHid; k += 256) W2[k / kHid][k % kHid] = P[kOffW2 + k];
+    if (t < kRows * kOut) {
+        const int rr = t / kOut, o = t % kOut;
+        DY[rr][o] = (r0 + rr < batch) ? dy[(r0 + rr) * kOut + o] : 0.0f;
+    }
+    __syncthreads();
+    const int r = t >> 2;
+    const int j0 = (t & 3) * 16;
+    const int64_t row = r0 + r;
+    const bool live = row < batch;
+    const float *a = act + (live ? row : 0) * 256;
+    // dz2 = (dy W3^T) * gelu'(z2)
+#pragma unroll
+    for (int jj = 0; jj < 16; jj++) {
+        const int j = j0 + jj;
+        float dh = 0.0f;
+#pragma unroll
+        for (int o = 0; o < kOut; o++) dh += DY[r][o] * P[kOffW3 + j * kOut + o];
+        DZ2[r][j] = live ? dh * gelu_grad(a[128 + j]) : 0.0f;
+    }
+    __syncthreads();
+    // dz1 = (dz2 W2^T) * gelu'(z1)
+#pragma unroll
+    for (int jj = 0; jj < 16; jj++) {
+        const int k = j0 + jj;
+        float dh = 0.0f;
+        for (int j = 0; j < kHid; j++) dh += DZ2[r][j] * W2[k][j];
+        DZ1[r][k] = live ? dh * gelu_grad(a[k]) : 0.0f;
+    }
+    __syncthreads();
+    float *pp = part + (int64_t)blockIdx.x * kNParams;
+    // dW2[k][j] = sum_r h1[r][k] dz2[r][j]: thread owns 16 (k, j) pairs
+    for (int e = t; e < kHid * kHid; e += 256) {
+        const int k = e / kHid, j = e % kHid;
+        float s = 0.0f;
+        for (int rr = 0; rr < kRows; rr++) {
+            const int64_t rw = r0 + rr;
+            if (rw < batch) s += act[rw * 256 + 64 + k] * DZ2[rr][j];
+        }
+        pp[kOffW2 + e] = s;
+    }
+    // dW1[k][j] = sum_r x[r][k] dz1[r][j]; db1, db2
+    for (int e = t; e < kIn * kHid; e += 256) {
+        const int k = e / kHid, j = e % kHid;
+        float s = 0.0f;
+        for (int rr = 0; rr < kRows; rr++) {
+            const int64_t rw = r0 + rr;
+            if (rw < batch) s += x[rw * kIn + k] * DZ1[rr][j];
+        }
+        pp[kOffW1 + e] = s;
+    }
+    if (t < kHid) {
+        float s1 = 0.0f, s2 = 0.0f;
+        for (int rr = 0; rr < kRows; rr++) {
+            s1 += DZ1[rr][t];
+            s2 += DZ2[rr][t];
+        }
+        pp[kOffB1 + t] = s1;
+        pp[kOffB2 + t] = s2;
+    }
+    // dW3[k][o] = sum_r h2[r][k] dy[r][o]; db3
+    if (t < kHid * kOut) {
+        const int k = t / kOut, o = t % kOut;
+        float s = 0.0f;
+        for (int rr = 0; rr < kRows; rr++) {
+            const int64_t rw = r0 + rr;
+            if (rw < batch) s += act[rw * 256 + 192 + k] * DY[rr][o];
+        }
+        pp[kOffW3 + t] = s;
+    }
+    if (t < kOut) {
+        float s = 0.0f;
+        for (int rr = 0; rr < kRows; rr++) s += DY[rr][t];
+        pp[kOffB3 + t] = s;
+    }
+}
+
+__global__ __launch_bounds__(256) void mlp_grad_reduce(const float *__restrict__ part, int nblocks,
+                                                       float *__restrict__ grads)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= kNParams) return;
+    float s = 0.0f;
+    for (int b = 0; b < nblocks; b++) s += part[(int64_t)b * kNParams + e];
+    grads[e] = s;
+}
+
+__global__ __launch_bounds__(256) void mlp_adam_kernel(float *__restrict__ p, const float *__restrict__ gr,
+                                                       float *__restrict__ m, float *__restrict__ v, int64_t n,
+                                                       float lr, float b1, float b2, float eps, float c1, float c2)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float g = gr[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * g;
+    const float vi = b2 * v[i] + (1.0f - b2) * g * g;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+__global__ __launch_bounds__(256) void mlp_mse_grad_kernel(int64_t n, const float *__restrict__ y,
+                                                           const float *__restrict__ t, float *__restrict__ dy,
+                                                           float scale, double *__restrict__ loss)
+{
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float e = y[i] - t[i];
+        dy[i] = 2.0f * scale * e;
+        acc += (double)e * (double)e;
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if ((threadIdx.x & 63) == 0 && acc != 0.0) atomicAdd(loss, acc * (double)scale);
+}
+
+__global__ __launch_bounds__(256) void gather_rows_kernel(const float *__restrict__ src,
+                                                          const int64_t *__restrict__ idx, int64_t nrows, int ncols,
+                                                          float *__restrict__ dst)
+{
+    const int64_t total = nrows * ncols;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = e / ncols;
+        const int c = (int)(e - r * ncols);
+        dst[e] = src[idx[r] * ncols + c];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct HostModel {
+    void *buf = nullptr;  // device: w1t | w2t | w3t | b1 | b2 | b3
+    MlpDev dev{};
+    bool loaded = false;
+    unsigned long long *counts = nullptr;  // device [unused, fallbacks]
+};
+static HostModel g_model;
+static int64_t g_learnedCells = 0;  // cells through the learned drag since the last stats read
+
+static int ensure_counts()
+{
+    if (g_model.counts) return SUCCESSFUL;
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    if (IM_HIP(hipMalloc(&g_model.counts, 2 * sizeof(unsigned long long)))) return ERROR_CUDA_BLEW_UP;
+    if (IM_HIP(hipMemsetAsync(g_model.counts, 0, 2 * sizeof(unsigned long long), st->stream)))
+        return ERROR_CUDA_BLEW_UP;
+    return IM_HIP(hipStreamSynchronize(st->stream));
+}
+
+int launch_learned_drag(double *g, double *d, int64_t pitch, int64_t n, const DragConsts &c, double dt,
+                        hipStream_t s)
+{
+    if (!g_model.loaded) return IM_FAIL(ERROR_INVALID_ARGS, "learned drag: no model (imogen_drag_set_model)");
+    if (ensure_counts()) return ERROR_CUDA_BLEW_UP;
+    int64_t waves = (n + 63) / 64;
+    if (waves > 8192) waves = 8192;
+    const int64_t blocks = (waves + 3) / 4;
+    const int slot = prof_begin(PROF_MLP, s);
+    hipLaunchKernelGGL(learned_drag_kernel, dim3((unsigned)blocks), dim3(256), 0, s, g, d, pitch, n, c, dt,
+                       g_model.dev, g_model.counts);
+    prof_end(slot, s);
+    int rc = IM_LAUNCHED("learned_drag_kernel");
+    if (rc) return rc;
+    g_learnedCells += n;
+    return SUCCESSFUL;
+}
+
+}  // namespace imogen
+
+using namespace imogen;
+
+extern "C" {
+
+int imogen_drag_set_model(const float *params, const DragModelNorm *norm)
+{
+    if (!params || !norm) return IM_FAIL(ERROR_NULL_POINTER, "null model");
+    for (int j = 0; j < 5; j++)
+        if (!(norm->fstd[j] > 0)) return IM_FAIL(ERROR_INVALID_ARGS, "feature std %d must be positive", j);
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    // host-side layout of the device block
+    const size_t n1 = 64 * 16, n2 = 64 * 64, n3 = 16 * 64;
+    std::vector<unsigned short> w(n1 + n2 + n3, 0);
+    auto bf = [](float f) {
+        unsigned u;
+        memcpy(&u, &f, 4);
+        if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)0x7fc0;
+        u += 0x7fffu + ((u >> 16) & 1u);
+        return (unsigned short)(u >> 16);
+    };
+    for (int o = 0; o < 64; o++)
+        for (int i = 0; i < kIn; i++) w[o * 16 + i] = bf(params[kOffW1 + i * kHid + o]);
+    for (int o = 0; o < 64; o++)
+        for (int i = 0; i < 64; i++) w[n1 + o * 64 + i] = bf(params[kOffW2 + i * kHid + o]);
+    for (int o = 0; o < kOut; o++)
+        for (int i = 0; i < 64; i++) w[n1 + n2 + o * 64 + i] = bf(params[kOffW3 + i * kOut + o]);
+    std::vector<float> b(64 + 64 + 16, 0.0f);
+    for (int j = 0; j < 64; j++) b[j] = params[kOffB1 + j];
+    for (int j = 0; j < 64; j++) b[64 + j] = params[kOffB2 + j];
+    for (int j = 0; j < kOut; j++) b[128 + j] = params[kOffB3 + j];
+    const size_t wbytes = w.size() * sizeof(unsigned short), bbytes = b.size() * sizeof(float);
+    if (!g_model.buf && IM_HIP(hipMalloc(&g_model.buf, wbytes + bbytes))) return ERROR_CUDA_BLEW_UP;
+    // ordered after any learned-drag kernel still reading the previous model
+    if (IM_HIP(hipMemcpyAsync(g_model.buf, w.data(), wbytes, hipMemcpyHostToDevice, st->stream)))
+        return ERROR_CUDA_BLEW_UP;
+    if (IM_HIP(hipMemcpyAsync((char *)g_model.buf + wbytes, b.data(), bbytes, hipMemcpyHostToDevice, st->stream)))
+        return ERROR_CUDA_BLEW_UP;
+    if (IM_HIP(hipStreamSynchronize(st->stream))) return ERROR_CUDA_BLEW_UP;
+    const unsigned short *wd = (const unsigned short *)g_model.buf;
+    const float *bd = (const float *)((char *)g_model.buf + wbytes);
+    g_model.dev.w1t = wd;
+    g_model.dev.w2t = wd + n1;
+    g_model.dev.w3t = wd + n1 + n2;
+    g_model.dev.b1 = bd;
+    g_model.dev.b2 = bd + 64;
+    g_model.dev.b3 = bd + 128;
+    g_model.dev.norm = *norm;
+    g_model.loaded = true;
+    return SUCCESSFUL;
+}
+
+int imogen_drag_learned_stats(int64_t *cells, int64_t *fallbacks)
+{
+    if (ensure_counts()) return ERROR_CUDA_BLEW_UP;
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    unsigned long long h[2];
+    // read and clear in order with the kernels on the library stream
+    if (IM_HIP(hipMemcpyAsync(h, g_model.counts, sizeof(h), hipMemcpyDeviceToHost, st->stream)))
+        return ERROR_CUDA_BLEW_UP;
+    if (IM_HIP(hipMemsetAsync(g_model.counts, 0, sizeof(h), st->stream))) return ERROR_CUDA_BLEW_UP;
+    if (IM_HIP(hipStreamSynchronize(st->stream))) return ERROR_CUDA_BLEW_UP;
+    if (cells) *cells = g_learnedCells;
+    if (fallbacks) *fallbacks = (int64_t)h[1];
+    g_learnedCells = 0;
+    return SUCCESSFUL;
+}
+
+int imogen_drag_training_samples(const ThermoDetails *gas, const ThermoDetails *dust, int64_t n,
+                                 const double *cells, float *features, float *targets)
+{
+    if (!gas || !dust || !cells || !features || !targets) return IM_FAIL(ERROR_NULL_POINTER, "null argument");
+    if (n <= 0) return SUCCESSFUL;
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    DragConsts c = drag_consts(gas, dust);
+    // targets are second-order remainders (divided by tau^3 relative to ln r):
+    // integrate the training paths well below the product tolerance
+    c.tol = 1e-14;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(drag_samples_kernel, dim3((unsigned)blocks), dim3(256), 0, st->stream, n, cells, c, features,
+                       targets);
+    return IM_LAUNCHED("drag_samples_kernel");
+}
+
+int imogen_mlp_forward(const float *params, int64_t batch, const float *x, float *act, float *y)
+{
+    if (!params || !x || !act || !y) return IM_FAIL(ERROR_NULL_POINTER, "null argument");
+    if (batch <= 0) return SUCCESSFUL;
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    const int64_t blocks = (batch + kRows - 1) / kRows;
+    hipLaunchKernelGGL(mlp_fwd_kernel, dim3((unsigned)blocks), dim3(256), 0, st->stream, params, batch, x, act, y);
+    return IM_LAUNCHED("mlp_fwd_kernel");
+}
+
+int imogen_mlp_backward(const float *params, int64_t batch, const float *x, const float *act, const float *dy,
+                        float *grads)
+{
+    if (!params || !x || !act || !dy || !grads) return IM_FAIL(ERROR_NULL_POINTER, "null argument");
+    if (batch <= 0) return SUCCESSFUL;
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    const int64_t blocks = (batch + kRows - 1) / kRows;
+    float *part = (float *)scratch((size_t)blocks * kNParams * sizeof(float));
+    if (!part) return IM_FAIL(ERROR_NOMEM, "no scratch for gradient partials");
+    hipLaunchKernelGGL(mlp_bwd_kernel, dim3((unsigned)blocks), dim3(256), 0, st->stream, params, batch, x, act, dy,
+                       part);
+    int rc = IM_LAUNCHED("mlp_bwd_kernel");
+    if (rc) return rc;
+    hipLaunchKernelGGL(mlp_grad_reduce, dim3((kNParams + 255) / 256), dim3(256), 0, st->stream, part, (int)blocks,
+                       grads);
+    return IM_LAUNCHED("mlp_grad_reduce");
+}
+
+int imogen_mlp_adam(float *params, const float *grads, float *m, float *v, int64_t n, float lr, float beta1,
+                    float beta2, float eps, int step)
+{
+    if (!params || !grads || !m || !v) return IM_FAIL(ERROR_NULL_POINTER, "null argument");
+    if (step < 1) return IM_FAIL(ERROR_INVALID_ARGS, "Adam step counts from 1");
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    const float c1 = 1.0f - powf(beta1, (float)step), c2 = 1.0f - powf(beta2, (float)step);
+    hipLaunchKernelGGL(mlp_adam_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st->stream, params, grads,
+                       m, v, n, lr, beta1, beta2, eps, c1, c2);
+    return IM_LAUNCHED("mlp_adam_kernel");
+}
+
+int imogen_mlp_mse_grad(int64_t batch, const float *y, const float *t, float *dy, double *lossAccum)
+{
+    if (!y || !t || !dy || !lossAccum) return IM_FAIL(ERROR_NULL_POINTER, "null argument");
+    if (batch <= 0) return SUCCESSFUL;
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    const int64_t n = batch * kOut;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(mlp_mse_grad_kernel, dim3((unsigned)blocks), dim3(256), 0, st->stream, n, y, t, dy,
+                       1.0f / (float)n, lossAccum);
+    return IM_LAUNCHED("mlp_mse_grad_kernel");
+}
+
+int imogen_gather_rows(const float *src, const int64_t *idx, int64_t nrows, int ncols, float *dst)
+{
+    if (!src || !idx || !dst) return IM_FAIL(ERROR_NULL_POINTER, "null argument");
+    if (nrows <= 0 || ncols <= 0) return SUCCESSFUL;
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    int64_t blocks = (nrows * ncols + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(gather_rows_kernel, dim3((unsigned)blocks), dim3(256), 0, st->stream, src, idx, nrows, ncols,
+                       dst);
+    return IM_LAUNCHED("gather_rows_kernel");
+}
+
+int imogen_mlp_infer_bf16(int64_t batch, const float *x, float *y)
+{
+    if (!g_model.loaded) return IM_FAIL(ERROR_INVALID_ARGS, "no model loaded (imogen_drag_set_model)");
+    if (!x || !y) return IM_FAIL(ERROR_NULL_POINTER, "null argument");
+    if (batch <= 0) return SUCCESSFUL;
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    int64_t waves = (batch + 63) / 64;
+    if (waves > 8192) waves = 8192;
+    hipLaunchKernelGGL(mlp_infer_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st->stream, batch, x, y,
+                       g_model.dev);
+    return IM_LAUNCHED("mlp_infer_kernel");
+}
+
+}  // extern "C"

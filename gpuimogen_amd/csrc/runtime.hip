@@ -153,6 +153,48 @@ int imogen_synchronize(void)
     return IM_HIP(hipStreamSynchronize(st->stream));
 }
 
+int imogen_device_alloc(size_t bytes, void **ptr)
+{
+    if (!ptr) return IM_FAIL(ERROR_NULL_POINTER, "null output pointer");
+    *ptr = nullptr;
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    if (bytes == 0) return SUCCESSFUL;
+    return IM_HIP(hipMalloc(ptr, bytes));
+}
+
+int imogen_device_free(void *ptr)
+{
+    if (!ptr) return SUCCESSFUL;
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    int rc = IM_HIP(hipStreamSynchronize(st->stream));
+    if (rc) return rc;
+    return IM_HIP(hipFree(ptr));
+}
+
+int imogen_memcpy_htod(void *dst, const void *src, size_t bytes)
+{
+    if (!bytes) return SUCCESSFUL;
+    if (!dst || !src) return IM_FAIL(ERROR_NULL_POINTER, "null pointer");
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    int rc = IM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st->stream));
+    if (rc) return rc;
+    return IM_HIP(hipStreamSynchronize(st->stream));
+}
+
+int imogen_memcpy_dtoh(void *dst, const void *src, size_t bytes)
+{
+    if (!bytes) return SUCCESSFUL;
+    if (!dst || !src) return IM_FAIL(ERROR_NULL_POINTER, "null pointer");
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    int rc = IM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st->stream));
+    if (rc) return rc;
+    return IM_HIP(hipStreamSynchronize(st->stream));
+}
+
 int imogen_profile_enable(int on)
 {
     DeviceState *st = state();

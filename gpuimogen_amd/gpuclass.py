@@ -139,13 +139,17 @@ class FluidManager:
     """One fluid: a 5-slab DataHolder (rho, E, px, py, pz) plus thermodynamics
     and boundary modes (fluid/FluidManager.m, save/uploadDataArrays.m:93-111)."""
 
-    def __init__(self, state, gamma, minmass=1e-8, bc_modes=(abi.circular,) * 6, cisothermal=-1.0):
+    def __init__(self, state, gamma, minmass=1e-8, bc_modes=(abi.circular,) * 6, cisothermal=-1.0,
+                 thermoDetails=None):
         state = np.ascontiguousarray(state, dtype=np.float64)
         if state.ndim != 4 or state.shape[0] != 5:
             raise ValueError("fluid state must be shaped (5, nz, ny, nx)")
         self.gamma = float(gamma)
         self.MINMASS = float(minmass)
         self.Cisothermal = float(cisothermal)
+        # microphysics for the drag source (FluidManager.thermoDetails)
+        self.thermoDetails = thermoDetails if thermoDetails is not None else abi.ThermoDetails(
+            self.gamma, 1.0, self.Cisothermal, 1.0, 0.0, 0.0, 1.0, 0.0)
         self.bcModes = tuple(int(m) for m in bc_modes)
         self.DataHolder = GPU_Type(state[0])
         self.DataHolder.createSlabs(5)
@@ -270,3 +274,27 @@ def setBoundaries(fluid, geometry, direction):
 
 def synchronize():
     call("imogen_synchronize")
+
+
+def cudaSource2FluidDrag(fluids, geometry, params):
+    """cudaSource2FluidDrag(FluidManager[2], geometry, [dt, method])
+    (gpuclass/cudaSource2FluidDrag.cu:125-180): fluids[0] is the gas,
+    fluids[1] the dust."""
+    if len(fluids) != 2 or len(params) != 2:
+        raise ValueError("need cudaSource2FluidDrag(FluidManager[2], geometry, [dt, solverMethod])")
+    g = geometry.params()
+    call("sourcefunction_2FluidDrag", fluids[0].arrays(), fluids[1].arrays(), ctypes.byref(g),
+         ctypes.byref(fluids[0].thermoDetails), ctypes.byref(fluids[1].thermoDetails), float(params[0]),
+         int(params[1]))
+
+
+def src0001(fluids, geometry, dTime, tFraction, dragMethod):
+    """sources/src0001.m: drag over dTime*tFraction as two half steps, then the
+    boundary conditions of every fluid in every direction."""
+    dt = dTime * tFraction
+    cudaSource2FluidDrag(fluids, geometry, [dt / 2, dragMethod])
+    cudaSource2FluidDrag(fluids, geometry, [dt / 2, dragMethod])
+    for f in fluids:
+        for d in (1, 2, 3):
+            if geometry.globalDomainRez[d - 1] > 1 or d == 1:
+                setBoundaries(f, geometry, d)

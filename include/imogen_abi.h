@@ -243,6 +243,73 @@ int imogen_cflTimestepFused(MGArray *fluid, GeometryParams *geom, int method, do
                             double *globalResolution, double *tstep);
 
 /* ======================================================================== */
+/* Gas-dust drag source                                                     */
+/* ======================================================================== */
+
+/* Drag integrators. 0..5 are the reference's codes (cudaSource2FluidDrag.cu:
+ * 244-263; 4 = LogTrap2 is its default, experiment/Initializer.m:126).
+ * 6 and 7 have no counterpart in the reference: 6 integrates the relaxation
+ * problem to a tight tolerance (adaptive Dormand-Prince 5(4) on ln|dv|,
+ * absolute tolerance 1e-12), 7 is the learned surrogate with per-cell fallback
+ * to 6 (needs imogen_drag_set_model first). */
+typedef enum DragMethods {
+    DRAG_EMP = 0, DRAG_RK4 = 1, DRAG_ETDRK1 = 2, DRAG_ETDRK2 = 3, DRAG_LOGTRAP2 = 4, DRAG_LOGTRAP3 = 5,
+    DRAG_EXACT = 6, DRAG_LEARNED = 7
+} DragMethods;
+
+/* cudaSource2FluidDrag.cu:189 (header cudaSource2FluidDrag.h). fluidA = gas,
+ * fluidB = dust canisters (5 slab views each, same grid). Updates gas E and
+ * momenta and dust momenta in place over dt. Built: methods 4-7; 0-3 return
+ * ERROR_NOIMPLEMENT. */
+int sourcefunction_2FluidDrag(MGArray *fluidA, MGArray *fluidB, GeometryParams *geo, ThermoDetails *thermogas,
+                              ThermoDetails *thermodust, double dt, int method);
+
+/* ---- learned drag surrogate (no counterpart in the reference) -------------
+ * Network 5 -> 64 -> 64 -> 3, GELU (tanh form), parameters in FP32 as
+ * W1[5][64] b1[64] W2[64][64] b2[64] W3[64][3] b3[3] (row = input unit),
+ * IMOGEN_MLP_NPARAMS floats. Features of a cell (natural logs): ln(dt k0),
+ * ln(rho_d/rho_g), ln(|dv|/c_s), ln Re_0, ln Kn_0, standardised with
+ * fmean/fstd. With tau = dt k0 and g = d ln k/d tau at 0 (analytic), the
+ * rates along the exact path are k/k0 = 1 + tau (c g + tau r), c = 1/2 for the
+ * mean rate and the rate at dt/2, c = 1 for the rate at dt; the targets are
+ * the three remainders r, standardised with tmean/tstd. */
+#define IMOGEN_MLP_NPARAMS 4739
+typedef struct DragModelNorm {
+    float fmean[5], fstd[5];   /* feature standardisation */
+    float tmean[3], tstd[3];   /* target standardisation */
+    float flo[5], fhi[5];      /* raw feature box the network was trained on */
+    float residTol;            /* fallback threshold on the end-rate residual */
+} DragModelNorm;
+
+/* Uploads a trained network (converted to BF16 for inference). */
+int imogen_drag_set_model(const float *params, const DragModelNorm *norm);
+/* Cells and fallbacks of the learned drag since the last call (resets). */
+int imogen_drag_learned_stats(int64_t *cells, int64_t *fallbacks);
+/* Training set from cell states on the device: cells[n][5] = (rho_g, rho_d,
+ * |dv|, u_0, dt k0) in FP64 (the step is given through its stiffness; dt =
+ * (dt k0)/k0); writes raw (unstandardised) features[n][5] and targets[n][3]
+ * in FP32 from the exact integrator (NaN rows where |dv| < 1e-9). */
+int imogen_drag_training_samples(const ThermoDetails *gas, const ThermoDetails *dust, int64_t n,
+                                 const double *cells, float *features, float *targets);
+/* FP32 training primitives on device buffers (batch rows, row-major).
+ * act holds batch x 256 floats (pre-activations z1, z2 and activations
+ * h1, h2). grads receives IMOGEN_MLP_NPARAMS floats (overwritten). */
+int imogen_mlp_forward(const float *params, int64_t batch, const float *x, float *act, float *y);
+int imogen_mlp_backward(const float *params, int64_t batch, const float *x, const float *act, const float *dy,
+                        float *grads);
+/* Mean-squared-error gradient dy = 2 (y - t) / (3 batch) over batch x 3
+ * outputs; adds sum((y - t)^2) / (3 batch) into *lossAccum (device double). */
+int imogen_mlp_mse_grad(int64_t batch, const float *y, const float *t, float *dy, double *lossAccum);
+/* dst[r][:] = src[idx[r]][:] for r < nrows (ncols floats per row). */
+int imogen_gather_rows(const float *src, const int64_t *idx, int64_t nrows, int ncols, float *dst);
+/* One Adam step (Kingma & Ba 2015) over n parameters; step counts from 1. */
+int imogen_mlp_adam(float *params, const float *grads, float *m, float *v, int64_t n, float lr, float beta1,
+                    float beta2, float eps, int step);
+/* BF16-MFMA inference of standardised features x[batch][5] -> y[batch][3]
+ * (standardised outputs), the path the learned drag uses. */
+int imogen_mlp_infer_bf16(int64_t batch, const float *x, float *y);
+
+/* ======================================================================== */
 /* Runtime utilities (no counterpart in the reference: it used the legacy   */
 /* default stream; this library issues all work on one stream per device).  */
 /* ======================================================================== */
@@ -251,6 +318,13 @@ int imogen_cflTimestepFused(MGArray *fluid, GeometryParams *geom, int method, do
 void *imogen_stream(void);
 /* Blocks until all work of this library on the current device is done. */
 int imogen_synchronize(void);
+/* Plain device buffers for callers without their own GPU runtime (the learned
+ * drag's training data). Copies are ordered on the library stream and return
+ * when complete; free waits for the stream first. */
+int imogen_device_alloc(size_t bytes, void **ptr);
+int imogen_device_free(void *ptr);
+int imogen_memcpy_htod(void *dst, const void *src, size_t bytes);
+int imogen_memcpy_dtoh(void *dst, const void *src, size_t bytes);
 /* Per-kernel timing with HIP events on the library stream. enable(1) clears
  * the record and starts timing every launch; read() returns the number of
  * timed launches and their summed duration for a kernel id:

@@ -84,6 +84,37 @@ void orc_soundspeed(const double *q, const orc_grid *g, double gamma, double *cs
 double orc_cfl_timestep(const double *q, const double *cs, const orc_grid *g, const double *h,
                         int method, const int *globalRez);
 
+/* ---- gas-dust drag (drag_oracle.c) ---- */
+
+/* Microphysics of one fluid; same fields and order as the reference's
+ * ThermoDetails (gpuclass/cudaCommon.h:146-158). */
+typedef struct {
+    double gamma, m, Cisothermal, kBolt, mu0, muTindex, sigma0, sigmaTindex;
+} orc_thermo;
+
+/* Drag integrators: the reference's method codes 4/5
+ * (cudaSource2FluidDrag.cu:257-262) plus this build's adaptive reference
+ * integrator. */
+enum orc_drag_method { ORC_DRAG_LOGTRAP2 = 4, ORC_DRAG_LOGTRAP3 = 5, ORC_DRAG_EXACT = 6 };
+
+/* The 9 drag constants hostDrag[0..8] (cudaSource2FluidDrag.cu:210-220). */
+void orc_drag_params(const orc_thermo *gas, const orc_thermo *dust, double *P9);
+
+/* Drag rate k(|dv|, u) of one cell (:836-890); 0 below |dv| = 1e-9. */
+double orc_drag_k(const double *P9, double rhog, double rhod, double magdv, double u);
+
+/* |dv(t)|/|dv(0)| of the relaxation problem, adaptive Dormand-Prince 5(4) on
+ * ln|dv| to absolute tolerance tol. steps: step count (negative if the budget
+ * of 4096 steps ran out). */
+double orc_drag_exact_ratio(const double *P9, double rhog, double rhod, double s0, double u0, double t, double tol,
+                            int *steps);
+
+/* One drag source call over the grid: updates gas E, p and dust p in place.
+ * Returns 0 or a negative status; *work (optional) receives the exact
+ * integrator's total step count. */
+int orc_drag(double *gas, double *dust, const orc_grid *g, const orc_thermo *tgas, const orc_thermo *tdust,
+             double dt, int method, double tol, long *work);
+
 /* Slope limiter evaluation (exported for the limiter tests). */
 double orc_slope_limiter(int which, double a, double b);
 

@@ -30,6 +30,15 @@ class _Grid(ctypes.Structure):
                 ("pitch", ctypes.c_long)]
 
 
+class Thermo(ctypes.Structure):
+    """ThermoDetails (gpuclass/cudaCommon.h:146-158)."""
+    _fields_ = [("gamma", ctypes.c_double), ("m", ctypes.c_double), ("Cisothermal", ctypes.c_double),
+                ("kBolt", ctypes.c_double), ("mu0", ctypes.c_double), ("muTindex", ctypes.c_double),
+                ("sigma0", ctypes.c_double), ("sigmaTindex", ctypes.c_double)]
+
+
+DRAG_LOGTRAP2, DRAG_LOGTRAP3, DRAG_EXACT = 4, 5, 6
+
 _lib = None
 
 
@@ -62,6 +71,15 @@ def lib():
         L.orc_cfl_timestep.restype = ctypes.c_double
         L.orc_slope_limiter.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
         L.orc_slope_limiter.restype = ctypes.c_double
+        L.orc_drag_params.argtypes = [ctypes.POINTER(Thermo), ctypes.POINTER(Thermo), dp]
+        L.orc_drag_k.argtypes = [dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.orc_drag_k.restype = ctypes.c_double
+        L.orc_drag_exact_ratio.argtypes = [dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_double, ip]
+        L.orc_drag_exact_ratio.restype = ctypes.c_double
+        L.orc_drag.argtypes = [dp, dp, ctypes.POINTER(_Grid), ctypes.POINTER(Thermo), ctypes.POINTER(Thermo),
+                               ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_long)]
+        L.orc_drag.restype = ctypes.c_int
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_get_threads.restype = ctypes.c_int
         _lib = L
@@ -162,3 +180,34 @@ def cfl_timestep(q, cs, h, method, global_rez=None):
 
 def slope_limiter(which, a, b):
     return lib().orc_slope_limiter(which, a, b)
+
+
+def drag_params(tgas, tdust):
+    P = np.zeros(9)
+    lib().orc_drag_params(ctypes.byref(tgas), ctypes.byref(tdust), _dptr(P))
+    return P
+
+
+def drag_k(P, rhog, rhod, magdv, u):
+    P = np.ascontiguousarray(P, dtype=np.float64)
+    return lib().orc_drag_k(_dptr(P), rhog, rhod, magdv, u)
+
+
+def drag_exact_ratio(P, rhog, rhod, s0, u0, t, tol=1e-12):
+    P = np.ascontiguousarray(P, dtype=np.float64)
+    steps = ctypes.c_int(0)
+    r = lib().orc_drag_exact_ratio(_dptr(P), rhog, rhod, s0, u0, t, tol, ctypes.byref(steps))
+    return r, steps.value
+
+
+def drag(gas, dust, tgas, tdust, dt, method=DRAG_LOGTRAP2, tol=1e-12):
+    """One drag source call (in place on both 5-slab states)."""
+    g = _grid(gas)
+    if dust.shape != gas.shape or not dust.flags.c_contiguous or dust.dtype != np.float64:
+        raise ValueError("dust must match the gas array")
+    work = ctypes.c_long(0)
+    rc = lib().orc_drag(_dptr(gas), _dptr(dust), ctypes.byref(g), ctypes.byref(tgas), ctypes.byref(tdust), dt,
+                        method, tol, ctypes.byref(work))
+    if rc:
+        raise RuntimeError(f"orc_drag returned {rc}")
+    return work.value

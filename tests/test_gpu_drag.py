@@ -1,0 +1,285 @@
+"""GPU parity of the drag source and the learned-drag network kernels.
+
+Bars: LogTrap2/3 match the oracle to 1e-12 relative L1 per slab (the same
+arithmetic, FMA contraction aside). The adaptive integrator matches to 1e-10:
+its step-size controller sees FP64 rounding differences, so both sides stop
+within the integration tolerance (1e-12 on ln|dv|) of the true solution but
+not necessarily on the same step sequence. The FP32 training kernels match the
+FP64 numpy restatement (oracle/mlp_oracle.py) to FP32 rounding; BF16 inference
+matches a BF16-rounding emulation of the same network.
+"""
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as mo
+from oracle import pyoracle as po
+
+from .test_gpu_hllc import gc
+from .test_oracle_drag import gas_dust_thermo
+
+pytestmark = pytest.mark.gpu
+
+
+def abi():
+    from gpuimogen_amd import abi as a
+    return a
+
+
+def ld():
+    from gpuimogen_amd import learned_drag
+    return learned_drag
+
+
+def thermo_pair():
+    a = abi()
+    tg, td = gas_dust_thermo()
+    return (a.ThermoDetails(*[getattr(tg, f) for f, _ in tg._fields_]),
+            a.ThermoDetails(*[getattr(td, f) for f, _ in td._fields_]))
+
+
+def random_pair(sh, seed, eps_median=0.1):
+    rng = np.random.default_rng(seed)
+    g = np.zeros((5,) + sh)
+    d = np.zeros((5,) + sh)
+    g[0] = rng.uniform(0.5, 2, sh)
+    d[0] = g[0] * np.exp(rng.normal(np.log(eps_median), 1.0, sh))
+    vg = rng.normal(0, 0.4, (3,) + sh)
+    vd = rng.normal(0, 0.4, (3,) + sh)
+    vd[:, 0, 0, :3] = vg[:, 0, 0, :3]  # no relative motion
+    g[2:] = g[0] * vg
+    d[2:] = d[0] * vd
+    g[1] = rng.uniform(0.5, 2, sh) / (5 / 3 - 1) + .5 * g[0] * (vg ** 2).sum(0)
+    d[1] = .5 * d[0] * (vd ** 2).sum(0)
+    return g, d
+
+
+def compare_slabs(got, ref, tol, what):
+    for s in range(5):
+        dsum = np.abs(got[s] - ref[s]).sum()
+        scale = np.abs(ref[s]).sum()
+        rel = dsum / scale if scale > 0 else dsum
+        print(f"{what} slab {s}: rel L1 {rel:.3e}", flush=True)
+        assert rel <= tol, (what, s, rel)
+
+
+def run_gpu_drag(g, d, dt, method, sh):
+    a = abi()
+    tg, td = thermo_pair()
+    fg = gc().FluidManager(g, tg.gamma, thermoDetails=tg)
+    fd = gc().FluidManager(d, td.gamma, thermoDetails=td)
+    nz, ny, nx = sh
+    geo = gc().GeometryManager((nx, ny, nz), (0.1, 0.1, 0.1))
+    gc().cudaSource2FluidDrag([fg, fd], geo, [dt, method])
+    out = fg.state, fd.state
+    fg.free()
+    fd.free()
+    return out
+
+
+@pytest.mark.parametrize("method,tol", [(4, 1e-12), (5, 1e-12), (6, 1e-10)])
+def test_drag_parity(method, tol):
+    sh = (6, 10, 70)
+    g, d = random_pair(sh, seed=method)
+    print(f"drag method {method} shape {sh}", flush=True)
+    gg, dd = run_gpu_drag(g, d, 0.02, method, sh)
+    tg, td = gas_dust_thermo()
+    rg, rd = g.copy(), d.copy()
+    po.drag(rg, rd, tg, td, 0.02, method)
+    compare_slabs(gg, rg, tol, f"gas m{method}")
+    compare_slabs(dd, rd, tol, f"dust m{method}")
+
+
+def test_unbuilt_drag_methods_fail_loudly():
+    sh = (2, 3, 4)
+    g, d = random_pair(sh, seed=1)
+    for m in (0, 1, 2, 3):
+        with pytest.raises(abi().ImogenError):
+            run_gpu_drag(g, d, 0.01, m, sh)
+
+
+# ---------------------------------------------------------------- network kernels
+def _dev(x, dtype=np.float32):
+    return ld().DeviceArray.from_host(np.asarray(x, dtype=dtype))
+
+
+def test_mlp_forward_backward_match_oracle():
+    L = ld()
+    rng = np.random.default_rng(3)
+    p = L.init_params(rng)
+    p[mo.OFF_B1:mo.OFF_W2] = rng.normal(0, 0.1, 64)
+    p[mo.OFF_B2:mo.OFF_W3] = rng.normal(0, 0.1, 64)
+    p[mo.OFF_B3:] = rng.normal(0, 0.1, 3)
+    B = 1000  # not a multiple of the 64-row block
+    x = rng.normal(0, 1, (B, 5)).astype(np.float32)
+    dy = rng.normal(0, 1, (B, 3)).astype(np.float32)
+    P, X, DY = _dev(p), _dev(x), _dev(dy)
+    act = L.DeviceArray((B, 256), np.float32)
+    y = L.DeviceArray((B, 3), np.float32)
+    gr = L.DeviceArray((mo.NPARAMS,), np.float32)
+    abi().call("imogen_mlp_forward", P.ptr(), B, X.ptr(), act.ptr(), y.ptr())
+    abi().call("imogen_mlp_backward", P.ptr(), B, X.ptr(), act.ptr(), DY.ptr(), gr.ptr())
+    y, act, gr = y.to_host(), act.to_host(), gr.to_host()
+    yr, (z1, h1, z2, h2) = mo.forward(p.astype(np.float64), x.astype(np.float64))
+    assert np.abs(y - yr).max() <= 1e-5 * np.abs(yr).max()
+    for k, ref in enumerate((z1, h1, z2, h2)):
+        assert np.abs(act[:, 64 * k:64 * (k + 1)] - ref).max() <= 1e-5 * max(1.0, np.abs(ref).max())
+    gref = mo.backward(p.astype(np.float64), x.astype(np.float64), (z1, h1, z2, h2), dy.astype(np.float64))
+    for name, lo, hi in (("W1", mo.OFF_W1, mo.OFF_B1), ("b1", mo.OFF_B1, mo.OFF_W2), ("W2", mo.OFF_W2, mo.OFF_B2),
+                         ("b2", mo.OFF_B2, mo.OFF_W3), ("W3", mo.OFF_W3, mo.OFF_B3), ("b3", mo.OFF_B3, mo.NPARAMS)):
+        err = np.abs(gr[lo:hi] - gref[lo:hi]).max() / np.abs(gref[lo:hi]).max()
+        print(f"grad {name}: max rel {err:.2e}", flush=True)
+        assert err <= 2e-5, name
+
+
+def test_loss_gradient_gather_and_adam_match_oracle():
+    L = ld()
+    rng = np.random.default_rng(4)
+    # mean-squared-error gradient and loss
+    B = 777
+    y, t = rng.normal(size=(B, 3)).astype(np.float32), rng.normal(size=(B, 3)).astype(np.float32)
+    Y, T = _dev(y), _dev(t)
+    DY = L.DeviceArray((B, 3), np.float32)
+    loss = _dev(np.zeros(1), np.float64)
+    abi().call("imogen_mlp_mse_grad", B, Y.ptr(), T.ptr(), DY.ptr(), loss.ptr())
+    e = y.astype(np.float64) - t
+    assert np.allclose(DY.to_host(), 2 * e / e.size, rtol=1e-6, atol=1e-9)
+    # the kernel squares the FP32 difference
+    assert loss.to_host()[0] == pytest.approx((e ** 2).mean(), rel=1e-6)
+    # row gather
+    src = rng.normal(size=(50, 5)).astype(np.float32)
+    idx = rng.integers(0, 50, 31).astype(np.int64)
+    S, I = _dev(src), _dev(idx, np.int64)
+    D = L.DeviceArray((31, 5), np.float32)
+    abi().call("imogen_gather_rows", S.ptr(), I.ptr(), 31, 5, D.ptr())
+    assert np.array_equal(D.to_host(), src[idx])
+    # Adam
+    n = abi().IMOGEN_MLP_NPARAMS
+    p, g = rng.normal(size=n).astype(np.float32), rng.normal(size=n).astype(np.float32)
+    m, v = rng.normal(size=n).astype(np.float32) * 0.1, rng.uniform(0, 0.1, n).astype(np.float32)
+    P, G, M, V = _dev(p), _dev(g), _dev(m), _dev(v)
+    abi().call("imogen_mlp_adam", P.ptr(), G.ptr(), M.ptr(), V.ptr(), n, 1e-2, 0.9, 0.999, 1e-8, 7)
+    got = P.to_host(), M.to_host(), V.to_host()
+    ref = mo.adam(p.astype(np.float64), g.astype(np.float64), m.astype(np.float64), v.astype(np.float64),
+                  1e-2, 0.9, 0.999, 1e-8, 7)
+    for a, b in zip(got, ref):
+        assert np.allclose(a, b, rtol=1e-5, atol=1e-6)
+
+
+def test_bf16_inference_matches_emulation():
+    L = ld()
+    rng = np.random.default_rng(5)
+    p = L.init_params(rng)
+    p[mo.OFF_B1:mo.OFF_W2] = rng.normal(0, 0.1, 64)
+    p[mo.OFF_B2:mo.OFF_W3] = rng.normal(0, 0.1, 64)
+    p[mo.OFF_B3:] = rng.normal(0, 0.1, 3)
+    norm = abi().DragModelNorm()
+    for j in range(5):
+        norm.fstd[j] = 1.0
+    L.set_model(p, norm)
+    B = 1000
+    x = rng.normal(0, 1, (B, 5)).astype(np.float32)
+    y = L.infer_bf16(x)
+    yb = mo.forward_bf16(p, x)
+    yf, _ = mo.forward(p.astype(np.float64), x.astype(np.float64))
+    scale = np.abs(yf).max()
+    err_emul = np.abs(y - yb).max() / scale
+    err_fp = np.abs(y - yf).max() / scale
+    print(f"bf16 inference: vs emulation {err_emul:.2e}, vs fp64 network {err_fp:.2e}", flush=True)
+    assert err_emul <= 1e-2
+    assert err_fp <= 5e-2
+
+
+# ---------------------------------------------------------------- learned drag
+def test_learned_drag_fallback_paths():
+    """residTol = 0 sends every cell to the exact integrator (result identical
+    to method 6); an unbounded tolerance and box accept every moving cell."""
+    L = ld()
+    rng = np.random.default_rng(6)
+    p = L.init_params(rng) * 0.0  # predicts tmean: a crude but valid model
+    norm = abi().DragModelNorm()
+    for j in range(5):
+        norm.fstd[j] = 1.0
+        norm.flo[j] = -1e30
+        norm.fhi[j] = 1e30
+    sh = (4, 6, 50)
+    g, d = random_pair(sh, seed=8)
+    n_moving = int((np.sqrt(((g[2:] / g[0] - d[2:] / d[0]) ** 2).sum(0)) >= 1e-9).sum())
+    norm.residTol = 0.0
+    L.set_model(p, norm)
+    L.learned_stats()
+    gl, dl = run_gpu_drag(g, d, 0.02, 7, sh)
+    cells, fb = L.learned_stats()
+    assert cells == g[0].size and fb == n_moving
+    ge, de = run_gpu_drag(g, d, 0.02, 6, sh)
+    # same integrator, compiled into two kernels (FMA contraction may differ)
+    assert np.allclose(gl, ge, rtol=1e-13, atol=0) and np.allclose(dl, de, rtol=1e-13, atol=0)
+    # with dt k0 <= 0.1 the baseline rate stays positive and every moving cell
+    # is accepted under an unbounded tolerance
+    norm.residTol = 1e30
+    L.set_model(p, norm)
+    run_gpu_drag(g, d, 1e-3, 7, sh)
+    cells, fb = L.learned_stats()
+    assert fb == 0
+    # A zero remainder leaves k/k0 = 1 + tau c g: for tau <= 1e-4 every cell
+    # passes a 1e-10 residual check and matches the exact integrator to
+    # O(tau^3); a wrong initial slope g would miss by O(tau^2).
+    norm.residTol = 1e-10
+    L.set_model(p, norm)
+    gl, dl = run_gpu_drag(g, d, 1e-6, 7, sh)
+    cells, fb = L.learned_stats()
+    assert fb == 0
+    ge, de = run_gpu_drag(g, d, 1e-6, 6, sh)
+    dvl = gl[2:] / gl[0] - dl[2:] / dl[0]
+    dve = ge[2:] / ge[0] - de[2:] / de[0]
+    assert np.abs(dvl - dve).max() <= 1e-10 * np.abs(dve).max()
+
+
+def cells_to_grid(cells):
+    """Gas/dust states on a 1 x 1 x n grid from sampled (rho_g, rho_d, |dv|,
+    u_0, .) rows: the gas moves along x, the dust is at rest."""
+    n = cells.shape[0]
+    g = np.zeros((5, 1, 1, n))
+    d = np.zeros((5, 1, 1, n))
+    rg, rd, s0, u0 = cells[:, 0], cells[:, 1], cells[:, 2], cells[:, 3]
+    g[0, 0, 0] = rg
+    g[2, 0, 0] = rg * s0
+    g[1, 0, 0] = u0 * rg + .5 * rg * s0 * s0
+    d[0, 0, 0] = rd
+    return g, d
+
+
+def test_trained_network_cuts_fallbacks():
+    """A short training run on the GPU (this library's kernels), then the same
+    held-out cells through the learned drag with an untrained (zero) network
+    and with the trained one: the trained network must pass the residual
+    check on far more cells, while every accepted cell stays accurate."""
+    L = ld()
+    tg, td = thermo_pair()
+    rng = np.random.default_rng(11)
+    cells = L.sample_cells(200_000, tg, rng, tau_range=(1e-3, 0.5))
+    f, t = L.training_set(cells, tg, td)
+    norm = L.fit_norm(f, t, resid_tol=1e-8)
+    params, hist = L.train(f, t, norm, epochs=12, batch=2048, lr=3e-3, seed=1, log=print)
+    assert hist[-1] < 0.3 * hist[0], hist
+    held = L.sample_cells(20_000, tg, np.random.default_rng(12))
+    g, d = cells_to_grid(held)
+    sh = g.shape[1:]
+    dt = 2e-3
+    ge, de = run_gpu_drag(g, d, dt, 6, sh)
+    dv_e = ge[2] / ge[0] - de[2] / de[0]
+    dv0 = g[2] / g[0]
+    moving = np.abs(dv0) >= 1e-9
+    counts = {}
+    for name, p in (("zero", np.zeros_like(params)), ("trained", params)):
+        nrm = L.fit_norm(f, t, resid_tol=1e-8)
+        L.set_model(p, nrm)
+        L.learned_stats()
+        gl, dl = run_gpu_drag(g, d, dt, 7, sh)
+        cells_n, fb = L.learned_stats()
+        dv_l = gl[2] / gl[0] - dl[2] / dl[0]
+        err = np.abs(np.log(dv_l[moving] / dv_e[moving]))
+        counts[name] = fb
+        print(f"{name} network: {fb} of {int(moving.sum())} cells fell back; max |ln r error| {err.max():.2e}",
+              flush=True)
+        assert err.max() <= 1e-6
+    assert counts["trained"] < 0.5 * counts["zero"], counts

@@ -1,0 +1,160 @@
+"""Pins the oracle's gas-dust drag (oracle/drag_oracle.c).
+
+What pins what:
+  drag constants + rate k   the independent dusty-box formulation of the
+                            acceleration (experiment/Advection/DustyBoxAnalyzer.m:
+                            91-104, 152-155), written out here from its text
+  exact integrator          scipy's DOP853 at rtol 1e-13 on that same ODE in
+                            |dv| (not ln|dv|), with P(|dv|) from energy
+                            conservation (DustyBoxAnalyzer.m:135-140)
+  LogTrap2 / LogTrap3       their convergence orders 2 and 3 against the exact
+                            solution (the reference's tsDustybox.m criterion)
+  the cell update           exact momentum and total-energy conservation,
+                            unchanged barycentric velocity, dv direction kept
+"""
+import numpy as np
+import pytest
+from scipy.integrate import solve_ivp
+
+from oracle import pyoracle as po
+
+AMU = 1.66053904e-27
+KB = 1.381e-23
+
+
+def gas_dust_thermo():
+    """fluidDetailModel('cold_molecular_hydrogen') and ('10um_iron_balls') with
+    the dust scaled as run/profile_3DXYZ_gasdust.m:43-46 does (sigma/10, m/1e4)."""
+    m = 2.016 * AMU
+    mu0 = 8.9135e-6
+    sigma = np.sqrt(2 / 27) * 1.7 * np.sqrt(KB * 298.15 * m) / mu0  # generateSigma
+    tg = po.Thermo(5 / 3, m, -1.0, KB, mu0, 0.7, sigma, 0.2)
+    td = po.Thermo(1.01, 3e-11 * 1e-4, 0.1, KB, 0.0, 0.0, np.pi * 25e-12 * 1e-1, 0.0)
+    return tg, td
+
+
+def dustybox_accel(tg, td, rhog, rhod, P, dv):
+    """d|dv|/dt of DustyBoxAnalyzer.computeAcceleration, as written there."""
+    T = (tg.m * P) / (rhog * tg.kBolt)
+    visc = tg.mu0 * (T / 298.15) ** tg.muTindex
+    Rey = rhog * dv * np.sqrt(td.sigma0 / np.pi) / visc
+    mfp = tg.m * (T / 298.15) ** tg.sigmaTindex / (rhog * tg.sigma0 * np.sqrt(2))
+    Kn = 2 * mfp / np.sqrt(td.sigma0 / np.pi)
+    C = (24 / Rey + 3.6 * Rey ** -.319 + .4072 * Rey / (8710 + Rey)) / (1 + Kn * (1.142 + 0.558 * np.exp(-0.999 / Kn)))
+    return -C * .5 * dv ** 2 * (td.sigma0 / 4) * (rhog + rhod) / td.m
+
+
+def test_drag_rate_matches_dustybox_formulation():
+    tg, td = gas_dust_thermo()
+    P9 = po.drag_params(tg, td)
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        rhog = rng.uniform(0.1, 3)
+        rhod = rng.uniform(0.01, 3)
+        P = rng.uniform(0.05, 5)
+        dv = 10 ** rng.uniform(-6, 0.5)
+        u = P / ((tg.gamma - 1) * rhog)
+        k = po.drag_k(P9, rhog, rhod, dv, u)
+        a = dustybox_accel(tg, td, rhog, rhod, P, dv)
+        assert k * dv == pytest.approx(-a, rel=1e-12)
+    assert po.drag_k(P9, 1.0, 0.1, 0.9e-9, 1.0) == 0.0  # no drag below |dv| = 1e-9 (:853-859)
+
+
+@pytest.mark.parametrize("case", range(6))
+def test_exact_integrator_matches_dop853(case):
+    tg, td = gas_dust_thermo()
+    P9 = po.drag_params(tg, td)
+    rng = np.random.default_rng(10 + case)
+    rhog = rng.uniform(0.3, 2)
+    rhod = rng.uniform(0.02, 2)
+    P0 = rng.uniform(0.2, 3)
+    dv0 = 10 ** rng.uniform(-3, 0)
+    k0 = po.drag_k(P9, rhog, rhod, dv0, P0 / ((tg.gamma - 1) * rhog))
+    t = (0.02, 0.2, 1.0, 3.0, 10.0, 0.5)[case] / k0  # from mild to very stiff
+    mu = rhog * rhod / (rhog + rhod)
+
+    def rhs(_t, y):
+        P = P0 + (tg.gamma - 1) * .5 * mu * (dv0 ** 2 - y[0] ** 2)
+        return [dustybox_accel(tg, td, rhog, rhod, P, y[0])]
+
+    sol = solve_ivp(rhs, (0, t), [dv0], method="DOP853", rtol=1e-13, atol=1e-20)
+    ref = sol.y[0, -1] / dv0
+    r, steps = po.drag_exact_ratio(P9, rhog, rhod, dv0, P0 / ((tg.gamma - 1) * rhog), t, 1e-12)
+    assert steps > 0
+    assert r == pytest.approx(ref, rel=2e-11), (r, ref, steps)
+
+
+def one_cell(rhog, vg, Pg, rhod, vd, gamma=5 / 3):
+    g = np.zeros((5, 1, 1, 1))
+    d = np.zeros((5, 1, 1, 1))
+    g[0] = rhog
+    g[2:, 0, 0, 0] = rhog * np.asarray(vg)
+    g[1] = Pg / (gamma - 1) + .5 * rhog * np.sum(np.square(vg))
+    d[0] = rhod
+    d[2:, 0, 0, 0] = rhod * np.asarray(vd)
+    d[1] = .5 * rhod * np.sum(np.square(vd))
+    return g, d
+
+
+def rel_dv(g, d):
+    return g[2:, 0, 0, 0] / g[0, 0, 0, 0] - d[2:, 0, 0, 0] / d[0, 0, 0, 0]
+
+
+def test_logtrap_convergence_orders():
+    tg, td = gas_dust_thermo()
+    P9 = po.drag_params(tg, td)
+    rhog, rhod, Pg = 1.0, 0.5, 1.0
+    vg, vd = [0.4, -0.1, 0.2], [-0.3, 0.1, 0.0]
+    g0, d0 = one_cell(rhog, vg, Pg, rhod, vd)
+    s0 = np.linalg.norm(rel_dv(g0, d0))
+    k0 = po.drag_k(P9, rhog, rhod, s0, Pg / ((tg.gamma - 1) * rhog))
+    T = 2.0 / k0
+    exact, _ = po.drag_exact_ratio(P9, rhog, rhod, s0, Pg / ((tg.gamma - 1) * rhog), T, 1e-13)
+    for method, order in ((po.DRAG_LOGTRAP2, 2), (po.DRAG_LOGTRAP3, 3)):
+        errs = []
+        for n in (8, 16, 32, 64):
+            g, d = g0.copy(), d0.copy()
+            for _ in range(n):
+                po.drag(g, d, tg, td, T / n, method)
+            errs.append(abs(np.linalg.norm(rel_dv(g, d)) / s0 - exact))
+        orders = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+        assert np.all(orders > order - 0.3), (method, errs, orders)
+        assert np.all(orders < order + 0.5), (method, errs, orders)
+
+
+@pytest.mark.parametrize("method", [po.DRAG_LOGTRAP2, po.DRAG_LOGTRAP3, po.DRAG_EXACT])
+def test_drag_invariants(method):
+    tg, td = gas_dust_thermo()
+    rng = np.random.default_rng(4)
+    sh = (4, 5, 6)
+    g = np.zeros((5,) + sh)
+    d = np.zeros((5,) + sh)
+    g[0] = rng.uniform(0.5, 2, sh)
+    d[0] = g[0] * np.exp(rng.normal(np.log(0.1), 1.0, sh))
+    vg = rng.normal(0, 0.4, (3,) + sh)
+    vd = rng.normal(0, 0.4, (3,) + sh)
+    vd[:, 0, 0, :2] = vg[:, 0, 0, :2]  # two cells without relative motion
+    g[2:] = g[0] * vg
+    d[2:] = d[0] * vd
+    g[1] = rng.uniform(0.5, 2, sh) / (tg.gamma - 1) + .5 * g[0] * (vg ** 2).sum(0)
+    d[1] = .5 * d[0] * (vd ** 2).sum(0)
+    g0, d0 = g.copy(), d.copy()
+    dv0 = g0[2:] / g0[0] - d0[2:] / d0[0]
+    po.drag(g, d, tg, td, 0.02, method)
+    # momentum per cell
+    assert np.allclose(g[2:] + d[2:], g0[2:] + d0[2:], rtol=0, atol=1e-14)
+    # total energy per cell: gas E + dust kinetic energy
+    ekd0 = .5 * (d0[2:] ** 2).sum(0) / d0[0]
+    ekd = .5 * (d[2:] ** 2).sum(0) / d[0]
+    assert np.allclose(g[1] + ekd, g0[1] + ekd0, rtol=1e-14, atol=1e-14)
+    # density untouched; dust energy slab untouched
+    assert np.array_equal(g[0], g0[0]) and np.array_equal(d[0], d0[0]) and np.array_equal(d[1], d0[1])
+    # dv keeps its direction and shrinks
+    dv = g[2:] / g[0] - d[2:] / d[0]
+    n0 = np.sqrt((dv0 ** 2).sum(0))
+    n1 = np.sqrt((dv ** 2).sum(0))
+    moving = n0 > 1e-9
+    assert np.all(n1[moving] < n0[moving])
+    cos = (dv * dv0).sum(0)[moving] / (n0 * n1)[moving]
+    assert np.all(cos > 1 - 1e-12)
+    assert np.allclose(dv[:, 0, 0, :2], 0, atol=1e-15)
