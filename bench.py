@@ -88,6 +88,60 @@ def run_gpu(args, q0, h):
     return elapsed, kern, simtime, final
 
 
+def run_gasdust(args):
+    """Gas + dust on a periodic n^3 box (HLLC gas, nearly pressureless HLL
+    isothermal dust, LogTrap-style drag source every half step as in
+    imogen.m): the same run with the adaptive drag integrator and with the
+    learned surrogate, from the same initial state."""
+    from gpuimogen_amd import abi, gpuclass as gc, imogen, learned_drag as L, problems
+
+    n = args.gasdust_n
+    gas0, dust0, h, ciso = problems.gasdust3d(n, seed=args.seed)
+    tg, td = L.cold_h2(), L.iron_dust()
+    params, norm, meta = L.load_model()
+    if args.drag_resid_tol is not None:
+        norm.residTol = args.drag_resid_tol
+    L.set_model(params, norm)
+    gm = gc.GeometryManager((n, n, n), h)
+    out = {}
+    for name, method in (("exact", abi.DRAG_EXACT), ("learned", abi.DRAG_LEARNED)):
+        fg = gc.FluidManager(gas0, tg.gamma, minmass=1e-8, bc_modes=(abi.circular,) * 6, thermoDetails=tg)
+        fd = gc.FluidManager(dust0, td.gamma, minmass=1e-8, bc_modes=(abi.circular,) * 6, cisothermal=ciso,
+                             thermoDetails=td)
+        sim = imogen.Simulation([fg, fd], [abi.METHOD_HLLC, abi.METHOD_HLL], gm, cfl=0.4, drag_method=method)
+        gc.synchronize()
+        abi.call("imogen_profile_enable", 1)
+        L.learned_stats()
+        t0 = time.perf_counter()
+        for k in range(args.gasdust_steps):
+            sim.step()
+            if (k + 1) % max(1, args.gasdust_steps // 4) == 0:
+                log(f"[bench] gas-dust {name}: step {k + 1}/{args.gasdust_steps}")
+        gc.synchronize()
+        el = time.perf_counter() - t0
+        prof = {}
+        for kid, kname in ((5, "drag"), (6, "learned_drag")):
+            cnt, tot = ctypes.c_int64(0), ctypes.c_double(0.0)
+            abi.call("imogen_profile_read", kid, ctypes.byref(cnt), ctypes.byref(tot))
+            prof[kname] = {"launches": int(cnt.value), "total_ms": tot.value}
+        abi.call("imogen_profile_enable", 0)
+        cells, fb = L.learned_stats()
+        out[name] = {"seconds": el, "ms_per_step": el / args.gasdust_steps * 1e3,
+                     "drag_ms_per_call": prof["drag"]["total_ms"] / max(1, prof["drag"]["launches"]),
+                     "sim_time": sim.time, "fallback_share": (fb / cells) if cells else None,
+                     "dust_mom": fd.state[2:5].copy()}
+        fg.free()
+        fd.free()
+    pe, pl = out["exact"].pop("dust_mom"), out["learned"].pop("dust_mom")
+    rel_l1 = float(np.abs(pl - pe).sum() / np.abs(pe).sum())
+    return {"workload": f"gasdust3d_{n}cubed", "steps": args.gasdust_steps,
+            "step_speedup": out["exact"]["seconds"] / out["learned"]["seconds"],
+            "drag_speedup": out["exact"]["drag_ms_per_call"] / out["learned"]["drag_ms_per_call"],
+            "dust_momentum_rel_l1": rel_l1, "exact": out["exact"], "learned": out["learned"],
+            "model": {"samples": meta.get("samples"), "epochs": meta.get("epochs"),
+                      "residTol": float(norm.residTol)}}
+
+
 def run_cpu_baseline(args, q0, h):
     from oracle import pyoracle as po
 
@@ -153,6 +207,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", action="store_true", help="verify the final state is finite and positive")
     ap.add_argument("--copy-probe", action="store_true", help="only run the 1 GiB copy-kernel calibration")
+    ap.add_argument("--gasdust-n", type=int, default=192)
+    ap.add_argument("--gasdust-steps", type=int, default=1000)
+    ap.add_argument("--no-gasdust", action="store_true")
+    ap.add_argument("--drag-resid-tol", type=float, default=None,
+                    help="override the learned drag's residual tolerance (default: the model file's)")
+    ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -187,6 +247,11 @@ def main():
         ok = bool(np.all(np.isfinite(final)) and np.all(final[0] > 0))
         log(f"[bench] final state finite/positive: {ok}")
 
+    gasdust = None
+    if not args.no_gasdust:
+        log(f"[bench] gas-dust {args.gasdust_n}^3: {args.gasdust_steps} iterations, exact vs learned drag")
+        gasdust = run_gasdust(args)
+
     cpu = None
     if not args.no_cpu:
         log(f"[bench] CPU oracle baseline: {args.cpu_iters} iterations")
@@ -200,7 +265,9 @@ def main():
                        "boundaries": "extrapConstant", "iteration": "CFL + 2 fluidsteps (6 sweeps)",
                        "sim_time_advanced": simtime},
             "roofline": roofline, "cpu_baseline": cpu,
-            "kernels": kern, "drag_speedup_learned_vs_exact": None}
+            "kernels": kern,
+            "drag_speedup_learned_vs_exact": gasdust["step_speedup"] if gasdust else None,
+            "gasdust": gasdust}
     print(json.dumps(line), flush=True)
 
 

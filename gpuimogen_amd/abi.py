@@ -10,7 +10,8 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(HERE, "libimogen_hip.so")
+# IMOGEN_HIP_LIB selects another build of the same library (e.g. a variant under test)
+LIBPATH = os.environ.get("IMOGEN_HIP_LIB") or os.path.join(HERE, "libimogen_hip.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "imogen_abi.h")
 
 # status codes (gpuclass/imogenChecks.h:12-24)

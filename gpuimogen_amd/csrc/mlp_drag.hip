@@ -29,6 +29,7 @@
 namespace imogen {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bfv4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kIn = 5, kHid = 64, kOut = 3;
@@ -87,14 +88,27 @@ __device__ __forceinline__ bf16x4 load_frag(const unsigned short *w, int ld, int
     return r;
 }
 
+// Inference GELU: the same tanh form as x * sigmoid(2 sqrt(2/pi) (x + 0.044715 x^3)),
+// from the hardware 2^x and reciprocal (about 1 ulp of FP32 each); the result
+// is rounded to BF16 right after, 2^-9 relative, far coarser.
+__device__ __forceinline__ float gelu_fast(float x)
+{
+    const float c1 = -2.0f * 0.7978845608028654f * 1.4426950408889634f;
+    const float c3 = c1 * 0.044715f;
+    const float e = __builtin_amdgcn_exp2f(x * fmaf(c3, x * x, c1));
+    return x * __builtin_amdgcn_rcpf(1.0f + e);
+}
+
+// four FP32 values to BF16 (round to nearest even: v_cvt_pk_bf16_f32)
+__device__ __forceinline__ bf16x4 to_bf16x4(const f32x4 &v)
+{
+    return __builtin_bit_cast(bf16x4, __builtin_convertvector(v, bfv4));
+}
+
 __device__ __forceinline__ bf16x4 act_to_b(const f32x4 &d)
 {
-    bf16x4 r;
-    r[0] = (short)f2bf(gelu_f(d[0]));
-    r[1] = (short)f2bf(gelu_f(d[1]));
-    r[2] = (short)f2bf(gelu_f(d[2]));
-    r[3] = (short)f2bf(gelu_f(d[3]));
-    return r;
+    const f32x4 a = {gelu_fast(d[0]), gelu_fast(d[1]), gelu_fast(d[2]), gelu_fast(d[3])};
+    return to_bf16x4(a);
 }
 
 // Network outputs for the 64 cells of a wave: every lane passes its own
@@ -113,11 +127,9 @@ __device__ __forceinline__ void mlp_wave(const MlpDev &m, const float (&f)[5], f
         float x[5];
 #pragma unroll
         for (int j = 0; j < 5; j++) x[j] = __shfl(f[j], src, 64);
-        bf16x4 xb;
-        xb[0] = (short)f2bf(q == 0 ? x[0] : (q == 1 ? x[4] : 0.0f));
-        xb[1] = (short)f2bf(q == 0 ? x[1] : 0.0f);
-        xb[2] = (short)f2bf(q == 0 ? x[2] : 0.0f);
-        xb[3] = (short)f2bf(q == 0 ? x[3] : 0.0f);
+        const f32x4 xf = {q == 0 ? x[0] : (q == 1 ? x[4] : 0.0f), q == 0 ? x[1] : 0.0f, q == 0 ? x[2] : 0.0f,
+                          q == 0 ? x[3] : 0.0f};
+        const bf16x4 xb = to_bf16x4(xf);
         // layer 1: 4 tiles of 16 hidden units
         bf16x4 h1[4];
 #pragma unroll
@@ -210,14 +222,17 @@ __device__ __forceinline__ void drag_features(const DragConsts &c, const DragCel
 // ---------------------------------------------------------------------------
 // learned drag source
 // ---------------------------------------------------------------------------
+// Cells the surrogate does not accept are appended to a queue (one atomic
+// per wave) and integrated by fallback_drag_kernel in dense waves, so a few
+// rejected lanes do not hold their whole wave in the adaptive integrator.
 __global__ __launch_bounds__(256) void learned_drag_kernel(double *__restrict__ g, double *__restrict__ d,
                                                            int64_t pitch, int64_t n, DragConsts c, double dt,
-                                                           MlpDev m, unsigned long long *__restrict__ counts)
+                                                           MlpDev m, int64_t *__restrict__ queue,
+                                                           unsigned long long *__restrict__ qcount)
 {
     const int lane = threadIdx.x & 63;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-    unsigned long long fallbacks = 0;
     for (int64_t base = wave * 64; base < n; base += nwaves * 64) {
         const int64_t i = base + lane;
         const bool live = i < n;
@@ -238,6 +253,7 @@ __global__ __launch_bounds__(256) void learned_drag_kernel(double *__restrict__ 
         float o[3];
         mlp_wave(m, z, o);
         double fac = 1.0;
+        bool reject = false;
         if (moving) {
             const double tau = dt * r0.k;
             const double q0 = inside ? (double)(o[0] * m.norm.tstd[0] + m.norm.tmean[0]) : 0.0;
@@ -254,20 +270,42 @@ __global__ __launch_bounds__(256) void learned_drag_kernel(double *__restrict__ 
             const double e1 = fabs(dt * (k1 - kend));
             const double e2 = fabs(dt * (kbar - ksimp));
             const bool ok = (e1 <= m.norm.residTol) && (e2 <= m.norm.residTol) && kbar > 0.0;
-            if (ok) {
-                fac = exp(-kbar * dt);
-            } else {
-                int steps;
-                fac = drag_exact_ratio(c, s.rg, s.rd, s.s0, s.u0, dt, &steps);
-                fallbacks += live ? 1 : 0;
-            }
+            if (ok) fac = exp(-kbar * dt);
+            else reject = live;
         }
-        if (live) drag_apply(g + i, d + i, pitch, s, fac);
+        const unsigned long long mask = __ballot(reject);
+        if (mask) {
+            const int leader = __ffsll((long long)mask) - 1;
+            unsigned long long slot = 0;
+            if (lane == leader) slot = atomicAdd(qcount, (unsigned long long)__popcll(mask));
+            slot = __shfl(slot, leader, 64);
+            if (reject) queue[slot + __popcll(mask & ((1ull << lane) - 1ull))] = i;
+        }
+        if (live && !reject) drag_apply(g + i, d + i, pitch, s, fac);
     }
-    if (counts) {
-        for (int off = 32; off > 0; off >>= 1) fallbacks += __shfl_xor(fallbacks, off, 64);
-        if (lane == 0 && fallbacks) atomicAdd(counts + 1, fallbacks);
+}
+
+__global__ __launch_bounds__(256) void fallback_drag_kernel(double *__restrict__ g, double *__restrict__ d,
+                                                            int64_t pitch, DragConsts c, double dt,
+                                                            const int64_t *__restrict__ queue,
+                                                            const unsigned long long *__restrict__ qcount,
+                                                            unsigned long long *__restrict__ counts)
+{
+    const unsigned long long nq = *qcount;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && nq) atomicAdd(counts + 1, nq);
+    for (unsigned long long t = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; t < nq;
+         t += (unsigned long long)gridDim.x * blockDim.x) {
+        const int64_t i = queue[t];
+        const DragCell s = drag_load(g + i, d + i, pitch);
+        int steps;
+        const double fac = drag_exact_ratio(c, s.rg, s.rd, s.s0, s.u0, dt, &steps);
+        drag_apply(g + i, d + i, pitch, s, fac);
     }
+}
+
+__global__ void zero_counter(unsigned long long *p)
+{
+    *p = 0;
 }
 
 // standalone BF16 inference (tests)
@@ -543,7 +581,7 @@ struct HostModel {
     void *buf = nullptr;  // device: w1t | w2t | w3t | b1 | b2 | b3
     MlpDev dev{};
     bool loaded = false;
-    unsigned long long *counts = nullptr;  // device [unused, fallbacks]
+    unsigned long long *counts = nullptr;  // device [unused, fallbacks, queue length, unused]
 };
 static HostModel g_model;
 static int64_t g_learnedCells = 0;  // cells through the learned drag since the last stats read
@@ -553,8 +591,8 @@ static int ensure_counts()
     if (g_model.counts) return SUCCESSFUL;
     DeviceState *st = state();
     if (!st) return ERROR_CUDA_BLEW_UP;
-    if (IM_HIP(hipMalloc(&g_model.counts, 2 * sizeof(unsigned long long)))) return ERROR_CUDA_BLEW_UP;
-    if (IM_HIP(hipMemsetAsync(g_model.counts, 0, 2 * sizeof(unsigned long long), st->stream)))
+    if (IM_HIP(hipMalloc(&g_model.counts, 4 * sizeof(unsigned long long)))) return ERROR_CUDA_BLEW_UP;
+    if (IM_HIP(hipMemsetAsync(g_model.counts, 0, 4 * sizeof(unsigned long long), st->stream)))
         return ERROR_CUDA_BLEW_UP;
     return IM_HIP(hipStreamSynchronize(st->stream));
 }
@@ -567,11 +605,21 @@ int launch_learned_drag(double *g, double *d, int64_t pitch, int64_t n, const Dr
     int64_t waves = (n + 63) / 64;
     if (waves > 8192) waves = 8192;
     const int64_t blocks = (waves + 3) / 4;
+    int64_t *queue = (int64_t *)scratch((size_t)n * sizeof(int64_t));
+    if (!queue) return IM_FAIL(ERROR_NOMEM, "no scratch for the drag fallback queue");
+    unsigned long long *qcount = g_model.counts + 2;
     const int slot = prof_begin(PROF_MLP, s);
+    hipLaunchKernelGGL(zero_counter, dim3(1), dim3(1), 0, s, qcount);
+    int rc = IM_LAUNCHED("zero_counter");
+    if (rc) return rc;
     hipLaunchKernelGGL(learned_drag_kernel, dim3((unsigned)blocks), dim3(256), 0, s, g, d, pitch, n, c, dt,
-                       g_model.dev, g_model.counts);
+                       g_model.dev, queue, qcount);
+    rc = IM_LAUNCHED("learned_drag_kernel");
+    if (rc) return rc;
+    hipLaunchKernelGGL(fallback_drag_kernel, dim3(1024), dim3(256), 0, s, g, d, pitch, c, dt, queue, qcount,
+                       g_model.counts);
     prof_end(slot, s);
-    int rc = IM_LAUNCHED("learned_drag_kernel");
+    rc = IM_LAUNCHED("fallback_drag_kernel");
     if (rc) return rc;
     g_learnedCells += n;
     return SUCCESSFUL;

@@ -83,6 +83,41 @@ __device__ __forceinline__ double sqrt_nr(double x)
     return fma(d, h, g);
 }
 
+// Reduced-refinement variants for the sweep arithmetic: one quadratic Newton
+// step for the reciprocal (relative error ~ e^2 + 1 ulp ~ 3e-15) and one
+// Goldschmidt step for the square root (~5e-15). The sweep results stay far
+// inside the 1e-12 parity bar; the correctly rounded quotient (div_cr) keeps
+// the full-precision reciprocal.
+#ifndef IMOGEN_SWEEP_FAST_MATH
+#define IMOGEN_SWEEP_FAST_MATH 1
+#endif
+__device__ __forceinline__ double rcp_fast(double x)
+{
+#if IMOGEN_SWEEP_FAST_MATH
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(r, fma(-x, r, 1.0), r);
+#else
+    return rcp_nr(x);
+#endif
+}
+
+__device__ __forceinline__ double div_fast(double a, double b)
+{
+    return a * rcp_fast(b);
+}
+
+__device__ __forceinline__ double sqrt_fast(double x)
+{
+#if IMOGEN_SWEEP_FAST_MATH
+    const double y = __builtin_amdgcn_rsq(x + 1e-300);
+    const double g = x * y;
+    const double r = fma(-g, 0.5 * y, 0.5);
+    return fma(g, r, g);
+#else
+    return sqrt_nr(x);
+#endif
+}
+
 // ---- half-slope limiters (limiter_functions.h) ----------------------------
 __device__ __forceinline__ double flux_minmod(double a, double b)
 {
@@ -98,12 +133,12 @@ __device__ __forceinline__ double slope_limit(double a, double b, int rt)
     case 0: {  // Ospre: .75 A B (A+B) / (A^2 + AB + B^2)
         double r = a * b;
         double s = a + b;
-        double v = div_nr(.75 * r * s, s * s - r);
+        double v = div_fast(.75 * r * s, s * s - r);
         return (r > 0) ? v : 0.0;
     }
     case 1: {  // van Leer: A B / (A+B)
         double r = a * b;
-        double v = div_nr(r, a + b);
+        double v = div_fast(r, a + b);
         return (r > 0.0) ? v : 0.0;
     }
     case 2: {  // minmod, returned through float as the reference does (:70)
@@ -137,7 +172,7 @@ __device__ __forceinline__ double slope_limit(double a, double b, int rt)
 // cudaFluidStep.cu:1218-1234.
 __device__ __forceinline__ V5 to_prim(const V5 &q)
 {
-    double F = rcp_nr(q.a);
+    double F = rcp_fast(q.a);
     double B = q.b - .5 * (q.c * q.c + q.d * q.d + q.e * q.e) * F;
     return V5{q.a, B, q.c * F, q.d * F, q.e * F};
 }
@@ -182,17 +217,17 @@ __device__ __forceinline__ V5 hllc_flux(const HydroConsts &k, const V5 &L, const
 
     // Einfeldt bounds (cudaFluidStep.cu:1728-1754). The two face densities'
     // reciprocals serve both sound speeds, the Roe ratio and the star flux.
-    const double iL = rcp_nr(L.a);
-    const double iR = rcp_nr(R.a);
+    const double iL = rcp_fast(L.a);
+    const double iR = rcp_fast(R.a);
     double csql = k.gamma * L.b * iL;
     double csqr = k.gamma * R.b * iR;
-    double Fa = sqrt_nr(R.a * iL);
-    double Fb = rcp_nr(1.0 + Fa);
+    double Fa = sqrt_fast(R.a * iL);
+    double Fb = rcp_fast(1.0 + Fa);
     double Croe = ((csql + Fa * csqr) + .5 * k.gm1 * dvsq * Fa * Fb) * Fb;
     double Vroe = (L.c + Fa * R.c) * Fb;
-    Croe = Vroe - sqrt_nr(Croe);
-    double sl = L.c - sqrt_nr(csql);
-    double sr = R.c + sqrt_nr(csqr);
+    Croe = Vroe - sqrt_fast(Croe);
+    double sl = L.c - sqrt_fast(csql);
+    double sr = R.c + sqrt_fast(csqr);
     sl = (Croe < sl) ? Croe : sl;
     Croe = 2 * Vroe - Croe;
     sr = (Croe > sr) ? Croe : sr;
@@ -200,7 +235,7 @@ __device__ __forceinline__ V5 hllc_flux(const HydroConsts &k, const V5 &L, const
     // contact speed, Batten et al. eq. 34 (cudaFluidStep.cu:1352-1357)
     double A = L.a * (sl - L.c);
     double B = R.a * (sr - R.c);
-    double sm = div_nr(B * R.c - R.b - A * L.c + L.b, B - A);
+    double sm = div_fast(B * R.c - R.b - A * L.c + L.b, B - A);
 
     // case selection (cudaFluidStep.cu:1363-1369)
     const bool left = sm > 0;
@@ -222,7 +257,7 @@ __device__ __forceinline__ V5 hllc_flux(const HydroConsts &k, const V5 &L, const
         out.e = S.e * m;
     } else {
         // cases 2/3 (cudaFluidStep.cu:1384-1415)
-        double rb = div_nr(S.a * (sk - S.c), sk - sm);  // rho * beta
+        double rb = div_fast(S.a * (sk - S.c), sk - sm);  // rho * beta
         double G = .5 * ksum;
         double m = rb * sm;
         double Cc = sk * (sm - S.c);
@@ -328,17 +363,17 @@ struct Hll {
         const double Ut = R.b - L.b, St1 = R.d - L.d, St2 = R.e - L.e;
         const double dvsq = Ut * Ut + St1 * St1 + St2 * St2;
         double sl, sr;
-        const double iL = rcp_nr(L.a);
-        const double Fa = sqrt_nr(R.a * iL);
-        const double Fb = rcp_nr(1.0 + Fa);
+        const double iL = rcp_fast(L.a);
+        const double Fa = sqrt_fast(R.a * iL);
+        const double Fb = rcp_fast(1.0 + Fa);
         const double Vroe = (L.b + Fa * R.b) * Fb;
         if (k.ciso == -1) {
             const double csql = k.gamma * L.c * iL;
-            const double csqr = k.gamma * R.c * rcp_nr(R.a);
+            const double csqr = k.gamma * R.c * rcp_fast(R.a);
             double Croe = ((csql + Fa * csqr) + .5 * k.gm1 * dvsq * Fa * Fb) * Fb;
-            Croe = Vroe - sqrt_nr(Croe);
-            sl = L.b - sqrt_nr(csql);
-            sr = R.b + sqrt_nr(csqr);
+            Croe = Vroe - sqrt_fast(Croe);
+            sl = L.b - sqrt_fast(csql);
+            sr = R.b + sqrt_fast(csqr);
             sl = (Croe < sl) ? Croe : sl;
             Croe = 2 * Vroe - Croe;
             sr = (Croe > sr) ? Croe : sr;
@@ -351,7 +386,7 @@ struct Hll {
             cs = 2 * Vroe - cs;
             sr = (cs > sr) ? cs : sr;
         }
-        const double At = rcp_nr(sr - sl);
+        const double At = rcp_fast(sr - sl);
         const double Fl = L.a * L.b, Fr = R.a * R.b;
         V5 o;
         if (sl >= 0 || sr <= 0) {  // one-sided (left unless the right case also holds)

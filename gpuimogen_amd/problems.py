@@ -49,3 +49,47 @@ def sod1d(n=1024, gamma=1.4):
     P = np.where(x < 0.5, 1.0, 0.1)
     z = np.zeros(n)
     return conserved(rho, z, z, z, P, gamma).reshape(5, 1, 1, n), (dx, 1.0, 1.0)
+
+
+def gaussian_field(n, kmax, rng):
+    """Unit-variance Gaussian random field on an n^3 periodic grid whose
+    Fourier modes are white for |k| <= kmax (integer wavenumbers) and zero
+    above: FFT-filtered white noise."""
+    white = rng.standard_normal((n, n, n))
+    k = np.fft.fftfreq(n) * n
+    kz, ky, kx = np.meshgrid(k, k, np.fft.rfftfreq(n) * n, indexing="ij")
+    spec = np.fft.rfftn(white)
+    spec[(kx * kx + ky * ky + kz * kz) > kmax * kmax] = 0.0
+    f = np.fft.irfftn(spec, s=(n, n, n))
+    f -= f.mean()
+    return f / f.std()
+
+
+def gasdust3d(n=192, seed=0, gamma=5.0 / 3.0, eps_median=0.1, eps_sigma_ln=1.0, mach=0.3, kmax=8,
+              dust_ciso=0.01):
+    """Two-fluid gas + dust box, n^3 periodic cells on the unit cube.
+
+    Gas: rho = 1, P = 1, gamma = 5/3. Dust-to-gas ratio lognormal with median
+    eps_median and log-width eps_sigma_ln, from a Gaussian random field with
+    |k| <= kmax. Gas and dust velocities: independent Gaussian random fields
+    (|k| <= kmax) per component with standard deviation mach * c_s. The dust
+    is carried as a nearly pressureless isothermal fluid (sound speed
+    dust_ciso * c_s; HLL isothermal sweeps): its energy slab holds the kinetic
+    energy plus rho_d dust_ciso^2 c_s^2. Returns (gas, dust, h, dust_ciso_abs)."""
+    rng = np.random.default_rng(seed)
+    cs = np.sqrt(gamma * 1.0 / 1.0)
+    rho = np.ones((n, n, n))
+    vg = [mach * cs * gaussian_field(n, kmax, rng) for _ in range(3)]
+    gas = conserved(rho, vg[0], vg[1], vg[2], np.ones((n, n, n)), gamma)
+    del vg
+    eps = eps_median * np.exp(eps_sigma_ln * gaussian_field(n, kmax, rng))
+    rd = eps * rho
+    vd = [mach * cs * gaussian_field(n, kmax, rng) for _ in range(3)]
+    ciso = dust_ciso * cs
+    dust = np.empty((5, n, n, n))
+    dust[0] = rd
+    dust[1] = 0.5 * rd * (vd[0] ** 2 + vd[1] ** 2 + vd[2] ** 2) + rd * ciso * ciso
+    for i in range(3):
+        dust[2 + i] = rd * vd[i]
+    dx = 1.0 / n
+    return gas, dust, (dx, dx, dx), ciso
