@@ -149,6 +149,13 @@ SIGNATURES = {
     "imogen_mlp_infer_bf16": (c_int, [c_int64, c_void_p, c_void_p]),
     "imogen_mlp_mse_grad": (c_int, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
     "imogen_gather_rows": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_void_p]),
+    "imogen_ensemble_create": (c_int, [c_int, _IP, _IP, _DP, _DP, c_double, c_int, c_double, _P(c_void_p)]),
+    "imogen_ensemble_set_state": (c_int, [c_void_p, c_int, _DP]),
+    "imogen_ensemble_get_state": (c_int, [c_void_p, c_int, _DP]),
+    "imogen_ensemble_run": (c_int, [c_void_p, c_int, c_int, c_int]),
+    "imogen_ensemble_times": (c_int, [c_void_p, _DP]),
+    "imogen_ensemble_autotune": (c_int, [c_void_p, c_int, _IP]),
+    "imogen_ensemble_destroy": (c_int, [c_void_p]),
     "imogen_device_alloc": (c_int, [ctypes.c_size_t, _P(c_void_p)]),
     "imogen_device_free": (c_int, [c_void_p]),
     "imogen_memcpy_htod": (c_int, [c_void_p, c_void_p, ctypes.c_size_t]),

@@ -8,70 +8,16 @@
 // the result is deterministic and no host round trip per block is needed.
 // Comparisons use `u < m ? u : m` exactly as the reference does, so NaN cells
 // are skipped as they are there.
+#include "cfl_device.h"
 #include "imogen_internal.h"
 
 namespace imogen {
-
-__device__ __forceinline__ double cs_of(double rho, double E, double px, double py, double pz, double gg1)
-{
-    double rhoinv = 1 / rho;
-    double csq = gg1 * (E - .5 * (px * px + py * py + pz * pz) * rhoinv) * rhoinv;
-    if (csq < 0.0) csq = 0.0;
-    return sqrt(csq);
-}
 
 __global__ void soundspeed_kernel(const double *__restrict__ q, int64_t pitch, int64_t n, double gg1,
                                   double *__restrict__ cs)
 {
     for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x)
         cs[x] = cs_of(q[x], q[pitch + x], q[2 * pitch + x], q[3 * pitch + x], q[4 * pitch + x], gg1);
-}
-
-__device__ __forceinline__ double wave_min(double v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        double o = __shfl_xor(v, off, 64);
-        v = (o < v) ? o : v;
-    }
-    return v;
-}
-
-// dim = simulation dimension (1..3), xinjin selects the per-direction form.
-// fused != 0: sound speed computed in-line from gamma instead of read from cs.
-struct CflArgs {
-    double gg1, hx, hy, hz, hmin;
-    int fused, dim, xinjin;
-};
-
-// time-step limit of one cell (cflTimestep.cu:292-335)
-__device__ __forceinline__ double cfl_cell(const CflArgs &a, double rho, double E, double px, double py,
-                                           double pz, double cIn)
-{
-    const double c = a.fused ? cs_of(rho, E, px, py, pz, a.gg1) : cIn;
-    double u;
-    if (!a.xinjin) {
-        double pm;
-        if (a.dim == 1) {
-            pm = fabs(px);
-        } else {
-            double p = px * px;
-            p = p + py * py;
-            if (a.dim > 2) p = p + pz * pz;
-            pm = sqrt(p);
-        }
-        u = pm / rho + c;
-        u = a.hmin / u;
-    } else {
-        u = a.hx / (fabs(px) / rho + c);
-        double v = a.hy / (fabs(py) / rho + c);
-        u = (u < v) ? u : v;
-        if (a.dim > 2) {
-            v = a.hz / (fabs(pz) / rho + c);
-            u = (u < v) ? u : v;
-        }
-    }
-    return u;
 }
 
 // Two cells per lane per load (16-byte loads) and two such pairs in flight

@@ -1,0 +1,576 @@
+// ensemble.hip -- many independent 2D HLLC problems of uneven sizes, advanced
+// together on one GPU.
+//
+// Every problem keeps the reference's fluid layout (5 slabs, x fastest) in one
+// pooled allocation. One Imogen iteration of the whole ensemble is
+//   per-problem CFL (two batched reduction passes: each problem's time step,
+//   lambda = dt/h per direction and its clock stay on the device)
+//   fluidstep(+1), fluidstep(-1): per direction one batched sweep launch over
+//   the lines of all problems, then one batched boundary launch,
+// with the sweep order of gpuclass/flux.cu:46 (z skipped: the problems are 2D).
+// Every kernel reads its problem's lambda from device memory, so an iteration
+// needs no host round trip and is captured once per sweep-order residue
+// (iteration mod 6) as a HIP graph and replayed.
+//
+// Work decomposition: X sweeps group problems by line length into the
+// register-blocking classes of the single-array X kernel (K cells per lane,
+// one launch per class, one wave per group of whole lines); Y sweeps use the
+// column march with a per-problem segment count S (columns split in S
+// segments held by lanes of one wave). The autotuner picks S per size class
+// by timing the candidates on the ensemble itself.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "cfl_device.h"
+#include "imogen_internal.h"
+#include "sweep_kernels.h"
+
+namespace imogen {
+
+struct EnsProblem {
+    double *q;
+    int64_t pitch;  // doubles between slabs
+    int nx, ny;
+    double hx, hy;
+};
+
+struct EnsItem {
+    int p;   // problem
+    int a;   // X: first line; Y: first column
+    int b;   // Y: segments S
+    int pad;
+};
+
+constexpr int kEnsCflBlocks = 32;  // CFL partial minima per problem
+static const int kXClassK[6] = {1, 2, 3, 4, 6, 8};
+
+static int x_class_of(int N)
+{
+    if (N <= 64) return 0;
+    if (N <= 128) return 1;
+    if (N <= 192) return 2;
+    if (N <= 256) return 3;
+    if (N <= 384) return 4;
+    if (N <= 512) return 5;
+    return -1;
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void ens_cfl_pass1(const EnsProblem *__restrict__ probs, double gg1,
+                                                     double *__restrict__ part)
+{
+    const EnsProblem P = probs[blockIdx.y];
+    const int64_t n = (int64_t)P.nx * P.ny;
+    CflArgs a{gg1, P.hx, P.hy, 1.0, P.hx < P.hy ? P.hx : P.hy, 1, 2, 0};
+    double m = 1e37;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double *c = P.q + i;
+        const double u = cfl_cell(a, c[0], c[P.pitch], c[2 * P.pitch], c[3 * P.pitch], c[4 * P.pitch], 0.0);
+        m = (u < m) ? u : m;
+    }
+    m = wave_min(m);
+    __shared__ double w[4];
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = w[0];
+        for (int k = 1; k < (int)(blockDim.x >> 6); k++) r = (w[k] < r) ? w[k] : r;
+        part[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = r;
+    }
+}
+
+// one wave per problem: tau (halved, HLLC rule cflTimestep.cu:232-237), dt,
+// lambda per direction, and the problem's clock t += 2 dt
+__global__ __launch_bounds__(64) void ens_cfl_pass2(const EnsProblem *__restrict__ probs, const double *part,
+                                                    int nb, double cfl, double *__restrict__ lam,
+                                                    double *__restrict__ clock)
+{
+    const int p = blockIdx.x;
+    double m = 1e38;
+    for (int k = threadIdx.x; k < nb; k += 64) m = (part[(int64_t)p * nb + k] < m) ? part[(int64_t)p * nb + k] : m;
+    m = wave_min(m);
+    if (threadIdx.x == 0) {
+        const double dt = cfl * (m / 2);
+        lam[2 * p] = dt / probs[p].hx;
+        lam[2 * p + 1] = dt / probs[p].hy;
+        clock[p] += 2 * dt;
+    }
+}
+
+// X sweep of whole lines, K cells per lane; one work item (problem, first line) per wave
+template <int K, int LIM>
+__global__ __launch_bounds__(256) void ens_sweep_x(const EnsProblem *__restrict__ probs,
+                                                   const EnsItem *__restrict__ work, int nwork,
+                                                   const double *__restrict__ lam, HydroConsts hc, int rt,
+                                                   double *__restrict__ dummy)
+{
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (wave >= nwork) return;
+    const int lane = threadIdx.x & 63;
+    const EnsItem it = work[wave];
+    const EnsProblem P = probs[it.p];
+    const double lambda = lam[2 * it.p];
+    const int N = P.nx;
+    const int nl = (N + K - 1) / K;
+    const int lpw = 64 / nl;
+    const int sub = lane / nl;
+    const int j = lane - sub * nl;
+    const int line = it.a + sub;
+    const bool own = sub < lpw && line < P.ny;
+    const int lbase = sub * nl;
+    const bool laneUsed = sub < lpw;
+    const int srcL = laneUsed ? lbase + (j == 0 ? nl - 1 : j - 1) : lane;
+    const int srcR = laneUsed ? lbase + (j == nl - 1 ? 0 : j + 1) : lane;
+    const int x0 = laneUsed ? j * K : 0;
+    int cnt = N - x0;
+    cnt = cnt < K ? cnt : K;
+    if (!laneUsed) cnt = 0;
+    const int lc = line < P.ny ? line : P.ny - 1;
+    double *b = P.q + (int64_t)lc * N + x0;
+    V5 Q[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) Q[k] = (k < cnt) ? load_cell<Hllc, 1>(b + k, P.pitch) : V5{1.0, 1.0, 0.0, 0.0, 0.0};
+    V5 F[K], Fm;
+    x_fluxes<Hllc, K, LIM>(hc, rt, Q, cnt, srcL, srcR, F, Fm);
+    V5 U1[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) U1[k] = axpy_update(Q[k], lambda, F[k], k == 0 ? Fm : F[k > 0 ? k - 1 : 0]);
+    x_fluxes<Hllc, K, LIM>(hc, rt, U1, cnt, srcL, srcR, F, Fm);
+    const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nthr = (int64_t)gridDim.x * blockDim.x;
+    double *dst = own ? b : dummy + gtid;
+    const int64_t st = own ? P.pitch : nthr;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        if (k < cnt) {
+            const V5 o = avg_update(add5(U1[k], Q[k]), lambda, F[k], k == 0 ? Fm : F[k > 0 ? k - 1 : 0]);
+            store_cell<Hllc, 1>(own ? dst + k : dst, st, o);
+        }
+    }
+}
+
+// Y sweep: column march, one work item (problem, first column, S) per wave
+template <int LIM>
+__global__ __launch_bounds__(256) void ens_sweep_y(const EnsProblem *__restrict__ probs,
+                                                   const EnsItem *__restrict__ work, int nwork,
+                                                   const double *__restrict__ lam, HydroConsts hc, int rt,
+                                                   double *__restrict__ defer)
+{
+    __shared__ double sring[4][4][5][64];
+    const int wib = threadIdx.x >> 6;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (wave >= nwork) return;
+    const int lane = threadIdx.x & 63;
+    const EnsItem it = work[wave];
+    const EnsProblem P = probs[it.p];
+    const double lambda = lam[2 * it.p + 1];
+    const int S = it.b;
+    const int W = 64 / S;
+    const int s = lane / W;
+    const int c = lane - s * W;
+    const int x = it.a + c;
+    const bool colValid = (x < P.nx) && (s < S);
+    const int N = P.ny;
+    const int seglen = (N + S - 1) / S;
+    const int j0 = s * seglen;
+    int L = N - j0;
+    L = L < seglen ? L : seglen;
+    if (!colValid) L = 0;
+    double *col = P.q + (colValid ? x : 0);
+    const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nthr = (int64_t)gridDim.x * blockDim.x;
+    march_body<Hllc, 2, LIM>(col, P.pitch, (int64_t)P.nx, N, seglen, j0, L, lambda, hc, rt, defer, gtid, nthr,
+                             sring[wib], lane);
+}
+
+// constant-extrapolation ghost cells (cudaStatics.cu:1339-1366, 1440-1466 with
+// the +y innermost-ghost convention), depth d, both faces of one direction;
+// one block per problem
+__global__ __launch_bounds__(256) void ens_bc_const(const EnsProblem *__restrict__ probs, int dir, int d)
+{
+    const EnsProblem P = probs[blockIdx.x];
+    const int nx = P.nx, ny = P.ny;
+    if (dir == 1) {
+        if (nx < d + 2) return;
+        const int64_t total = (int64_t)ny * 5 * 2 * d;
+        for (int64_t t = threadIdx.x; t < total; t += blockDim.x) {
+            const int g = (int)(t % d);
+            int64_t r = t / d;
+            const int side = (int)(r % 2);
+            r /= 2;
+            const int slab = (int)(r % 5);
+            const int64_t row = r / 5;
+            double *a = P.q + slab * P.pitch + row * nx;
+            if (side == 0) a[g] = a[d];
+            else a[nx - d + g] = a[nx - d - 1];
+        }
+    } else {
+        if (ny < d + 2) return;
+        const int64_t total = (int64_t)nx * 5 * 2;
+        for (int64_t t = threadIdx.x; t < total; t += blockDim.x) {
+            const int x = (int)(t % nx);
+            int64_t r = t / nx;
+            const int side = (int)(r % 2);
+            const int slab = (int)(r / 2);
+            double *a = P.q + slab * P.pitch + x;
+            if (side == 0) {
+                const double f = a[(int64_t)d * nx];
+                for (int g = 0; g < d; g++) a[(int64_t)g * nx] = f;
+            } else {
+                const double f = a[(int64_t)(ny - d) * nx];
+                for (int g = 0; g < d; g++) a[(int64_t)(ny - d + g) * nx] = f;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct EnsembleImpl {
+    int P = 0;
+    double gamma = 1.4, cfl = 0.4;
+    int limiter = 0;
+    std::vector<EnsProblem> host;
+    std::vector<int> segS;  // chosen Y segments per problem
+    double *pool = nullptr;
+    EnsProblem *dprobs = nullptr;
+    double *lam = nullptr, *clock = nullptr, *cflPart = nullptr;
+    EnsItem *xwork[6] = {nullptr};
+    int nx_work[6] = {0};
+    EnsItem *ywork = nullptr;
+    int ny_work = 0;
+    double *xdummy = nullptr, *ydefer = nullptr;
+    int64_t xThreads = 0, yThreads = 0;
+    hipGraphExec_t graph[6] = {nullptr};
+    int64_t cells = 0;
+};
+
+static void free_graphs(EnsembleImpl *e)
+{
+    for (int k = 0; k < 6; k++) {
+        if (e->graph[k]) (void)hipGraphExecDestroy(e->graph[k]);
+        e->graph[k] = nullptr;
+    }
+}
+
+static int build_work(EnsembleImpl *e)
+{
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    // the tables below are replaced while no kernel can still read them
+    if (IM_HIP(hipStreamSynchronize(st->stream))) return ERROR_CUDA_BLEW_UP;
+    std::vector<EnsItem> xw[6], yw;
+    for (int p = 0; p < e->P; p++) {
+        const EnsProblem &P = e->host[p];
+        const int cls = x_class_of(P.nx);
+        const int K = kXClassK[cls];
+        const int nl = (P.nx + K - 1) / K;
+        const int lpw = 64 / nl;
+        for (int l = 0; l < P.ny; l += lpw) xw[cls].push_back(EnsItem{p, l, 0, 0});
+        const int S = e->segS[p];
+        const int W = 64 / S;
+        for (int x = 0; x < P.nx; x += W) yw.push_back(EnsItem{p, x, S, 0});
+    }
+    for (int c = 0; c < 6; c++) {
+        if (e->xwork[c]) (void)hipFree(e->xwork[c]);
+        e->xwork[c] = nullptr;
+        e->nx_work[c] = (int)xw[c].size();
+        if (xw[c].empty()) continue;
+        if (IM_HIP(hipMalloc(&e->xwork[c], xw[c].size() * sizeof(EnsItem)))) return ERROR_CUDA_BLEW_UP;
+        if (IM_HIP(hipMemcpy(e->xwork[c], xw[c].data(), xw[c].size() * sizeof(EnsItem), hipMemcpyHostToDevice)))
+            return ERROR_CUDA_BLEW_UP;
+    }
+    if (e->ywork) (void)hipFree(e->ywork);
+    e->ny_work = (int)yw.size();
+    if (IM_HIP(hipMalloc(&e->ywork, yw.size() * sizeof(EnsItem)))) return ERROR_CUDA_BLEW_UP;
+    if (IM_HIP(hipMemcpy(e->ywork, yw.data(), yw.size() * sizeof(EnsItem), hipMemcpyHostToDevice)))
+        return ERROR_CUDA_BLEW_UP;
+    // per-lane scratch: X dummy slots (5 doubles), Y deferred outputs (25 doubles)
+    int64_t xmax = 0;
+    for (int c = 0; c < 6; c++) xmax = std::max<int64_t>(xmax, e->nx_work[c]);
+    const int64_t xt = ((xmax + 3) / 4) * 256, yt = (((int64_t)e->ny_work + 3) / 4) * 256;
+    if (xt > e->xThreads) {
+        if (e->xdummy) (void)hipFree(e->xdummy);
+        if (IM_HIP(hipMalloc(&e->xdummy, (size_t)xt * 5 * sizeof(double)))) return ERROR_CUDA_BLEW_UP;
+        e->xThreads = xt;
+    }
+    if (yt > e->yThreads) {
+        if (e->ydefer) (void)hipFree(e->ydefer);
+        if (IM_HIP(hipMalloc(&e->ydefer, (size_t)yt * 25 * sizeof(double)))) return ERROR_CUDA_BLEW_UP;
+        e->yThreads = yt;
+    }
+    free_graphs(e);
+    return SUCCESSFUL;
+}
+
+template <int LIM>
+static int ens_sweep(EnsembleImpl *e, int dir, const HydroConsts &hc, int rt, hipStream_t s)
+{
+    if (dir == 1) {
+        for (int c = 0; c < 6; c++) {
+            if (!e->nx_work[c]) continue;
+            const unsigned blocks = (unsigned)((e->nx_work[c] + 3) / 4);
+#define IM_EX(KK)                                                                                              \
+    hipLaunchKernelGGL((ens_sweep_x<KK, LIM>), dim3(blocks), dim3(256), 0, s, e->dprobs, e->xwork[c],           \
+                       e->nx_work[c], e->lam, hc, rt, e->xdummy)
+            switch (kXClassK[c]) {
+            case 1: IM_EX(1); break;
+            case 2: IM_EX(2); break;
+            case 3: IM_EX(3); break;
+            case 4: IM_EX(4); break;
+            case 6: IM_EX(6); break;
+            case 8: IM_EX(8); break;
+            }
+#undef IM_EX
+            int rc = IM_LAUNCHED("ens_sweep_x");
+            if (rc) return rc;
+        }
+    } else {
+        const unsigned blocks = (unsigned)((e->ny_work + 3) / 4);
+        hipLaunchKernelGGL((ens_sweep_y<LIM>), dim3(blocks), dim3(256), 0, s, e->dprobs, e->ywork, e->ny_work,
+                           e->lam, hc, rt, e->ydefer);
+        int rc = IM_LAUNCHED("ens_sweep_y");
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(ens_bc_const, dim3((unsigned)e->P), dim3(256), 0, s, e->dprobs, dir, 4);
+    return IM_LAUNCHED("ens_bc_const");
+}
+
+// one iteration: CFL, fluidstep(+1), fluidstep(-1)
+static int ens_iteration(EnsembleImpl *e, int it, hipStream_t s)
+{
+    static const int kFluxcall[3][6] = {{1, 2, 1, 2, 3, 3}, {3, 1, 2, 3, 1, 2}, {2, 3, 3, 1, 2, 1}};
+    HydroConsts hc;
+    hc.gamma = e->gamma;
+    hc.gm1 = e->gamma - 1.0;
+    hc.govergm1 = e->gamma / (e->gamma - 1.0);
+    hc.rhomin = 0.0;
+    hc.emin = 0.0;
+    hc.ciso = -1.0;
+    hc.cisosq = 1.0;
+    hipLaunchKernelGGL(ens_cfl_pass1, dim3(kEnsCflBlocks, (unsigned)e->P), dim3(256), 0, s, e->dprobs,
+                       e->gamma * (e->gamma - 1.0), e->cflPart);
+    int rc = IM_LAUNCHED("ens_cfl_pass1");
+    if (rc) return rc;
+    hipLaunchKernelGGL(ens_cfl_pass2, dim3((unsigned)e->P), dim3(64), 0, s, e->dprobs, e->cflPart, kEnsCflBlocks,
+                       e->cfl, e->lam, e->clock);
+    rc = IM_LAUNCHED("ens_cfl_pass2");
+    if (rc) return rc;
+    for (int dirn = 1; dirn >= -1; dirn -= 2) {
+        int sweep = (it + 3 * (dirn > 0)) % 6;
+        for (int n = 0; n < 3; n++) {
+            const int d = kFluxcall[n][sweep];
+            if (d == 3) continue;  // 2D problems
+            switch (e->limiter) {
+            case LIMITER_OSPRE: rc = ens_sweep<0>(e, d, hc, 0, s); break;
+            case LIMITER_VANLEER: rc = ens_sweep<1>(e, d, hc, 1, s); break;
+            default: rc = ens_sweep<-1>(e, d, hc, e->limiter, s); break;
+            }
+            if (rc) return rc;
+        }
+    }
+    return SUCCESSFUL;
+}
+
+}  // namespace imogen
+
+using namespace imogen;
+
+struct ImogenEnsemble {
+    EnsembleImpl impl;
+};
+
+extern "C" {
+
+int imogen_ensemble_create(int nprob, const int *nx, const int *ny, const double *hx, const double *hy, double gamma,
+                           int limiter, double cfl, ImogenEnsemble **out)
+{
+    if (!out || !nx || !ny || !hx || !hy) return IM_FAIL(ERROR_NULL_POINTER, "null argument");
+    *out = nullptr;
+    if (nprob <= 0) return IM_FAIL(ERROR_INVALID_ARGS, "need at least one problem");
+    if (limiter < 0 || limiter > LIMITER_ZERO) return IM_FAIL(ERROR_INVALID_ARGS, "unknown slope limiter %d", limiter);
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    ImogenEnsemble *E = new ImogenEnsemble();
+    EnsembleImpl *e = &E->impl;
+    e->P = nprob;
+    e->gamma = gamma;
+    e->cfl = cfl;
+    e->limiter = limiter;
+    int64_t total = 0;
+    std::vector<int64_t> off(nprob);
+    for (int p = 0; p < nprob; p++) {
+        if (nx[p] < 8 || ny[p] < 8 || x_class_of(nx[p]) < 0) {
+            delete E;
+            return IM_FAIL(ERROR_INVALID_ARGS, "problem %d: %d x %d (need 8..512 cells per direction)", p, nx[p],
+                           ny[p]);
+        }
+        const int64_t n = (int64_t)nx[p] * ny[p];
+        const int64_t pitch = (n + 31) / 32 * 32;  // 256-byte slabs
+        off[p] = total;
+        total += 5 * pitch;
+        e->host.push_back(EnsProblem{nullptr, pitch, nx[p], ny[p], hx[p], hy[p]});
+        e->cells += n;
+        e->segS.push_back(ny[p] >= 256 ? 2 : 1);
+    }
+    int rc = SUCCESSFUL;
+    if (IM_HIP(hipMalloc(&e->pool, (size_t)total * sizeof(double)))) rc = ERROR_CUDA_BLEW_UP;
+    for (int p = 0; p < nprob && !rc; p++) e->host[p].q = e->pool + off[p];
+    if (!rc && IM_HIP(hipMalloc(&e->dprobs, nprob * sizeof(EnsProblem)))) rc = ERROR_CUDA_BLEW_UP;
+    if (!rc && IM_HIP(hipMemcpy(e->dprobs, e->host.data(), nprob * sizeof(EnsProblem), hipMemcpyHostToDevice)))
+        rc = ERROR_CUDA_BLEW_UP;
+    if (!rc && IM_HIP(hipMalloc(&e->lam, 2 * nprob * sizeof(double)))) rc = ERROR_CUDA_BLEW_UP;
+    if (!rc && IM_HIP(hipMalloc(&e->clock, nprob * sizeof(double)))) rc = ERROR_CUDA_BLEW_UP;
+    if (!rc && IM_HIP(hipMemset(e->clock, 0, nprob * sizeof(double)))) rc = ERROR_CUDA_BLEW_UP;
+    if (!rc && IM_HIP(hipMalloc(&e->cflPart, (size_t)nprob * kEnsCflBlocks * sizeof(double)))) rc = ERROR_CUDA_BLEW_UP;
+    if (!rc) rc = build_work(e);
+    if (rc) {
+        imogen_ensemble_destroy(E);
+        return rc;
+    }
+    *out = E;
+    return SUCCESSFUL;
+}
+
+int imogen_ensemble_set_state(ImogenEnsemble *E, int p, const double *host)
+{
+    if (!E || !host) return IM_FAIL(ERROR_NULL_POINTER, "null argument");
+    EnsembleImpl *e = &E->impl;
+    if (p < 0 || p >= e->P) return IM_FAIL(ERROR_INVALID_ARGS, "problem %d", p);
+    DeviceState *st = state();
+    const EnsProblem &P = e->host[p];
+    const int64_t n = (int64_t)P.nx * P.ny;
+    for (int k = 0; k < 5; k++)
+        if (IM_HIP(hipMemcpyAsync(P.q + k * P.pitch, host + k * n, n * sizeof(double), hipMemcpyHostToDevice,
+                                  st->stream)))
+            return ERROR_CUDA_BLEW_UP;
+    return IM_HIP(hipStreamSynchronize(st->stream));
+}
+
+int imogen_ensemble_get_state(ImogenEnsemble *E, int p, double *host)
+{
+    if (!E || !host) return IM_FAIL(ERROR_NULL_POINTER, "null argument");
+    EnsembleImpl *e = &E->impl;
+    if (p < 0 || p >= e->P) return IM_FAIL(ERROR_INVALID_ARGS, "problem %d", p);
+    DeviceState *st = state();
+    const EnsProblem &P = e->host[p];
+    const int64_t n = (int64_t)P.nx * P.ny;
+    for (int k = 0; k < 5; k++)
+        if (IM_HIP(hipMemcpyAsync(host + k * n, P.q + k * P.pitch, n * sizeof(double), hipMemcpyDeviceToHost,
+                                  st->stream)))
+            return ERROR_CUDA_BLEW_UP;
+    return IM_HIP(hipStreamSynchronize(st->stream));
+}
+
+int imogen_ensemble_run(ImogenEnsemble *E, int firstIteration, int iterations, int useGraph)
+{
+    if (!E) return IM_FAIL(ERROR_NULL_POINTER, "null ensemble");
+    EnsembleImpl *e = &E->impl;
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    for (int k = 0; k < iterations; k++) {
+        const int it = firstIteration + k;
+        if (!useGraph) {
+            int rc = ens_iteration(e, it, st->stream);
+            if (rc) return rc;
+            continue;
+        }
+        hipGraphExec_t &g = e->graph[it % 6];
+        if (!g) {
+            hipGraph_t graph;
+            if (IM_HIP(hipStreamBeginCapture(st->stream, hipStreamCaptureModeThreadLocal))) return ERROR_CUDA_BLEW_UP;
+            int rc = ens_iteration(e, it, st->stream);
+            hipError_t ec = hipStreamEndCapture(st->stream, &graph);
+            if (rc) return rc;
+            if (IM_HIP(ec)) return ERROR_CUDA_BLEW_UP;
+            ec = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (IM_HIP(ec)) return ERROR_CUDA_BLEW_UP;
+        }
+        if (IM_HIP(hipGraphLaunch(g, st->stream))) return ERROR_CUDA_BLEW_UP;
+    }
+    return SUCCESSFUL;
+}
+
+int imogen_ensemble_times(ImogenEnsemble *E, double *t)
+{
+    if (!E || !t) return IM_FAIL(ERROR_NULL_POINTER, "null argument");
+    EnsembleImpl *e = &E->impl;
+    DeviceState *st = state();
+    if (IM_HIP(hipMemcpyAsync(t, e->clock, e->P * sizeof(double), hipMemcpyDeviceToHost, st->stream)))
+        return ERROR_CUDA_BLEW_UP;
+    return IM_HIP(hipStreamSynchronize(st->stream));
+}
+
+int imogen_ensemble_autotune(ImogenEnsemble *E, int trials, int *chosen)
+{
+    if (!E) return IM_FAIL(ERROR_NULL_POINTER, "null ensemble");
+    EnsembleImpl *e = &E->impl;
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    if (trials < 1) trials = 1;
+    // Y-march segments per size class (ny <= 128, <= 256, <= 512) among {1, 2, 4},
+    // one class at a time, timing whole iterations with the other classes fixed.
+    // The state advances during tuning; callers tune before loading their data
+    // or reload it afterwards.
+    static const int cand[3] = {1, 2, 4};
+    auto cls_of = [](int ny) { return ny <= 128 ? 0 : (ny <= 256 ? 1 : 2); };
+    hipEvent_t a, b;
+    if (IM_HIP(hipEventCreate(&a)) || IM_HIP(hipEventCreate(&b))) return ERROR_CUDA_BLEW_UP;
+    int best[3];
+    for (int cls = 0; cls < 3; cls++) {
+        float bestMs = 1e30f;
+        best[cls] = 1;
+        for (int ci = 0; ci < 3; ci++) {
+            for (int p = 0; p < e->P; p++)
+                if (cls_of(e->host[p].ny) == cls) e->segS[p] = cand[ci];
+            int rc = build_work(e);
+            if (!rc) rc = ens_iteration(e, 0, st->stream);  // warm
+            if (!rc) rc = IM_HIP(hipEventRecord(a, st->stream));
+            for (int k = 0; k < trials && !rc; k++) rc = ens_iteration(e, k, st->stream);
+            if (!rc) rc = IM_HIP(hipEventRecord(b, st->stream));
+            if (!rc) rc = IM_HIP(hipEventSynchronize(b));
+            float ms = 0;
+            if (!rc) rc = IM_HIP(hipEventElapsedTime(&ms, a, b));
+            if (rc) {
+                (void)hipEventDestroy(a);
+                (void)hipEventDestroy(b);
+                return rc;
+            }
+            if (ms < bestMs) {
+                bestMs = ms;
+                best[cls] = cand[ci];
+            }
+        }
+        for (int p = 0; p < e->P; p++)
+            if (cls_of(e->host[p].ny) == cls) e->segS[p] = best[cls];
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (chosen)
+        for (int c = 0; c < 3; c++) chosen[c] = best[c];
+    if (IM_HIP(hipMemset(e->clock, 0, e->P * sizeof(double)))) return ERROR_CUDA_BLEW_UP;
+    return build_work(e);
+}
+
+int imogen_ensemble_destroy(ImogenEnsemble *E)
+{
+    if (!E) return SUCCESSFUL;
+    EnsembleImpl *e = &E->impl;
+    DeviceState *st = state();
+    if (st) (void)hipStreamSynchronize(st->stream);
+    free_graphs(e);
+    for (int c = 0; c < 6; c++)
+        if (e->xwork[c]) (void)hipFree(e->xwork[c]);
+    void *bufs[] = {e->pool, e->dprobs, e->lam, e->clock, e->cflPart, e->ywork, e->xdummy, e->ydefer};
+    for (void *p : bufs)
+        if (p) (void)hipFree(p);
+    delete E;
+    return SUCCESSFUL;
+}
+
+}  // extern "C"

@@ -310,6 +310,36 @@ int imogen_mlp_adam(float *params, const float *grads, float *m, float *v, int64
 int imogen_mlp_infer_bf16(int64_t batch, const float *x, float *y);
 
 /* ======================================================================== */
+/* Ensembles of independent 2D problems (no counterpart in the reference)   */
+/* ======================================================================== */
+
+/* Many independent 2D HLLC problems of uneven size (each 8..512 cells per
+ * direction, constant-extrapolation boundaries, the same gamma, limiter and
+ * CFL number), advanced together: each iteration computes every problem's own
+ * CFL time step on the device and applies fluidstep(+1), fluidstep(-1) with
+ * batched sweeps, optionally replayed from HIP graphs. Problem p's state is
+ * 5 x ny[p] x nx[p] doubles (rho, E, px, py, pz; x fastest), grid spacing
+ * hx[p], hy[p]. */
+typedef struct ImogenEnsemble ImogenEnsemble;
+int imogen_ensemble_create(int nprob, const int *nx, const int *ny, const double *hx, const double *hy,
+                           double gamma, int limiter, double cfl, ImogenEnsemble **out);
+int imogen_ensemble_set_state(ImogenEnsemble *e, int p, const double *host);
+int imogen_ensemble_get_state(ImogenEnsemble *e, int p, double *host);
+/* Runs iterations firstIteration .. firstIteration+iterations-1 (the
+ * iteration number picks the sweep order as stepNumber does); useGraph != 0
+ * captures one HIP graph per sweep-order residue and replays it. Returns
+ * without waiting for the device. */
+int imogen_ensemble_run(ImogenEnsemble *e, int firstIteration, int iterations, int useGraph);
+/* Simulation time reached by every problem (nprob doubles; waits). */
+int imogen_ensemble_times(ImogenEnsemble *e, double *t);
+/* Times `trials` iterations per candidate and keeps, per size class (ny <= 128,
+ * <= 256, <= 512), the fastest segment count S in {1, 2, 4} of the Y-sweep
+ * march; chosen[3] (optional) receives them. The ensemble state advances
+ * during tuning and the clocks are reset to 0 afterwards. */
+int imogen_ensemble_autotune(ImogenEnsemble *e, int trials, int *chosen);
+int imogen_ensemble_destroy(ImogenEnsemble *e);
+
+/* ======================================================================== */
 /* Runtime utilities (no counterpart in the reference: it used the legacy   */
 /* default stream; this library issues all work on one stream per device).  */
 /* ======================================================================== */
