@@ -88,11 +88,39 @@ def run_gpu(args, q0, h):
     return elapsed, kern, simtime, final
 
 
+def run_device_loop(args, q0, h):
+    """The same iterations with dTime kept on the device and every iteration
+    replayed from a HIP graph (imogen.DeviceLoop): no host round trip per CFL
+    reduction. Same kernels and bitwise the same results as run_gpu."""
+    from gpuimogen_amd import abi, gpuclass as gc, imogen
+
+    n = q0.shape[-1]
+    f = gc.FluidManager(q0, 5.0 / 3.0, minmass=1e-8, bc_modes=(abi.extrapConstant,) * 6)
+    gm = gc.GeometryManager((n, n, n), h)
+    loop = imogen.DeviceLoop([f], [abi.METHOD_HLLC], gm, cfl=args.cfl, limiter=LIMITERS[args.limiter])
+    loop.run(args.warmup, graph=True)  # with warmup >= 6 this captures all six sweep-order graphs
+    gc.synchronize()
+    t0 = time.perf_counter()
+    done = 0
+    chunk = max(1, args.steps // 4)
+    while done < args.steps:
+        k = min(chunk, args.steps - done)
+        loop.run(k, graph=True)
+        done += k
+        gc.synchronize()
+        log(f"[bench] device loop: step {done}/{args.steps}")
+    elapsed = time.perf_counter() - t0
+    simtime = loop.time
+    loop.free()
+    f.free()
+    return elapsed, simtime
+
+
 def run_gasdust(args):
-    """Gas + dust on a periodic n^3 box (HLLC gas, nearly pressureless HLL
-    isothermal dust, LogTrap-style drag source every half step as in
-    imogen.m): the same run with the adaptive drag integrator and with the
-    learned surrogate, from the same initial state."""
+    """Gas + dust on a periodic n^3 box (HLLC gas, HLL isothermal dust with
+    sound speed 0.1, drag source around each fluid step as in imogen.m): the
+    same run with the adaptive drag integrator and with the learned
+    surrogate, from the same initial state."""
     from gpuimogen_amd import abi, gpuclass as gc, imogen, learned_drag as L, problems
 
     n = args.gasdust_n
@@ -108,15 +136,18 @@ def run_gasdust(args):
         fg = gc.FluidManager(gas0, tg.gamma, minmass=1e-8, bc_modes=(abi.circular,) * 6, thermoDetails=tg)
         fd = gc.FluidManager(dust0, td.gamma, minmass=1e-8, bc_modes=(abi.circular,) * 6, cisothermal=ciso,
                              thermoDetails=td)
-        sim = imogen.Simulation([fg, fd], [abi.METHOD_HLLC, abi.METHOD_HLL], gm, cfl=0.4, drag_method=method)
+        sim = imogen.Simulation([fg, fd], [abi.METHOD_HLLC, abi.METHOD_HLLC], gm, cfl=0.4, drag_method=method)
         gc.synchronize()
         abi.call("imogen_profile_enable", 1)
         L.learned_stats()
         t0 = time.perf_counter()
+        last = t0
         for k in range(args.gasdust_steps):
             sim.step()
-            if (k + 1) % max(1, args.gasdust_steps // 4) == 0:
-                log(f"[bench] gas-dust {name}: step {k + 1}/{args.gasdust_steps}")
+            now = time.perf_counter()
+            if (k + 1) % max(1, args.gasdust_steps // 4) == 0 or now - last > 20:
+                log(f"[bench] gas-dust {name}: step {k + 1}/{args.gasdust_steps} ({now - t0:.1f} s)")
+                last = now
         gc.synchronize()
         el = time.perf_counter() - t0
         prof = {}
@@ -140,6 +171,51 @@ def run_gasdust(args):
             "dust_momentum_rel_l1": rel_l1, "exact": out["exact"], "learned": out["learned"],
             "model": {"samples": meta.get("samples"), "epochs": meta.get("epochs"),
                       "residTol": float(norm.residTol)}}
+
+
+def run_ensemble(args):
+    """The batched ensemble: `count` independent 2D four-quadrant HLLC problems
+    with log-uniform sizes, per-problem CFL 0.4 on the device, one launch per
+    sweep direction over all problems, each iteration a replayed hipGraph. The
+    march segment counts are autotuned on the ensemble itself, then the
+    initial states are reloaded."""
+    from gpuimogen_amd import ensemble as ens, gpuclass as gc
+
+    sizes, states = ens.quadrant_problems(args.ensemble_count, seed=args.seed)
+    e = ens.Ensemble(sizes, sizes, 1.0 / sizes, 1.0 / sizes)
+    for p, s in enumerate(states):
+        e.set_state(p, s)
+    t0 = time.perf_counter()
+    chosen = e.autotune(trials=2)
+    tune_s = time.perf_counter() - t0
+    for p, s in enumerate(states):
+        e.set_state(p, s)
+    e.iteration = 0
+    e.run(args.warmup, graph=True)
+    gc.synchronize()
+    t0 = time.perf_counter()
+    done = 0
+    chunk = max(1, args.ensemble_steps // 4)
+    while done < args.ensemble_steps:
+        k = min(chunk, args.ensemble_steps - done)
+        e.run(k, graph=True)
+        done += k
+        log(f"[bench] ensemble: iteration {done}/{args.ensemble_steps}")
+    gc.synchronize()
+    el = time.perf_counter() - t0
+    cells = int(np.sum(sizes.astype(np.int64) ** 2))
+    times = e.times()
+    ok = all(bool(np.all(np.isfinite(e.state(p)))) for p in (0, len(sizes) // 2, len(sizes) - 1))
+    e.free()
+    cu = cells * 2 * args.ensemble_steps
+    return {"workload": f"ensemble2d_quadrant_{args.ensemble_count}", "problems": len(sizes),
+            "sizes": [int(sizes.min()), int(sizes.max())], "cells": cells, "iterations": args.ensemble_steps,
+            "warmup": args.warmup, "seconds": el, "cell_updates_per_s": cu / el,
+            "ms_per_iteration": el / args.ensemble_steps * 1e3,
+            "sweep_bytes_GBs": cells * 4 * BYTES_PER_CELL_SWEEP * args.ensemble_steps / el / 1e9,
+            "segments_by_ny_class": {"<=128": chosen[0], "<=256": chosen[1], "<=512": chosen[2]},
+            "autotune_s": tune_s, "sim_time_range": [float(times.min()), float(times.max())],
+            "finite": ok, "dust": "not carried"}
 
 
 def run_cpu_baseline(args, q0, h):
@@ -199,7 +275,7 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=6)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--cfl", type=float, default=0.4)
     ap.add_argument("--limiter", choices=sorted(LIMITERS), default="vanleer")
@@ -212,6 +288,9 @@ def main():
     ap.add_argument("--no-gasdust", action="store_true")
     ap.add_argument("--drag-resid-tol", type=float, default=None,
                     help="override the learned drag's residual tolerance (default: the model file's)")
+    ap.add_argument("--ensemble-count", type=int, default=512)
+    ap.add_argument("--ensemble-steps", type=int, default=500)
+    ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args()
     if args.copy_probe:
@@ -227,9 +306,11 @@ def main():
 
     q0, h, ndep = problems.sedov3d(args.n)
     log(f"[bench] Sedov {args.n}^3, {ndep} deposit cells, {args.steps} timed iterations")
-    elapsed, kern, simtime, final = run_gpu(args, q0, h)
+    elapsed_host, kern, simtime, final = run_gpu(args, q0, h)
     cells = args.n ** 3
     cu = cells * 2 * args.steps
+    log(f"[bench] Sedov {args.n}^3: device-side time loop, {args.steps} timed iterations")
+    elapsed, simtime_dev = run_device_loop(args, q0, h)
     value = cu / elapsed
 
     sweeps = {k: v for k, v in kern.items() if k.startswith("hllc_sweep") and v["avg_ms"]}
@@ -252,6 +333,11 @@ def main():
         log(f"[bench] gas-dust {args.gasdust_n}^3: {args.gasdust_steps} iterations, exact vs learned drag")
         gasdust = run_gasdust(args)
 
+    ensemble = None
+    if not args.no_ensemble:
+        log(f"[bench] ensemble of {args.ensemble_count} 2D problems: {args.ensemble_steps} iterations")
+        ensemble = run_ensemble(args)
+
     cpu = None
     if not args.no_cpu:
         log(f"[bench] CPU oracle baseline: {args.cpu_iters} iterations")
@@ -263,11 +349,15 @@ def main():
             "config": {"workload": f"sedov3d_hllc_{args.n}cubed", "cells": cells, "method": "HLLC",
                        "limiter": args.limiter, "cfl": args.cfl, "time_scheme": "SSP-RK2",
                        "boundaries": "extrapConstant", "iteration": "CFL + 2 fluidsteps (6 sweeps)",
-                       "sim_time_advanced": simtime},
+                       "loop": "device-side dTime, one hipGraph per iteration",
+                       "sim_time_advanced": simtime_dev},
+            "host_loop": {"cell_updates_per_s": cu / elapsed_host, "ms_per_step": elapsed_host / args.steps * 1e3,
+                          "sim_time_advanced": simtime,
+                          "note": "CFL read back by the host each iteration; per-kernel times below are from this run"},
             "roofline": roofline, "cpu_baseline": cpu,
             "kernels": kern,
             "drag_speedup_learned_vs_exact": gasdust["step_speedup"] if gasdust else None,
-            "gasdust": gasdust}
+            "gasdust": gasdust, "ensemble": ensemble}
     print(json.dumps(line), flush=True)
 
 

@@ -156,6 +156,12 @@ SIGNATURES = {
     "imogen_ensemble_times": (c_int, [c_void_p, _DP]),
     "imogen_ensemble_autotune": (c_int, [c_void_p, c_int, _IP]),
     "imogen_ensemble_destroy": (c_int, [c_void_p]),
+    "imogen_timeloop_create": (c_int, [_MGP, c_int, _P(GeometryParams), _P(FluidStepParams), _DP, c_double,
+                                       c_double, _P(c_void_p)]),
+    "imogen_timeloop_run": (c_int, [c_void_p, c_int, c_int, c_int]),
+    "imogen_timeloop_clock": (c_int, [c_void_p, _DP, _DP]),
+    "imogen_timeloop_set_time": (c_int, [c_void_p, c_double]),
+    "imogen_timeloop_destroy": (c_int, [c_void_p]),
     "imogen_device_alloc": (c_int, [ctypes.c_size_t, _P(c_void_p)]),
     "imogen_device_free": (c_int, [c_void_p]),
     "imogen_memcpy_htod": (c_int, [c_void_p, c_void_p, ctypes.c_size_t]),

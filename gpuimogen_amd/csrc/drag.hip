@@ -4,6 +4,9 @@
 //   |dv| and u_0 (cukern_prepareForERK_h0, gpuclass/cudaSource2FluidDrag.cu:1883-1908),
 //   a decay factor f of the relative velocity over dt, from
 //     LogTrap2 / LogTrap3 (cukern_LogTrapSolve, :1532-1630), or
+//     the explicit midpoint / RK4 integrators of |dv| (solveDragEMP,
+//     solveDragRK4, :305-467; these rebuild the state with
+//     cukern_applyFinalDeltaV, :1160-1237), or
 //     the adaptive reference integrator (this library's "exact" method), or
 //     the learned surrogate with per-cell fallback (mlp_drag.hip),
 //   then new momenta about the barycentric velocity and the gas energy change
@@ -38,6 +41,17 @@ __global__ __launch_bounds__(256) void drag_kernel(double *__restrict__ g, doubl
         // one atomic per wave
         for (int off = 32; off > 0; off >>= 1) steps += __shfl_xor(steps, off, 64);
         if ((threadIdx.x & 63) == 0 && steps) atomicAdd(work, steps);
+    }
+}
+
+// explicit midpoint (method 0) / RK4 (method 1): 72 bytes read, 64 written per cell
+template <int METHOD>
+__global__ __launch_bounds__(256) void drag_rk_kernel(double *__restrict__ g, double *__restrict__ d, int64_t pitch,
+                                                      int64_t n, DragConsts c, double dt)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const DragCell s = drag_rk_load(g + i, d + i, pitch);
+        drag_rk_apply(g + i, d + i, pitch, drag_rk_magdv<METHOD>(c, s.rg, s.rd, s.s0, s.u0, dt));
     }
 }
 
@@ -78,6 +92,12 @@ int launch_drag(double *g, double *d, int64_t pitch, int64_t n, const DragConsts
         hipLaunchKernelGGL((drag_kernel<DRAG_EXACT>), dim3((unsigned)blocks), dim3(256), 0, s, g, d, pitch, n, c,
                            dt, work);
         break;
+    case DRAG_EMP:
+        hipLaunchKernelGGL((drag_rk_kernel<DRAG_EMP>), dim3((unsigned)blocks), dim3(256), 0, s, g, d, pitch, n, c, dt);
+        break;
+    case DRAG_RK4:
+        hipLaunchKernelGGL((drag_rk_kernel<DRAG_RK4>), dim3((unsigned)blocks), dim3(256), 0, s, g, d, pitch, n, c, dt);
+        break;
     default: return IM_FAIL(ERROR_INVALID_ARGS, "drag method %d", method);
     }
     return IM_LAUNCHED("drag_kernel");
@@ -112,10 +132,10 @@ int sourcefunction_2FluidDrag(MGArray *fluidA, MGArray *fluidB, GeometryParams *
     switch (method) {
     case DRAG_LOGTRAP2:
     case DRAG_LOGTRAP3:
-    case DRAG_EXACT: rc = launch_drag(g, d, pg, n, c, dt, method, nullptr, st->stream); break;
-    case DRAG_LEARNED: rc = launch_learned_drag(g, d, pg, n, c, dt, st->stream); break;
+    case DRAG_EXACT:
     case DRAG_EMP:
-    case DRAG_RK4:
+    case DRAG_RK4: rc = launch_drag(g, d, pg, n, c, dt, method, nullptr, st->stream); break;
+    case DRAG_LEARNED: rc = launch_learned_drag(g, d, pg, n, c, dt, st->stream); break;
     case DRAG_ETDRK1:
     case DRAG_ETDRK2: rc = IM_FAIL(ERROR_NOIMPLEMENT, "drag method %d is not built", method); break;
     default: rc = IM_FAIL(ERROR_INVALID_ARGS, "unknown drag method %d", method);

@@ -31,8 +31,8 @@ namespace imogen {
 // host launchers
 // ---------------------------------------------------------------------------
 template <class M, int LIM>
-static int launch_x(double *q, int64_t pitch, int N, int64_t nlines, double lambda, const HydroConsts &hc,
-                    int rt, hipStream_t s)
+static int launch_x(double *q, int64_t pitch, int N, int64_t nlines, double lambda, const double *lamp,
+                    const HydroConsts &hc, int rt, hipStream_t s)
 {
     int K;
     if (N <= 64) K = 1;
@@ -59,10 +59,10 @@ static int launch_x(double *q, int64_t pitch, int N, int64_t nlines, double lamb
     case KK:                                                                                            \
         if (full)                                                                                       \
             hipLaunchKernelGGL((sweep_x<M, KK, LIM, true>), grid, block, 0, s, q, pitch, N, nlines, nl,  \
-                               lambda, hc, rt, dummy);                                                  \
+                               lambda, lamp, hc, rt, dummy);                                            \
         else                                                                                            \
             hipLaunchKernelGGL((sweep_x<M, KK, LIM, false>), grid, block, 0, s, q, pitch, N, nlines, nl, \
-                               lambda, hc, rt, dummy);                                                  \
+                               lambda, lamp, hc, rt, dummy);                                            \
         break;
     switch (K) {
         IM_XK(1)
@@ -80,7 +80,7 @@ static int launch_x(double *q, int64_t pitch, int N, int64_t nlines, double lamb
 
 template <class M, int LIM>
 static int launch_march(double *q, int64_t pitch, int nx, int ny, int nz, int dir, double lambda,
-                        const HydroConsts &hc, int rt, hipStream_t s)
+                        const double *lamp, const HydroConsts &hc, int rt, hipStream_t s)
 {
     const int N = (dir == 2) ? ny : nz;
     const int64_t sstride = (dir == 2) ? (int64_t)nx : (int64_t)nx * ny;
@@ -101,43 +101,43 @@ static int launch_march(double *q, int64_t pitch, int nx, int ny, int nz, int di
     if (!defer) return IM_FAIL(ERROR_NOMEM, "no scratch for deferred outputs");
     if (dir == 2)
         hipLaunchKernelGGL((sweep_march<M, 2, LIM>), grid, block, 0, s, q, pitch, nx, sstride, ostride, N, S,
-                           seglen, wavesPerRow, lambda, hc, rt, defer);
+                           seglen, wavesPerRow, lambda, lamp, hc, rt, defer);
     else
         hipLaunchKernelGGL((sweep_march<M, 3, LIM>), grid, block, 0, s, q, pitch, nx, sstride, ostride, N, S,
-                           seglen, wavesPerRow, lambda, hc, rt, defer);
+                           seglen, wavesPerRow, lambda, lamp, hc, rt, defer);
     return IM_LAUNCHED("sweep_march");
 }
 
 template <class M, int LIM>
 static int launch_dir(double *q, int64_t pitch, int nx, int ny, int nz, int dir, double lambda,
-                      const HydroConsts &hc, int rt, hipStream_t s)
+                      const double *lamp, const HydroConsts &hc, int rt, hipStream_t s)
 {
     const int slot = prof_begin(PROF_SWEEP_X + dir - 1, s);
     int rc;
-    if (dir == 1) rc = launch_x<M, LIM>(q, pitch, nx, (int64_t)ny * nz, lambda, hc, rt, s);
-    else rc = launch_march<M, LIM>(q, pitch, nx, ny, nz, dir, lambda, hc, rt, s);
+    if (dir == 1) rc = launch_x<M, LIM>(q, pitch, nx, (int64_t)ny * nz, lambda, lamp, hc, rt, s);
+    else rc = launch_march<M, LIM>(q, pitch, nx, ny, nz, dir, lambda, lamp, hc, rt, s);
     prof_end(slot, s);
     return rc;
 }
 
 template <class M>
 static int launch_method(double *base, int64_t pitch, int nx, int ny, int nz, int dir, double lambda,
-                         const HydroConsts &hc, int limiter, hipStream_t s)
+                         const double *lamp, const HydroConsts &hc, int limiter, hipStream_t s)
 {
     switch (limiter) {
-    case LIMITER_OSPRE: return launch_dir<M, 0>(base, pitch, nx, ny, nz, dir, lambda, hc, 0, s);
-    case LIMITER_VANLEER: return launch_dir<M, 1>(base, pitch, nx, ny, nz, dir, lambda, hc, 1, s);
+    case LIMITER_OSPRE: return launch_dir<M, 0>(base, pitch, nx, ny, nz, dir, lambda, lamp, hc, 0, s);
+    case LIMITER_VANLEER: return launch_dir<M, 1>(base, pitch, nx, ny, nz, dir, lambda, lamp, hc, 1, s);
     case LIMITER_MINMOD:
     case LIMITER_SUPERBEE:
     case LIMITER_VANALBADA:
     case LIMITER_MC:
-    case LIMITER_ZERO: return launch_dir<M, -1>(base, pitch, nx, ny, nz, dir, lambda, hc, limiter, s);
+    case LIMITER_ZERO: return launch_dir<M, -1>(base, pitch, nx, ny, nz, dir, lambda, lamp, hc, limiter, s);
     default: return IM_FAIL(ERROR_INVALID_ARGS, "unknown slope limiter %d", limiter);
     }
 }
 
 int launch_fluid_sweep(double *base, int64_t pitch, int nx, int ny, int nz, int dir, int method, double lambda,
-                       double gamma, double rhomin, double ciso, int limiter, hipStream_t s)
+                       double gamma, double rhomin, double ciso, int limiter, hipStream_t s, const double *lamp)
 {
     if (dir < 1 || dir > 3) return IM_FAIL(ERROR_INVALID_ARGS, "sweep direction %d", dir);
     const int N = (dir == 1) ? nx : (dir == 2 ? ny : nz);
@@ -150,8 +150,8 @@ int launch_fluid_sweep(double *base, int64_t pitch, int nx, int ny, int nz, int 
     hc.emin = 1e-5 / (gamma - 1.0);  // cudaFluidStep.cu:298
     hc.ciso = ciso;
     hc.cisosq = ciso * ciso;
-    if (method == METHOD_HLLC) return launch_method<Hllc>(base, pitch, nx, ny, nz, dir, lambda, hc, limiter, s);
-    if (method == METHOD_HLL) return launch_method<Hll>(base, pitch, nx, ny, nz, dir, lambda, hc, limiter, s);
+    if (method == METHOD_HLLC) return launch_method<Hllc>(base, pitch, nx, ny, nz, dir, lambda, lamp, hc, limiter, s);
+    if (method == METHOD_HLL) return launch_method<Hll>(base, pitch, nx, ny, nz, dir, lambda, lamp, hc, limiter, s);
     return IM_FAIL(ERROR_INVALID_ARGS, "sweep kernel for method %d", method);
 }
 

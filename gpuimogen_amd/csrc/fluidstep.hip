@@ -61,6 +61,65 @@ int sas_code(const MGArray *a, int direction, int side, int *code)
 
 }  // namespace
 
+namespace imogen {
+
+int update_1d(MGArray *fluid, const FluidStepParams &params, const double *lampBase)
+{
+    double *base;
+    int64_t pitch;
+    int rc = fluid_view(fluid, &base, &pitch);
+    if (rc) return rc;
+    if (params.geometry.shape != SQUARE)
+        return IM_FAIL(ERROR_NOIMPLEMENT, "only SQUARE (cartesian) geometry is built");
+    if (params.onlyHydro != 1) return IM_FAIL(ERROR_NOIMPLEMENT, "MHD is not built (dead in the reference)");
+    const int dir = params.stepDirection;
+    if (dir < 1 || dir > 3) return IM_FAIL(ERROR_INVALID_ARGS, "stepDirection %d", dir);
+    const double lambda = params.dt / params.geometry.h[dir - 1];
+    const double *lamp = lampBase ? lampBase + (dir - 1) : nullptr;
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    switch (params.stepMethod) {
+    case METHOD_HLLC:
+    case METHOD_HLL:
+        return launch_fluid_sweep(base, pitch, fluid->dim[0], fluid->dim[1], fluid->dim[2], dir,
+                                  (int)params.stepMethod, lambda, params.thermoGamma, params.minimumRho,
+                                  params.Cisothermal, params.slopeLimiter, st->stream, lamp);
+    case METHOD_XINJIN:
+        return launch_xinjin_sweep(base, pitch, fluid->dim[0], fluid->dim[1], fluid->dim[2], dir, lambda,
+                                   params.thermoGamma, params.minimumRho, st->stream, lamp);
+    default: return IM_FAIL(ERROR_INVALID_ARGS, "unknown fluid method %d", (int)params.stepMethod);
+    }
+}
+
+// flux.cu:27-155 on one rank: the sweep order of (stepNumber, stepDirection),
+// each sweep followed by that direction's boundary conditions.
+int update_3d(MGArray *fluid, const FluidStepParams &fsp, const double *lampBase)
+{
+    int rc;
+    FluidStepParams p = fsp;
+    if (fluid->dim[1] == 1 && fluid->dim[2] == 1) {  // flux.cu:57-66
+        p.stepDirection = 1;
+        rc = update_1d(fluid, p, lampBase);
+        if (rc) return rc;
+        return setFluidBoundary(fluid, &p.geometry, 1);
+    }
+    int sweep = (fsp.stepNumber + 3 * (fsp.stepDirection > 0)) % 6;
+    if (sweep < 0) sweep += 6;
+    for (int n = 0; n < 3; n++) {
+        const int d = kFluxcall[n][sweep];
+        if (fluid->dim[d - 1] > 3) {  // flux.cu:102
+            p.stepDirection = d;
+            rc = update_1d(fluid, p, lampBase);
+            if (rc) return rc;
+            rc = setFluidBoundary(fluid, &p.geometry, d);
+            if (rc) return rc;
+        }
+    }
+    return SUCCESSFUL;
+}
+
+}  // namespace imogen
+
 extern "C" {
 
 int setFluidBoundary(MGArray *fluid, GeometryParams *geo, int direction)
@@ -115,57 +174,18 @@ int setArrayBoundaryConditions(MGArray *phi, GeometryParams *geo, int direction,
 int performFluidUpdate_1D(MGArray *fluid, FluidStepParams params, ParallelTopology *topo, MGArray *tmpst)
 {
     (void)tmpst;
-    double *base;
-    int64_t pitch;
-    int rc = fluid_view(fluid, &base, &pitch);
+    int rc = check_topology(topo);
     if (rc) return rc;
-    rc = check_topology(topo);
-    if (rc) return rc;
-    if (params.geometry.shape != SQUARE)
-        return IM_FAIL(ERROR_NOIMPLEMENT, "only SQUARE (cartesian) geometry is built");
-    if (params.onlyHydro != 1) return IM_FAIL(ERROR_NOIMPLEMENT, "MHD is not built (dead in the reference)");
-    const int dir = params.stepDirection;
-    if (dir < 1 || dir > 3) return IM_FAIL(ERROR_INVALID_ARGS, "stepDirection %d", dir);
-    const double lambda = params.dt / params.geometry.h[dir - 1];
-    DeviceState *st = state();
-    if (!st) return ERROR_CUDA_BLEW_UP;
-    switch (params.stepMethod) {
-    case METHOD_HLLC:
-    case METHOD_HLL:
-        return launch_fluid_sweep(base, pitch, fluid->dim[0], fluid->dim[1], fluid->dim[2], dir,
-                                  (int)params.stepMethod, lambda, params.thermoGamma, params.minimumRho,
-                                  params.Cisothermal, params.slopeLimiter, st->stream);
-    case METHOD_XINJIN:
-        return launch_xinjin_sweep(base, pitch, fluid->dim[0], fluid->dim[1], fluid->dim[2], dir, lambda,
-                                   params.thermoGamma, params.minimumRho, st->stream);
-    default: return IM_FAIL(ERROR_INVALID_ARGS, "unknown fluid method %d", (int)params.stepMethod);
-    }
+    return imogen::update_1d(fluid, params, nullptr);
 }
 
 int performFluidUpdate_3D(MGArray *fluid, ParallelTopology *parallelTopo, FluidStepParams fsp,
                           MGArray *tempStorage)
 {
-    int rc;
-    FluidStepParams p = fsp;
-    if (fluid->dim[1] == 1 && fluid->dim[2] == 1) {  // flux.cu:57-66
-        p.stepDirection = 1;
-        rc = performFluidUpdate_1D(fluid, p, parallelTopo, tempStorage);
-        if (rc) return rc;
-        return setFluidBoundary(fluid, &fsp.geometry, 1);
-    }
-    int sweep = (fsp.stepNumber + 3 * (fsp.stepDirection > 0)) % 6;
-    if (sweep < 0) sweep += 6;
-    for (int n = 0; n < 3; n++) {
-        const int d = kFluxcall[n][sweep];
-        if (fluid->dim[d - 1] > 3) {  // flux.cu:102
-            p.stepDirection = d;
-            rc = performFluidUpdate_1D(fluid, p, parallelTopo, tempStorage);
-            if (rc) return rc;
-            rc = setFluidBoundary(fluid, &fsp.geometry, d);
-            if (rc) return rc;
-        }
-    }
-    return SUCCESSFUL;
+    (void)tempStorage;
+    int rc = check_topology(parallelTopo);
+    if (rc) return rc;
+    return imogen::update_3d(fluid, fsp, nullptr);
 }
 
 int calculateSoundspeed(MGArray *fluid, MGArray *mag, MGArray *cs, double gamma)

@@ -40,6 +40,8 @@ enum ProfKernel { PROF_SWEEP_X = 0, PROF_SWEEP_Y = 1, PROF_SWEEP_Z = 2, PROF_BC 
                   PROF_DRAG = 5, PROF_MLP = 6, PROF_NKERNELS = 8 };
 int prof_begin(int kernel, hipStream_t s);
 void prof_end(int slot, hipStream_t s);
+// Nested suspension of the event timing (kept off while a graph is captured).
+void prof_suspend(int on);
 // Device scratch of at least `bytes`; contents are undefined on return.
 void *scratch(size_t bytes);
 
@@ -47,13 +49,20 @@ void *scratch(size_t bytes);
 // order, one GPU) and returns its base pointer and slab pitch in doubles.
 int fluid_view(MGArray *fluid, double **base, int64_t *pitch);
 
+// performFluidUpdate_1D / _3D without the topology check. lampBase: when not
+// null, the sweeps of direction d read lambda from lampBase[d - 1] (device).
+int update_1d(MGArray *fluid, const FluidStepParams &params, const double *lampBase);
+int update_3d(MGArray *fluid, const FluidStepParams &fsp, const double *lampBase);
+
 // Launchers (fluid_sweep.hip, xinjin.hip, statics.hip, cfl.hip, drag.hip)
 // method: METHOD_HLLC or METHOD_HLL. rhomin, ciso: HLL floors and isothermal
-// sound speed (-1: adiabatic).
+// sound speed (-1: adiabatic). lamp: when not null, the kernels read lambda
+// = dt/h from this device address instead of the `lambda` argument.
 int launch_fluid_sweep(double *base, int64_t pitch, int nx, int ny, int nz, int dir, int method, double lambda,
-                       double gamma, double rhomin, double ciso, int limiter, hipStream_t s);
+                       double gamma, double rhomin, double ciso, int limiter, hipStream_t s,
+                       const double *lamp = nullptr);
 int launch_xinjin_sweep(double *base, int64_t pitch, int nx, int ny, int nz, int dir, double lambda,
-                        double gamma, double rhomin, hipStream_t s);
+                        double gamma, double rhomin, hipStream_t s, const double *lamp = nullptr);
 int launch_fluid_bc(double *base, int64_t pitch, int nx, int ny, int nz, int dir, const int *modes10,
                     int depth, hipStream_t s);
 int launch_soundspeed(const double *base, int64_t pitch, int64_t n, double gamma, double *cs,

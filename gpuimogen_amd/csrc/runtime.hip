@@ -79,10 +79,16 @@ static const int kProfSlots = 16384;
 static ProfSlot g_prof[kProfSlots];
 static int g_profUsed = 0;
 static bool g_profOn = false;
+static int g_profSuspended = 0;
+
+void prof_suspend(int on)
+{
+    g_profSuspended += on ? 1 : -1;
+}
 
 int prof_begin(int kernel, hipStream_t s)
 {
-    if (!g_profOn || g_profUsed >= kProfSlots) return -1;
+    if (!g_profOn || g_profSuspended > 0 || g_profUsed >= kProfSlots) return -1;
     ProfSlot &p = g_prof[g_profUsed];
     if (!p.a) {
         if (hipEventCreate(&p.a) != hipSuccess || hipEventCreate(&p.b) != hipSuccess) return -1;

@@ -118,6 +118,19 @@ __device__ __forceinline__ double sqrt_fast(double x)
 #endif
 }
 
+// sqrt_fast for arguments known to be > 0 (ratios of densities)
+__device__ __forceinline__ double sqrt_fast_pos(double x)
+{
+#if IMOGEN_SWEEP_FAST_MATH
+    const double y = __builtin_amdgcn_rsq(x);
+    const double g = x * y;
+    const double r = fma(-g, 0.5 * y, 0.5);
+    return fma(g, r, g);
+#else
+    return sqrt_nr(x);
+#endif
+}
+
 // ---- half-slope limiters (limiter_functions.h) ----------------------------
 __device__ __forceinline__ double flux_minmod(double a, double b)
 {
@@ -221,13 +234,15 @@ __device__ __forceinline__ V5 hllc_flux(const HydroConsts &k, const V5 &L, const
     const double iR = rcp_fast(R.a);
     double csql = k.gamma * L.b * iL;
     double csqr = k.gamma * R.b * iR;
-    double Fa = sqrt_fast(R.a * iL);
+    double Fa = sqrt_fast_pos(R.a * iL);
     double Fb = rcp_fast(1.0 + Fa);
     double Croe = ((csql + Fa * csqr) + .5 * k.gm1 * dvsq * Fa * Fb) * Fb;
     double Vroe = (L.c + Fa * R.c) * Fb;
     Croe = Vroe - sqrt_fast(Croe);
     double sl = L.c - sqrt_fast(csql);
     double sr = R.c + sqrt_fast(csqr);
+    // the reference's selects, not fmin/fmax: a NaN bound (negative face
+    // pressure) must propagate as it does there
     sl = (Croe < sl) ? Croe : sl;
     Croe = 2 * Vroe - Croe;
     sr = (Croe > sr) ? Croe : sr;
@@ -245,28 +260,25 @@ __device__ __forceinline__ V5 hllc_flux(const HydroConsts &k, const V5 &L, const
     const double iS = left ? iL : iR;
 
     double ksum = S.c * S.c + S.d * S.d + S.e * S.e;  // |v|^2 of the upwind face
+    // Plain flux of the upwind side (cases 0/1, cudaFluidStep.cu:1372-1383,
+    // 1416-1430) and its star flux (cases 2/3, :1384-1415) share their form:
+    // with mass flux m (rho v of the face, or rho beta S_M), the energy flux
+    // is m (gamma/(gamma-1) P/rho + |v|^2/2 [+ S_K (S_M - v)]) and the
+    // tangential fluxes are v_t m, so one select per differing term replaces
+    // forming both fluxes and selecting five results.
+    const double rb = div_fast(S.a * (sk - S.c), sk - sm);  // rho * beta (star)
+    const double G = .5 * ksum;
+    const double mp = S.a * S.c;
+    const double Cc = sk * (sm - S.c);
+    const double m = star ? rb * sm : mp;
+    const double cz = star ? Cc : 0.0;
+    const double tn = star ? rb * (Cc + sm * S.c) : mp * S.c;
     V5 out;
-    if (!star) {
-        // cases 0/1 (cudaFluidStep.cu:1372-1383, 1416-1430)
-        double Ek = .5 * ksum * S.a;
-        double m = S.a * S.c;
-        out.a = m;
-        out.b = S.c * (Ek + k.govergm1 * S.b);
-        out.c = m * S.c + S.b;
-        out.d = S.d * m;
-        out.e = S.e * m;
-    } else {
-        // cases 2/3 (cudaFluidStep.cu:1384-1415)
-        double rb = div_fast(S.a * (sk - S.c), sk - sm);  // rho * beta
-        double G = .5 * ksum;
-        double m = rb * sm;
-        double Cc = sk * (sm - S.c);
-        out.a = m;
-        out.b = rb * sm * (k.govergm1 * S.b * iS + G + Cc);
-        out.c = S.b + rb * (Cc + sm * S.c);
-        out.d = S.d * m;
-        out.e = S.e * m;
-    }
+    out.a = m;
+    out.b = m * (k.govergm1 * S.b * iS + G + cz);
+    out.c = S.b + tn;
+    out.d = S.d * m;
+    out.e = S.e * m;
     return out;
 }
 

@@ -119,10 +119,11 @@ __device__ __forceinline__ void x_fluxes(const HydroConsts &hc, int rt, V5 (&U)[
 // wait only for the loads it needs); lanes past the last line read a clamped
 // valid line and write into a private dummy slot.
 template <class M, int K, int LIM, bool FULL>
-__global__ __launch_bounds__(256) void sweep_x(double *__restrict__ q, int64_t pitch, int N, int64_t nlines,
-                                               int nl, double lambda, HydroConsts hc, int rt,
-                                               double *__restrict__ dummy)
+__global__ __launch_bounds__(256, (K <= 4) ? 2 : 1) void sweep_x(double *__restrict__ q, int64_t pitch, int N, int64_t nlines,
+                                               int nl, double lambda, const double *__restrict__ lamp,
+                                               HydroConsts hc, int rt, double *__restrict__ dummy)
 {
+    if (lamp) lambda = lamp[0];  // time step kept on the device
     const int lane = threadIdx.x & 63;
     const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
@@ -210,6 +211,9 @@ __device__ __forceinline__ void march_body(double *__restrict__ col, int64_t pit
     int mw = j0 - 4;
     if (mw < 0) mw += N;
     if (mw < 0) mw += N;
+    double *outp = col + (int64_t)(j0 - 9) * sstride;  // address of output t - 9
+    double *const dfr = defer + gtid;
+    const int64_t dstride = 5 * nthr;
 
     const V5 one = V5{1.0, 1.0, 0.0, 0.0, 0.0};
     const V5 zero = V5{0.0, 0.0, 0.0, 0.0, 0.0};
@@ -267,11 +271,16 @@ __device__ __forceinline__ void march_body(double *__restrict__ col, int64_t pit
         // output yet / past the segment). With a fixed store count per step the
         // compiler can wait for a prefetched load without waiting for stores
         // (stores share the vector-memory counter on gfx9).
+        // outp walks the column one cell per step (it points at output oi);
+        // the deferred slots are wave-uniform offsets of this lane's block.
         const int oi = t - 9;  // output index within the segment
-        const bool direct = (oi >= 4) && (oi < L);
-        const int slot = (oi >= 0 && oi < 4 && oi < L) ? oi : 4;
-        double *dst = direct ? col + (int64_t)(j0 + oi) * sstride : defer + gtid + (int64_t)slot * 5 * nthr;
-        store_cell<M, DIR>(dst, direct ? pitch : nthr, o);
+        const bool direct = (oi >= 4) & (oi < L);
+        const int slot = ((oi >= 0) & (oi < 4) & (oi < L)) ? oi : 4;
+        double *dq = dfr + (int64_t)slot * dstride;
+        double *dst = direct ? outp : dq;
+        const int64_t st = direct ? pitch : nthr;
+        outp += sstride;
+        store_cell<M, DIR>(dst, st, o);
     };
 
     const int maxsteps = seglen + 9;
@@ -288,17 +297,19 @@ __device__ __forceinline__ void march_body(double *__restrict__ col, int64_t pit
     // lane's dummy slot, which also pins them after the load.
     store_cell<M, DIR>(defer + gtid + (int64_t)4 * 5 * nthr, nthr, N0);
 
+    // Steps go in pairs. When maxsteps is odd the last pair's second step is
+    // one past the segment: it loads a valid (wrapped) cell and stores to the
+    // dummy slot, so running it unconditionally changes no output and keeps
+    // the loop to one path (no register copies at the back edge).
     for (int t = 0; t < maxsteps; t += 2) {
         mw = (mw + 1 == N) ? 0 : mw + 1;
         N1 = load_cell<M, DIR>(col + (int64_t)mw * sstride, pitch);
         step(t, N0, QA, U1A, U1B, eA, eB, wA, wB, dA, dB, RA, RB, FAA, FAB, w1A, w1B, d1A, d1B, R1A, R1B, FBA,
              FBB);
-        if (t + 1 < maxsteps) {
-            mw = (mw + 1 == N) ? 0 : mw + 1;
-            N0 = load_cell<M, DIR>(col + (int64_t)mw * sstride, pitch);
-            step(t + 1, N1, QB, U1B, U1A, eB, eA, wB, wA, dB, dA, RB, RA, FAB, FAA, w1B, w1A, d1B, d1A, R1B, R1A,
-                 FBB, FBA);
-        }
+        mw = (mw + 1 == N) ? 0 : mw + 1;
+        N0 = load_cell<M, DIR>(col + (int64_t)mw * sstride, pitch);
+        step(t + 1, N1, QB, U1B, U1A, eB, eA, wB, wA, dB, dA, RB, RA, FAB, FAA, w1B, w1A, d1B, d1A, R1B, R1A, FBB,
+             FBA);
     }
 
     // every lane of the wave has finished reading: release the deferred outputs
@@ -312,9 +323,10 @@ __device__ __forceinline__ void march_body(double *__restrict__ col, int64_t pit
 template <class M, int DIR, int LIM>
 __global__ __launch_bounds__(256) void sweep_march(double *__restrict__ q, int64_t pitch, int nx, int64_t sstride,
                                                    int64_t ostride, int N, int S, int seglen, int wavesPerRow,
-                                                   double lambda, HydroConsts hc, int rt,
-                                                   double *__restrict__ defer)
+                                                   double lambda, const double *__restrict__ lamp,
+                                                   HydroConsts hc, int rt, double *__restrict__ defer)
 {
+    if (lamp) lambda = lamp[0];  // time step kept on the device
     const int lane = threadIdx.x & 63;
     const int wib = threadIdx.x >> 6;
     const int W = 64 / S;

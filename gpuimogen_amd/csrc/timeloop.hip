@@ -1,0 +1,271 @@
+// timeloop.hip -- Imogen iterations with the time step kept on the device.
+//
+// One iteration is what imogen.m:115-139 runs for hydrodynamics:
+//     TimeManager.update (time/TimeManager.m:124-186): tau = min over fluids of
+//         the CFL limit (cflTimestep.cu:107-377); dTime = CFL * tau, clamped so
+//         that t + 2 dTime does not pass tMax; t += 2 dTime
+//     fluidstep(+1) for every fluid, then fluidstep(-1) for every fluid
+//         (flux.cu:27-155: three directional sweeps, each followed by its
+//         boundary conditions)
+// The host-driven form of this loop reads tau back after every CFL reduction
+// (cflTimestep.cu:214 synchronises), so the GPU idles while the host issues
+// the next sweeps. Here the CFL partial minima, dTime and the per-direction
+// lambda = dTime / h stay in device memory: the sweep kernels read lambda
+// from there, and each iteration is one replayed hipGraph (one per sweep
+// order, iteration mod 6). The kernels, and so the results, are those of
+// performFluidUpdate_3D and imogen_cflTimestepFused.
+#include <cmath>
+#include <cstring>
+
+#include "imogen_internal.h"
+
+namespace imogen {
+
+// device clock: [0] time, [1] dTime, [2..4] lambda x, y, z
+constexpr int kClockDoubles = 8;
+constexpr int kCflSlot = 2056;  // tau + 7 spare + up to 2048 block minima (launch_cfl)
+constexpr int kMaxFluids = 4;
+
+__global__ void clock_advance(const double *__restrict__ cflOut, int nfluids, double cfl, double tmax,
+                              double hx, double hy, double hz, double *__restrict__ clk)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double tau = cflOut[0];
+    for (int f = 1; f < nfluids; f++) {
+        const double t = cflOut[(int64_t)f * kCflSlot];
+        tau = (t < tau) ? t : tau;
+    }
+    double dt = cfl * tau;
+    const double t = clk[0];
+    if (t + 2 * dt > tmax) dt = 0.5 * (tmax - t);
+    clk[1] = dt;
+    clk[2] = dt / hx;
+    clk[3] = dt / hy;
+    clk[4] = dt / hz;
+    clk[0] = t + 2 * dt;
+}
+
+struct TimeloopImpl {
+    int nf = 0;
+    MGArray fluid[kMaxFluids][5];
+    FluidStepParams p[kMaxFluids];
+    double cflGamma[kMaxFluids];  // gamma of the CFL sound speed (the fluid's own)
+    GeometryParams geo;
+    double cfl = 0.4, tmax = HUGE_VAL;
+    double *dbuf = nullptr;  // clock, then one CFL slot per fluid
+    hipGraphExec_t graph[6] = {nullptr};
+    void *scratchAt = nullptr;  // scratch block the graphs were captured with
+    size_t scratchBytes = 0;
+};
+
+}  // namespace imogen
+
+struct ImogenTimeloop {
+    imogen::TimeloopImpl impl;
+};
+
+namespace imogen {
+
+static void drop_graphs(TimeloopImpl *L)
+{
+    for (int k = 0; k < 6; k++)
+        if (L->graph[k]) {
+            (void)hipGraphExecDestroy(L->graph[k]);
+            L->graph[k] = nullptr;
+        }
+}
+
+// CFL limit of fluid f into its slot (cfl_common in fluidstep.hip, on the device)
+static int cfl_of(TimeloopImpl *L, int f, hipStream_t s)
+{
+    double *base;
+    int64_t pitch;
+    int rc = fluid_view(L->fluid[f], &base, &pitch);
+    if (rc) return rc;
+    const GeometryParams &g = L->geo;
+    int spacedim = 0;
+    if (g.globalRez[1] > 1) spacedim = 1;
+    if (g.globalRez[2] > 1) spacedim = 2;
+    const int dim = spacedim + 1;
+    double hmin = g.h[0];
+    if (dim > 1 && g.h[1] < hmin) hmin = g.h[1];
+    if (dim > 2 && g.h[2] < hmin) hmin = g.h[2];
+    const int method = (int)L->p[f].stepMethod;
+    const int xj = (method == METHOD_XINJIN) && dim > 1;
+    const int halve = (method == METHOD_HLL || method == METHOD_HLLC);
+    const int slot = prof_begin(PROF_CFL, s);
+    rc = launch_cfl(base, pitch, L->fluid[f][0].partNumel[0], nullptr, L->cflGamma[f], 1, dim, xj, halve,
+                    g.h[0], g.h[1], g.h[2], hmin, L->dbuf + kClockDoubles + (int64_t)f * kCflSlot, s);
+    prof_end(slot, s);
+    return rc;
+}
+
+static int iteration(TimeloopImpl *L, int it, hipStream_t s)
+{
+    int rc;
+    for (int f = 0; f < L->nf; f++)
+        if ((rc = cfl_of(L, f, s))) return rc;
+    hipLaunchKernelGGL(clock_advance, dim3(1), dim3(64), 0, s, L->dbuf + kClockDoubles, L->nf, L->cfl, L->tmax,
+                       L->geo.h[0], L->geo.h[1], L->geo.h[2], L->dbuf);
+    if ((rc = IM_LAUNCHED("clock_advance"))) return rc;
+    for (int sgn = 1; sgn >= -1; sgn -= 2)
+        for (int f = 0; f < L->nf; f++) {
+            FluidStepParams p = L->p[f];
+            p.stepNumber = it;
+            p.stepDirection = sgn;
+            p.dt = 0.0;  // unused: lambda comes from the device clock
+            if ((rc = update_3d(L->fluid[f], p, L->dbuf + 2))) return rc;
+        }
+    return SUCCESSFUL;
+}
+
+}  // namespace imogen
+
+using namespace imogen;
+
+extern "C" {
+
+int imogen_timeloop_create(MGArray *fluids, int nfluids, const GeometryParams *geo, const FluidStepParams *params,
+                           const double *cflGamma, double cfl, double tmax, ImogenTimeloop **out)
+{
+    if (!fluids || !geo || !params || !out) return IM_FAIL(ERROR_NULL_POINTER, "null argument");
+    *out = nullptr;
+    if (nfluids < 1 || nfluids > kMaxFluids)
+        return IM_FAIL(ERROR_INVALID_ARGS, "1..%d fluids (got %d)", kMaxFluids, nfluids);
+    if (!(cfl > 0)) return IM_FAIL(ERROR_INVALID_ARGS, "CFL factor %g", cfl);
+    for (int f = 0; f < nfluids; f++) {
+        double *base;
+        int64_t pitch;
+        int rc = fluid_view(fluids + 5 * f, &base, &pitch);
+        if (rc) return rc;
+        if (params[f].geometry.shape != SQUARE) return IM_FAIL(ERROR_NOIMPLEMENT, "only SQUARE geometry is built");
+        if (params[f].onlyHydro != 1) return IM_FAIL(ERROR_NOIMPLEMENT, "MHD is not built");
+        const int m = (int)params[f].stepMethod;
+        if (m != METHOD_HLL && m != METHOD_HLLC && m != METHOD_XINJIN)
+            return IM_FAIL(ERROR_INVALID_ARGS, "unknown fluid method %d", m);
+        for (int d = 0; d < 3; d++)
+            if (!(geo->h[d] > 0)) return IM_FAIL(ERROR_INVALID_ARGS, "cell size h[%d] = %g", d, geo->h[d]);
+    }
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    ImogenTimeloop *T = new ImogenTimeloop();
+    TimeloopImpl *L = &T->impl;
+    L->nf = nfluids;
+    for (int f = 0; f < nfluids; f++) {
+        memcpy(L->fluid[f], fluids + 5 * f, sizeof(MGArray) * 5);
+        L->p[f] = params[f];
+        L->cflGamma[f] = cflGamma ? cflGamma[f] : params[f].thermoGamma;
+    }
+    L->geo = *geo;
+    L->cfl = cfl;
+    L->tmax = (tmax > 0) ? tmax : HUGE_VAL;
+    const size_t bytes = sizeof(double) * (kClockDoubles + (size_t)nfluids * kCflSlot);
+    if (IM_HIP(hipMalloc(&L->dbuf, bytes))) {
+        delete T;
+        return ERROR_NOMEM;
+    }
+    // start at t = 0
+    static const double zeros[kClockDoubles] = {0};
+    int rc = IM_HIP(hipMemcpyAsync(L->dbuf, zeros, sizeof(zeros), hipMemcpyHostToDevice, st->stream));
+    if (!rc) rc = IM_HIP(hipStreamSynchronize(st->stream));
+    if (rc) {
+        (void)hipFree(L->dbuf);
+        delete T;
+        return rc;
+    }
+    *out = T;
+    return SUCCESSFUL;
+}
+
+// Runs `iterations` iterations numbered firstIteration, firstIteration + 1, ...
+// (the number selects the sweep order, flux.cu:80-86). useGraph != 0 replays
+// one captured graph per sweep order; the first iteration of each order is
+// captured after an eager warm-up has sized the library's scratch block, and
+// the graphs are re-captured if that block has moved since.
+int imogen_timeloop_run(ImogenTimeloop *T, int firstIteration, int iterations, int useGraph)
+{
+    if (!T) return IM_FAIL(ERROR_NULL_POINTER, "null time loop");
+    TimeloopImpl *L = &T->impl;
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    for (int k = 0; k < iterations; k++) {
+        const int it = firstIteration + k;
+        if (!useGraph) {
+            int rc = iteration(L, it, st->stream);
+            if (rc) return rc;
+            continue;
+        }
+        if (L->scratchAt != st->scratch || L->scratchBytes != st->scratchBytes) {
+            drop_graphs(L);
+            L->scratchAt = st->scratch;
+            L->scratchBytes = st->scratchBytes;
+        }
+        hipGraphExec_t &g = L->graph[((it % 6) + 6) % 6];
+        if (!g) {
+            // eager run of this iteration: sizes the scratch block (no allocation
+            // may happen while capturing) and advances the state as the replay would
+            int rc = iteration(L, it, st->stream);
+            if (rc) return rc;
+            if (L->scratchAt != st->scratch || L->scratchBytes != st->scratchBytes) {
+                drop_graphs(L);
+                L->scratchAt = st->scratch;
+                L->scratchBytes = st->scratchBytes;
+            }
+            // record (not run) the same iteration as this order's graph
+            hipGraph_t graph;
+            prof_suspend(1);
+            if (IM_HIP(hipStreamBeginCapture(st->stream, hipStreamCaptureModeThreadLocal))) {
+                prof_suspend(0);
+                return ERROR_CUDA_BLEW_UP;
+            }
+            rc = iteration(L, it, st->stream);
+            hipError_t ec = hipStreamEndCapture(st->stream, &graph);
+            prof_suspend(0);
+            if (rc) return rc;
+            if (IM_HIP(ec)) return ERROR_CUDA_BLEW_UP;
+            ec = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (IM_HIP(ec)) return ERROR_CUDA_BLEW_UP;
+            continue;  // this iteration already ran eagerly
+        }
+        if (IM_HIP(hipGraphLaunch(g, st->stream))) return ERROR_CUDA_BLEW_UP;
+    }
+    return SUCCESSFUL;
+}
+
+int imogen_timeloop_clock(ImogenTimeloop *T, double *time, double *dt)
+{
+    if (!T) return IM_FAIL(ERROR_NULL_POINTER, "null time loop");
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    double c[2];
+    int rc = IM_HIP(hipMemcpyAsync(c, T->impl.dbuf, sizeof(c), hipMemcpyDeviceToHost, st->stream));
+    if (!rc) rc = IM_HIP(hipStreamSynchronize(st->stream));
+    if (rc) return rc;
+    if (time) *time = c[0];
+    if (dt) *dt = c[1];
+    return SUCCESSFUL;
+}
+
+int imogen_timeloop_set_time(ImogenTimeloop *T, double time)
+{
+    if (!T) return IM_FAIL(ERROR_NULL_POINTER, "null time loop");
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    int rc = IM_HIP(hipMemcpyAsync(T->impl.dbuf, &time, sizeof(double), hipMemcpyHostToDevice, st->stream));
+    if (!rc) rc = IM_HIP(hipStreamSynchronize(st->stream));
+    return rc;
+}
+
+int imogen_timeloop_destroy(ImogenTimeloop *T)
+{
+    if (!T) return SUCCESSFUL;
+    DeviceState *st = state();
+    if (st) (void)hipStreamSynchronize(st->stream);
+    drop_graphs(&T->impl);
+    if (T->impl.dbuf) (void)hipFree(T->impl.dbuf);
+    delete T;
+    return SUCCESSFUL;
+}
+
+}  // extern "C"

@@ -121,8 +121,10 @@ __global__ __launch_bounds__(256) void xj_freeze_march(const double *__restrict_
 template <bool CORRECT>
 __global__ __launch_bounds__(256) void xj_stage_x(const double *__restrict__ src, const double *base,
                                                   double *dst, int64_t pitch, int N, int64_t nlines, int nseg,
-                                                  double lambda, XjConsts k, const double *__restrict__ cf)
+                                                  double lambda, const double *__restrict__ lamp, double lscale,
+                                                  XjConsts k, const double *__restrict__ cf)
 {
+    if (lamp) lambda = lscale * lamp[0];  // time step kept on the device
     const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (wave >= nlines * nseg) return;
     const int lane = threadIdx.x & 63;
@@ -168,8 +170,10 @@ template <int DIR, bool CORRECT>
 __global__ __launch_bounds__(256) void xj_stage_march(const double *__restrict__ src, const double *base,
                                                       double *dst, int64_t pitch, int nx, int64_t sstride,
                                                       int64_t ostride, int N, int S, int seglen, double lambda,
-                                                      XjConsts k, const double *__restrict__ cf)
+                                                      const double *__restrict__ lamp, double lscale, XjConsts k,
+                                                      const double *__restrict__ cf)
 {
+    if (lamp) lambda = lscale * lamp[0];  // time step kept on the device
     constexpr int NS = DIR + 1;
     const int lane = threadIdx.x & 63;
     const int W = 64 / S;
@@ -252,14 +256,15 @@ static int xj_freeze(const double *q, int64_t pitch, int nx, int ny, int nz, int
 
 template <bool CORRECT>
 static int xj_stage(const double *src, const double *base, double *dst, int64_t pitch, int nx, int ny, int nz,
-                    int dir, double lambda, const XjConsts &k, const double *cf, hipStream_t s)
+                    int dir, double lambda, const double *lamp, double lscale, const XjConsts &k, const double *cf,
+                    hipStream_t s)
 {
     if (dir == 1) {
         const int64_t nlines = (int64_t)ny * nz;
         const int nseg = (nx + 59) / 60;
         const int64_t waves = nlines * nseg;
         hipLaunchKernelGGL((xj_stage_x<CORRECT>), dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, src, base,
-                           dst, pitch, nx, nlines, nseg, lambda, k, cf);
+                           dst, pitch, nx, nlines, nseg, lambda, lamp, lscale, k, cf);
         return IM_LAUNCHED("xj_stage_x");
     }
     const int N = (dir == 2) ? ny : nz;
@@ -274,15 +279,15 @@ static int xj_stage(const double *src, const double *base, double *dst, int64_t 
     dim3 grid((unsigned)((nx + colsPerBlock - 1) / colsPerBlock), (unsigned)nother), block(256);
     if (dir == 2)
         hipLaunchKernelGGL((xj_stage_march<2, CORRECT>), grid, block, 0, s, src, base, dst, pitch, nx, sstride,
-                           ostride, N, S, seglen, lambda, k, cf);
+                           ostride, N, S, seglen, lambda, lamp, lscale, k, cf);
     else
         hipLaunchKernelGGL((xj_stage_march<3, CORRECT>), grid, block, 0, s, src, base, dst, pitch, nx, sstride,
-                           ostride, N, S, seglen, lambda, k, cf);
+                           ostride, N, S, seglen, lambda, lamp, lscale, k, cf);
     return IM_LAUNCHED("xj_stage_march");
 }
 
 int launch_xinjin_sweep(double *base, int64_t pitch, int nx, int ny, int nz, int dir, double lambda,
-                        double gamma, double rhomin, hipStream_t s)
+                        double gamma, double rhomin, hipStream_t s, const double *lamp)
 {
     if (dir < 1 || dir > 3) return IM_FAIL(ERROR_INVALID_ARGS, "sweep direction %d", dir);
     const int N = (dir == 1) ? nx : (dir == 2 ? ny : nz);
@@ -296,9 +301,9 @@ int launch_xinjin_sweep(double *base, int64_t pitch, int nx, int ny, int nz, int
     double *cf = U + 5 * pitch;
     const int slot = prof_begin(PROF_SWEEP_X + dir - 1, s);
     int rc = xj_freeze(base, pitch, nx, ny, nz, dir, k, cf, s);
-    if (!rc) rc = xj_stage<false>(base, base, U, pitch, nx, ny, nz, dir, 0.25 * lambda, k, cf, s);
+    if (!rc) rc = xj_stage<false>(base, base, U, pitch, nx, ny, nz, dir, 0.25 * lambda, lamp, 0.25, k, cf, s);
     if (!rc) rc = xj_freeze(U, pitch, nx, ny, nz, dir, k, cf, s);
-    if (!rc) rc = xj_stage<true>(U, base, base, pitch, nx, ny, nz, dir, 0.5 * lambda, k, cf, s);
+    if (!rc) rc = xj_stage<true>(U, base, base, pitch, nx, ny, nz, dir, 0.5 * lambda, lamp, 0.5, k, cf, s);
     prof_end(slot, s);
     return rc;
 }

@@ -14,6 +14,8 @@ is set, and nothing otherwise.
 """
 from __future__ import annotations
 
+import ctypes
+
 from . import abi
 from . import gpuclass as gc
 
@@ -62,3 +64,54 @@ class Simulation:
         self.time += 2 * dt
         self.iteration += 1
         return dt
+
+
+class DeviceLoop:
+    """Hydrodynamic iterations of Simulation (no sources) with dTime kept on
+    the device and each iteration replayed from a HIP graph
+    (imogen_timeloop_*). Bitwise the same results as Simulation.step()."""
+
+    def __init__(self, fluids, methods, geometry, cfl=0.4, limiter=abi.LIMITER_OSPRE, t_max=None):
+        if len(methods) != len(fluids):
+            raise ValueError("one fluid method per fluid")
+        self.fluids = list(fluids)
+        nf = len(fluids)
+        arr = (abi.MGArray * (5 * nf))()
+        params = (abi.FluidStepParams * nf)()
+        gam = (ctypes.c_double * nf)()
+        for i, (f, m) in enumerate(zip(fluids, methods)):
+            a = f.arrays()
+            for k in range(5):
+                arr[5 * i + k] = a[k]
+            params[i] = gc.step_params(0.0, 1, 0, int(m), geometry, f.gamma, f.MINMASS, f.Cisothermal, limiter)
+            gam[i] = f.gamma
+        g = geometry.params()
+        h = ctypes.c_void_p(0)
+        abi.call("imogen_timeloop_create", arr, nf, ctypes.byref(g), params, gam, float(cfl),
+                 float(t_max) if t_max is not None else 0.0, ctypes.byref(h))
+        self.h = h
+        self.iteration = 0
+
+    def run(self, iterations, graph=True):
+        abi.call("imogen_timeloop_run", self.h, self.iteration, int(iterations), 1 if graph else 0)
+        self.iteration += int(iterations)
+
+    def clock(self):
+        t, dt = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        abi.call("imogen_timeloop_clock", self.h, ctypes.byref(t), ctypes.byref(dt))
+        return t.value, dt.value
+
+    @property
+    def time(self):
+        return self.clock()[0]
+
+    def free(self):
+        if self.h:
+            abi.call("imogen_timeloop_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

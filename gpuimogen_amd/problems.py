@@ -66,16 +66,21 @@ def gaussian_field(n, kmax, rng):
 
 
 def gasdust3d(n=192, seed=0, gamma=5.0 / 3.0, eps_median=0.1, eps_sigma_ln=1.0, mach=0.3, kmax=8,
-              dust_ciso=0.01):
+              dust_cs=0.3, dust_gamma=1.01):
     """Two-fluid gas + dust box, n^3 periodic cells on the unit cube.
 
     Gas: rho = 1, P = 1, gamma = 5/3. Dust-to-gas ratio lognormal with median
     eps_median and log-width eps_sigma_ln, from a Gaussian random field with
     |k| <= kmax. Gas and dust velocities: independent Gaussian random fields
-    (|k| <= kmax) per component with standard deviation mach * c_s. The dust
-    is carried as a nearly pressureless isothermal fluid (sound speed
-    dust_ciso * c_s; HLL isothermal sweeps): its energy slab holds the kinetic
-    energy plus rho_d dust_ciso^2 c_s^2. Returns (gas, dust, h, dust_ciso_abs)."""
+    (|k| <= kmax) per component with standard deviation mach * c_s. Both
+    fluids are swept with HLLC, as the reference's gas-dust profile runs them
+    (run/profile_3DXYZ_gasdust.m:55-57). The dust is a gamma = 1.01 fluid
+    (fluidDetailModel('10um_iron_balls'), astroParams/fluidDetailModel.m:148)
+    with sound speed dust_cs: the drag source leaves the dust energy slab
+    unchanged (cudaSource2FluidDrag.cu:1678-1680, the dust is taken as
+    pressureless), so the dust's internal energy absorbs its kinetic energy
+    changes and must start large enough to stay positive.
+    Returns (gas, dust, h, dust_cs)."""
     rng = np.random.default_rng(seed)
     cs = np.sqrt(gamma * 1.0 / 1.0)
     rho = np.ones((n, n, n))
@@ -85,11 +90,8 @@ def gasdust3d(n=192, seed=0, gamma=5.0 / 3.0, eps_median=0.1, eps_sigma_ln=1.0, 
     eps = eps_median * np.exp(eps_sigma_ln * gaussian_field(n, kmax, rng))
     rd = eps * rho
     vd = [mach * cs * gaussian_field(n, kmax, rng) for _ in range(3)]
-    ciso = dust_ciso * cs
-    dust = np.empty((5, n, n, n))
-    dust[0] = rd
-    dust[1] = 0.5 * rd * (vd[0] ** 2 + vd[1] ** 2 + vd[2] ** 2) + rd * ciso * ciso
-    for i in range(3):
-        dust[2 + i] = rd * vd[i]
+    dust = conserved(rd, vd[0], vd[1], vd[2], rd * dust_cs ** 2 / dust_gamma, dust_gamma)
+    del vd
     dx = 1.0 / n
+    ciso = dust_cs
     return gas, dust, (dx, dx, dx), ciso

@@ -340,6 +340,36 @@ int imogen_ensemble_autotune(ImogenEnsemble *e, int trials, int *chosen);
 int imogen_ensemble_destroy(ImogenEnsemble *e);
 
 /* ======================================================================== */
+/* Time loop on the device                                                  */
+/* ======================================================================== */
+
+/* The iteration of imogen.m:115-139 for hydrodynamics (TimeManager.update,
+ * time/TimeManager.m:124-186, then fluidstep(+1) and fluidstep(-1) of every
+ * fluid, flux.cu:27-155) with dTime kept in device memory: no host round trip
+ * per CFL reduction (cflTimestep.cu:214), one replayed HIP graph per
+ * iteration. The host-driven equivalent is imogen_cflTimestepFused on every
+ * fluid, dTime = cfl * min(tau) (clamped so t + 2 dTime <= tmax when
+ * tmax > 0), then performFluidUpdate_3D(+1), (-1) per fluid with that dTime:
+ * the results are bitwise the same. fluids holds nfluids x 5 MGArray (each
+ * fluid a 5-slab canister), params one FluidStepParams per fluid (method,
+ * gamma, floors, C_iso, limiter; dt, stepNumber, stepDirection are ignored),
+ * cflGamma (optional) the gamma of each fluid's CFL sound speed, i.e. the
+ * gamma imogen_cflTimestepFused would get (default: thermoGamma; the two
+ * differ for isothermal fluids, whose fluid step uses gamma = 2,
+ * flux_ML_iface.cu:128-133). The arrays must stay allocated while the loop
+ * exists. */
+typedef struct ImogenTimeloop ImogenTimeloop;
+int imogen_timeloop_create(MGArray *fluids, int nfluids, const GeometryParams *geo, const FluidStepParams *params,
+                           const double *cflGamma, double cfl, double tmax, ImogenTimeloop **out);
+/* Iterations firstIteration .. firstIteration + iterations - 1 (the number
+ * picks the sweep order as stepNumber does); returns without waiting. */
+int imogen_timeloop_run(ImogenTimeloop *loop, int firstIteration, int iterations, int useGraph);
+/* Simulation time and the last dTime (waits). */
+int imogen_timeloop_clock(ImogenTimeloop *loop, double *time, double *dt);
+int imogen_timeloop_set_time(ImogenTimeloop *loop, double time);
+int imogen_timeloop_destroy(ImogenTimeloop *loop);
+
+/* ======================================================================== */
 /* Runtime utilities (no counterpart in the reference: it used the legacy   */
 /* default stream; this library issues all work on one stream per device).  */
 /* ======================================================================== */

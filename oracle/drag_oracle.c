@@ -6,8 +6,12 @@
  *
  * Restates sourcefunction_2FluidDrag's constants (gpuclass/cudaSource2FluidDrag.cu:189-243),
  * the drag-rate evaluation cukern_GasDustDrag_GeneralLinearCore + allregimeCdrag
- * (:819-890), the initial-state preparation cukern_prepareForERK_h0 (:1874-1908)
- * and the LogTrap2/3 integrator cukern_LogTrapSolve (:1532-1686).
+ * (:819-890), the initial-state preparation cukern_prepareForERK_h0 (:1874-1908),
+ * the LogTrap2/3 integrator cukern_LogTrapSolve (:1532-1686), and the explicit
+ * midpoint and classical RK4 integrators of |dv| (solveDragEMP :305-370,
+ * solveDragRK4 :372-467: cukern_findInitialDeltaV :1014-1056,
+ * cukern_GasDustDrag_GeneralAccel :898-946, cukern_SolveRK_single/final
+ * :1062-1100, cukern_applyFinalDeltaV :1160-1237).
  *
  * The "exact" integrator (method 6) has no counterpart in the reference's
  * compiled code; it integrates the same scalar relaxation problem that the
@@ -182,10 +186,106 @@ static void apply_factor(double *g, double *d, long pitch, double f)
     g[pitch] -= .5 * pdustsq / d[0];
 }
 
+/* cukern_GasDustDrag_GeneralAccel (:898-946): d|dv|/dt at |dv| = magdv and
+ * specific internal energy u (no small-|dv| guard, unlike the linear core). */
+static double rk_accel(const double *P, double rhog, double rhod, double magdv, double u)
+{
+    double Tn = P[3] * u;
+    double Re = P[5] * rhog * magdv * pow(Tn, P[1]);
+    double Kn = P[4] * pow(Tn, P[2]) / rhog;
+    double Cd = cdrag(Re, Kn);
+    return -(Cd * magdv * magdv * (rhog + rhod) * P[6]);
+}
+
+/* the gas heating of a stage at |dv| = s (GeneralAccel<false>, :921-924) */
+static double rk_u(double u0, double s0, double s, double rhog, double rhod)
+{
+    return u0 + .5 * rhod * (s0 * s0 - s * s) / (rhog + rhod);
+}
+
+/* cukern_findInitialDeltaV (:1014-1056) */
+static void rk_initial(const double *g, const double *d, long pitch, double *magdv, double *u0)
+{
+    double rhoginv = 1 / g[0];
+    double rhodinv = 1 / d[0];
+    double q = g[2 * pitch];
+    double u = q * rhoginv - d[2 * pitch] * rhodinv;
+    double momsq = q * q;
+    double dvsq = u * u;
+    q = g[3 * pitch];
+    u = q * rhoginv - d[3 * pitch] * rhodinv;
+    momsq += q * q;
+    dvsq += u * u;
+    q = g[4 * pitch];
+    u = q * rhoginv - d[4 * pitch] * rhodinv;
+    momsq += q * q;
+    dvsq += u * u;
+    *magdv = sqrt(dvsq);
+    *u0 = (g[pitch] - .5 * momsq * rhoginv) * rhoginv;
+}
+
+/* cukern_applyFinalDeltaV (:1160-1237): rescale dv to |dv| = s1, rebuild the
+ * momenta about the barycentric velocity, conserve gas E + dust KE, and reset
+ * the dust energy to its kinetic energy plus 1e-4 rho_d (:1230-1233). */
+static void rk_apply(double *g, double *d, long pitch, double s1)
+{
+    double rhog = g[0], rhod = d[0];
+    double vstick[3], dvhat[3];
+    for (int c = 0; c < 3; c++) {
+        double p1 = g[(c + 2) * pitch], p2 = d[(c + 2) * pitch];
+        vstick[c] = (p1 + p2) / (rhog + rhod);
+        dvhat[c] = p1 / rhog - p2 / rhod;
+    }
+    double a = s1 / sqrt(dvhat[0] * dvhat[0] + dvhat[1] * dvhat[1] + dvhat[2] * dvhat[2]);
+    dvhat[0] *= a;
+    dvhat[1] *= a;
+    dvhat[2] *= a;
+    double b = rhog * rhod / (rhog + rhod);
+    double dustmom = 0.0, dustmomfin = 0.0;
+    for (int c = 0; c < 3; c++) {
+        double *gp = g + (c + 2) * pitch, *dp = d + (c + 2) * pitch;
+        if (c == 0) dustmom = dp[0] * dp[0];
+        else dustmom += dp[0] * dp[0];
+        gp[0] = rhog * vstick[c] + dvhat[c] * b;
+        double q = rhod * vstick[c] - dvhat[c] * b;
+        dp[0] = q;
+        if (c == 0) dustmomfin = q * q;
+        else dustmomfin += q * q;
+    }
+    g[pitch] += .5 * (dustmom - dustmomfin) / d[0];
+    d[pitch] = .5 * dustmomfin / d[0] + 1e-4 * d[0];
+}
+
+/* |dv| after dt by the explicit midpoint rule (solveDragEMP, :305-370) or
+ * Kutta's classical RK4 (solveDragRK4, :372-467), in the reference's order of
+ * stage and accumulator updates. */
+double orc_drag_rk_magdv(const double *P, double rhog, double rhod, double s0, double u0, double dt, int method)
+{
+    double k = rk_accel(P, rhog, rhod, s0, u0);
+    if (method == ORC_DRAG_EMP) {
+        double acc = 0.0 * k;                    /* SolveRK_single<true>, B = 0 */
+        double s = s0 + (.5 * dt) * k;
+        k = rk_accel(P, rhog, rhod, s, rk_u(u0, s0, s, rhog, rhod));
+        return s0 + dt * (acc + 1.0 * k);         /* SolveRK_final, B = 1, W = dt */
+    }
+    double acc = 1.0 * k;
+    double s = s0 + (.5 * dt) * k;
+    k = rk_accel(P, rhog, rhod, s, rk_u(u0, s0, s, rhog, rhod));
+    s = s0 + (.5 * dt) * k;
+    acc += 2.0 * k;
+    k = rk_accel(P, rhog, rhod, s, rk_u(u0, s0, s, rhog, rhod));
+    s = s0 + (1.0 * dt) * k;
+    acc += 2.0 * k;
+    k = rk_accel(P, rhog, rhod, s, rk_u(u0, s0, s, rhog, rhod));
+    return s0 + (dt / 6.0) * (acc + 1.0 * k);
+}
+
 int orc_drag(double *gas, double *dust, const orc_grid *g, const orc_thermo *tg, const orc_thermo *td, double dt,
              int method, double tol, long *work)
 {
-    if (method != ORC_DRAG_LOGTRAP2 && method != ORC_DRAG_LOGTRAP3 && method != ORC_DRAG_EXACT) return -105;
+    if (method != ORC_DRAG_LOGTRAP2 && method != ORC_DRAG_LOGTRAP3 && method != ORC_DRAG_EXACT &&
+        method != ORC_DRAG_EMP && method != ORC_DRAG_RK4)
+        return -105;
     double P[9];
     orc_drag_params(tg, td, P);
     const long n = (long)g->nx * g->ny * g->nz;
@@ -196,6 +296,11 @@ int orc_drag(double *gas, double *dust, const orc_grid *g, const orc_thermo *tg,
     for (i = 0; i < n; i++) {
         double *gc = gas + i, *dc = dust + i;
         double s0, u0;
+        if (method == ORC_DRAG_EMP || method == ORC_DRAG_RK4) {
+            rk_initial(gc, dc, pitch, &s0, &u0);
+            rk_apply(gc, dc, pitch, orc_drag_rk_magdv(P, gc[0], dc[0], s0, u0, dt, method));
+            continue;
+        }
         prepare(gc, dc, pitch, &s0, &u0);
         double f;
         if (method == ORC_DRAG_EXACT) {

@@ -95,7 +95,13 @@ typedef struct {
 /* Drag integrators: the reference's method codes 4/5
  * (cudaSource2FluidDrag.cu:257-262) plus this build's adaptive reference
  * integrator. */
-enum orc_drag_method { ORC_DRAG_LOGTRAP2 = 4, ORC_DRAG_LOGTRAP3 = 5, ORC_DRAG_EXACT = 6 };
+enum orc_drag_method {
+    ORC_DRAG_EMP = 0,
+    ORC_DRAG_RK4 = 1,
+    ORC_DRAG_LOGTRAP2 = 4,
+    ORC_DRAG_LOGTRAP3 = 5,
+    ORC_DRAG_EXACT = 6
+};
 
 /* The 9 drag constants hostDrag[0..8] (cudaSource2FluidDrag.cu:210-220). */
 void orc_drag_params(const orc_thermo *gas, const orc_thermo *dust, double *P9);
@@ -106,6 +112,7 @@ double orc_drag_k(const double *P9, double rhog, double rhod, double magdv, doub
 /* |dv(t)|/|dv(0)| of the relaxation problem, adaptive Dormand-Prince 5(4) on
  * ln|dv| to absolute tolerance tol. steps: step count (negative if the budget
  * of 4096 steps ran out). */
+double orc_drag_rk_magdv(const double *P, double rhog, double rhod, double s0, double u0, double dt, int method);
 double orc_drag_exact_ratio(const double *P9, double rhog, double rhod, double s0, double u0, double t, double tol,
                             int *steps);
 

@@ -37,7 +37,7 @@ class Thermo(ctypes.Structure):
                 ("sigma0", ctypes.c_double), ("sigmaTindex", ctypes.c_double)]
 
 
-DRAG_LOGTRAP2, DRAG_LOGTRAP3, DRAG_EXACT = 4, 5, 6
+DRAG_EMP, DRAG_RK4, DRAG_LOGTRAP2, DRAG_LOGTRAP3, DRAG_EXACT = 0, 1, 4, 5, 6
 
 _lib = None
 
@@ -74,6 +74,9 @@ def lib():
         L.orc_drag_params.argtypes = [ctypes.POINTER(Thermo), ctypes.POINTER(Thermo), dp]
         L.orc_drag_k.argtypes = [dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double]
         L.orc_drag_k.restype = ctypes.c_double
+        L.orc_drag_rk_magdv.argtypes = [dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_int]
+        L.orc_drag_rk_magdv.restype = ctypes.c_double
         L.orc_drag_exact_ratio.argtypes = [dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                            ctypes.c_double, ctypes.c_double, ip]
         L.orc_drag_exact_ratio.restype = ctypes.c_double
@@ -191,6 +194,11 @@ def drag_params(tgas, tdust):
 def drag_k(P, rhog, rhod, magdv, u):
     P = np.ascontiguousarray(P, dtype=np.float64)
     return lib().orc_drag_k(_dptr(P), rhog, rhod, magdv, u)
+
+
+def drag_rk_magdv(P, rhog, rhod, s0, u0, dt, method):
+    """|dv| after dt by the EMP (method 0) or RK4 (method 1) integrator."""
+    return lib().orc_drag_rk_magdv(_dptr(P), rhog, rhod, s0, u0, dt, method)
 
 
 def drag_exact_ratio(P, rhog, rhod, s0, u0, t, tol=1e-12):

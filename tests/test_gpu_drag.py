@@ -89,10 +89,35 @@ def test_drag_parity(method, tol):
     compare_slabs(dd, rd, tol, f"dust m{method}")
 
 
+@pytest.mark.parametrize("method", [0, 1])
+def test_drag_emp_rk4_parity(method):
+    """Explicit midpoint / RK4 (solveDragEMP / solveDragRK4). They are only
+    conditionally stable and, like the reference, give NaN at |dv| = 0 or when
+    a stage |dv| turns negative, so the state has relative motion everywhere
+    and dt keeps k dt <= 1/2."""
+    sh = (5, 9, 66)
+    g, d = random_pair(sh, seed=20 + method)
+    d[2:, 0, 0, :3] = d[2:, 0, 0, :3] * 0.5 + 0.01
+    tg, td = gas_dust_thermo()
+    P9 = po.drag_params(tg, td)
+    dv = g[2:] / g[0] - d[2:] / d[0]
+    s0 = np.sqrt((dv ** 2).sum(0))
+    u0 = (g[1] - .5 * (g[2:] ** 2).sum(0) / g[0]) / g[0]
+    kmax = max(po.drag_k(P9, a, b, c, e) for a, b, c, e in zip(g[0].ravel(), d[0].ravel(), s0.ravel(), u0.ravel()))
+    dt = 0.5 / kmax
+    print(f"drag method {method} shape {sh} dt {dt:.3e}", flush=True)
+    gg, dd = run_gpu_drag(g, d, dt, method, sh)
+    rg, rd = g.copy(), d.copy()
+    po.drag(rg, rd, tg, td, dt, method)
+    assert np.all(np.isfinite(rg)) and np.all(np.isfinite(rd))
+    compare_slabs(gg, rg, 1e-12, f"gas m{method}")
+    compare_slabs(dd, rd, 1e-12, f"dust m{method}")
+
+
 def test_unbuilt_drag_methods_fail_loudly():
     sh = (2, 3, 4)
     g, d = random_pair(sh, seed=1)
-    for m in (0, 1, 2, 3):
+    for m in (2, 3):
         with pytest.raises(abi().ImogenError):
             run_gpu_drag(g, d, 0.01, m, sh)
 

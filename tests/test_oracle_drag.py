@@ -9,6 +9,10 @@ What pins what:
                             conservation (DustyBoxAnalyzer.m:135-140)
   LogTrap2 / LogTrap3       their convergence orders 2 and 3 against the exact
                             solution (the reference's tsDustybox.m criterion)
+  EMP / RK4 on |dv|         local truncation orders 3 and 5 (global 2 and 4)
+                            against the exact solution; in the linear-drag
+                            limit the Taylor polynomials of exp(-x) to 2nd
+                            and 4th order, which fix every stage weight
   the cell update           exact momentum and total-energy conservation,
                             unchanged barycentric velocity, dv direction kept
 """
@@ -158,3 +162,91 @@ def test_drag_invariants(method):
     cos = (dv * dv0).sum(0)[moving] / (n0 * n1)[moving]
     assert np.all(cos > 1 - 1e-12)
     assert np.allclose(dv[:, 0, 0, :2], 0, atol=1e-15)
+
+
+def test_emp_rk4_local_orders_against_exact():
+    """One step of size dt: the |dv| error shrinks as dt^3 (EMP) and dt^5 (RK4)."""
+    tg, td = gas_dust_thermo()
+    P9 = po.drag_params(tg, td)
+    rhog, rhod, Pg = 1.0, 0.3, 1.0
+    u0 = Pg / ((tg.gamma - 1) * rhog)
+    s0 = 0.5
+    k0 = po.drag_k(P9, rhog, rhod, s0, u0)
+    for method, local, xs in ((po.DRAG_EMP, 3, (0.4, 0.2, 0.1, 0.05)), (po.DRAG_RK4, 5, (0.1, 0.05, 0.025))):
+        errs = []
+        for x in xs:
+            dt = x / k0
+            exact, _ = po.drag_exact_ratio(P9, rhog, rhod, s0, u0, dt, 1e-14)
+            errs.append(abs(po.drag_rk_magdv(P9, rhog, rhod, s0, u0, dt, method) - exact * s0))
+        orders = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+        assert np.all(orders > local - 0.45), (method, errs, orders)
+        assert np.all(orders < local + 0.6), (method, errs, orders)
+
+
+def test_emp_rk4_linear_limit_is_taylor_polynomial():
+    """With k independent of |dv| and no heating (Stokes regime, Re -> 0,
+    Kn -> 0, rho_d -> 0): d|dv|/dt = -K |dv|, and one step must return
+    s0 (1 - x + x^2/2) for EMP and s0 sum_{j<=4} (-x)^j / j! for RK4, x = K dt.
+    A wrong stage node or weight changes a coefficient."""
+    tg, td = gas_dust_thermo()
+    P9 = po.drag_params(tg, td)
+    rhog, rhod, u0 = 100.0, 0.5, 1.0  # Kn ~ 1e-10
+    s0 = 1e-21                        # Re ~ 2e-20: the .45 Re^-.319 term is 1e-14 of 3/Re
+    # Stokes-limit rate from the constants alone: Cd = (3/Re) / Cunningham,
+    # k = Cd |dv| (rho_g + rho_d) epsilon, independent of |dv|
+    T = P9[3] * u0
+    Kn = P9[4] * T ** P9[2] / rhog
+    cunningham = 1 + Kn * (1.142 + 0.558 * np.exp(-0.999 / Kn))
+    K = 3 * (rhog + rhod) * P9[6] / (P9[5] * rhog * T ** P9[1] * cunningham)
+    for method, coeffs in ((po.DRAG_EMP, [1, -1, 1 / 2]), (po.DRAG_RK4, [1, -1, 1 / 2, -1 / 6, 1 / 24])):
+        for x in (0.3, 0.6, 0.9):  # larger x drives an RK4 stage |dv| negative (NaN, as in the reference)
+            got = po.drag_rk_magdv(P9, rhog, rhod, s0, u0, x / K, method) / s0
+            want = sum(c * x ** j for j, c in enumerate(coeffs))
+            assert got == pytest.approx(want, rel=2e-8), (method, x, got, want)
+
+
+@pytest.mark.parametrize("method", [po.DRAG_EMP, po.DRAG_RK4])
+def test_emp_rk4_cell_update(method):
+    """cukern_applyFinalDeltaV: momentum conserved, gas E + dust KE conserved,
+    dv keeps its direction with |dv| = the integrator's value, and the dust
+    energy reset to its kinetic energy + 1e-4 rho_d."""
+    tg, td = gas_dust_thermo()
+    P9 = po.drag_params(tg, td)
+    rng = np.random.default_rng(9)
+    sh = (3, 4, 5)
+    g = np.zeros((5,) + sh)
+    d = np.zeros((5,) + sh)
+    g[0] = rng.uniform(0.5, 2, sh)
+    d[0] = g[0] * np.exp(rng.normal(np.log(0.1), 1.0, sh))
+    vg = rng.normal(0, 0.4, (3,) + sh)
+    vd = rng.normal(0, 0.4, (3,) + sh)
+    g[2:] = g[0] * vg
+    d[2:] = d[0] * vd
+    g[1] = rng.uniform(0.5, 2, sh) / (tg.gamma - 1) + .5 * g[0] * (vg ** 2).sum(0)
+    d[1] = .5 * d[0] * (vd ** 2).sum(0) + 0.3
+    g0, d0 = g.copy(), d.copy()
+    # explicit methods are conditionally stable: keep k dt <= 1/2 in every cell
+    # (beyond it a stage |dv| can go negative and the reference gives NaN)
+    kmax = 0.0
+    for idx in np.ndindex(*sh):
+        c = (slice(None),) + idx
+        s0 = np.linalg.norm(g0[2:][c] / g0[0][idx] - d0[2:][c] / d0[0][idx])
+        u0 = (g0[1][idx] - .5 * (g0[2:][c] ** 2).sum() / g0[0][idx]) / g0[0][idx]
+        kmax = max(kmax, po.drag_k(P9, g0[0][idx], d0[0][idx], s0, u0))
+    dt = 0.5 / kmax
+    po.drag(g, d, tg, td, dt, method)
+    assert np.all(np.isfinite(g)) and np.all(np.isfinite(d))
+    assert np.allclose(g[2:] + d[2:], g0[2:] + d0[2:], rtol=0, atol=1e-14)
+    ekd0 = .5 * (d0[2:] ** 2).sum(0) / d0[0]
+    ekd = .5 * (d[2:] ** 2).sum(0) / d[0]
+    assert np.allclose(g[1] + ekd, g0[1] + ekd0, rtol=1e-14, atol=1e-14)
+    assert np.allclose(d[1], ekd + 1e-4 * d[0], rtol=1e-15, atol=0)
+    dv0 = g0[2:] / g0[0] - d0[2:] / d0[0]
+    dv = g[2:] / g[0] - d[2:] / d[0]
+    n0 = np.sqrt((dv0 ** 2).sum(0))
+    n1 = np.sqrt((dv ** 2).sum(0))
+    assert np.all((dv * dv0).sum(0) / (n0 * n1) > 1 - 1e-12)
+    for idx in np.ndindex(*sh):
+        u0 = (g0[1][idx] - .5 * (g0[2:][(slice(None),) + idx] ** 2).sum() / g0[0][idx]) / g0[0][idx]
+        want = po.drag_rk_magdv(P9, g0[0][idx], d0[0][idx], n0[idx], u0, dt, method)
+        assert n1[idx] == pytest.approx(want, rel=1e-12)
