@@ -220,6 +220,9 @@ __device__ __forceinline__ void faces(const HydroConsts &k, const V5 &w, const V
 // The four reference cases are evaluated without divergence: the upwind side
 // and its wave speed are selected first, then the plain and star fluxes of
 // that side are both formed and one is kept.
+// BRANCH: form the plain or the star flux under a branch (fewer live values;
+// the X kernel's register budget needs it) instead of the select form below.
+template <bool BRANCH>
 __device__ __forceinline__ V5 hllc_flux(const HydroConsts &k, const V5 &L, const V5 &R)
 {
     // velocity jump squared (cudaFluidStep.cu:1318-1322)
@@ -260,6 +263,31 @@ __device__ __forceinline__ V5 hllc_flux(const HydroConsts &k, const V5 &L, const
     const double iS = left ? iL : iR;
 
     double ksum = S.c * S.c + S.d * S.d + S.e * S.e;  // |v|^2 of the upwind face
+    if (BRANCH) {
+        V5 out;
+        if (!star) {
+            // cases 0/1 (cudaFluidStep.cu:1372-1383, 1416-1430)
+            const double Ek = .5 * ksum * S.a;
+            const double m = S.a * S.c;
+            out.a = m;
+            out.b = S.c * (Ek + k.govergm1 * S.b);
+            out.c = m * S.c + S.b;
+            out.d = S.d * m;
+            out.e = S.e * m;
+        } else {
+            // cases 2/3 (cudaFluidStep.cu:1384-1415)
+            const double rb = div_fast(S.a * (sk - S.c), sk - sm);  // rho * beta
+            const double G = .5 * ksum;
+            const double m = rb * sm;
+            const double Cc = sk * (sm - S.c);
+            out.a = m;
+            out.b = rb * sm * (k.govergm1 * S.b * iS + G + Cc);
+            out.c = S.b + rb * (Cc + sm * S.c);
+            out.d = S.d * m;
+            out.e = S.e * m;
+        }
+        return out;
+    }
     // Plain flux of the upwind side (cases 0/1, cudaFluidStep.cu:1372-1383,
     // 1416-1430) and its star flux (cases 2/3, :1384-1415) share their form:
     // with mass flux m (rho v of the face, or rho beta S_M), the energy flux
@@ -307,9 +335,10 @@ struct Hllc {
     {
         faces<LIM>(k, w, dl, dr, rt, L, R);
     }
+    template <bool LOWREG = false>
     __device__ static __forceinline__ V5 flux(const HydroConsts &k, const V5 &L, const V5 &R)
     {
-        return hllc_flux(k, L, R);
+        return hllc_flux<LOWREG>(k, L, R);
     }
 };
 
@@ -370,6 +399,7 @@ struct Hll {
         R.e = w.e + s;
     }
     // L, R = (rho, vN, P, vT1, vT2), the reference's A..E (cudaFluidStep.cu:2124-2196)
+    template <bool LOWREG = false>
     __device__ static __forceinline__ V5 flux(const HydroConsts &k, const V5 &L, const V5 &R)
     {
         const double Ut = R.b - L.b, St1 = R.d - L.d, St2 = R.e - L.e;

@@ -103,7 +103,7 @@ __device__ __forceinline__ void x_fluxes(const HydroConsts &hc, int rt, V5 (&U)[
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const V5 right = (k + 1 < K) ? sel5(k + 1 < cnt, Lf[(k + 1 < K) ? k + 1 : 0], LfR) : LfR;
-        F[k] = M::flux(hc, Rf[k], right);
+        F[k] = M::template flux<(K >= 4)>(hc, Rf[k], right);
     }
     V5 Flast = F[0];
 #pragma unroll
