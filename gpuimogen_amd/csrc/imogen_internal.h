@@ -27,7 +27,7 @@ struct DeviceState {
     void *scratch = nullptr;
     size_t scratchBytes = 0;
     double *pinned = nullptr;   // 64 doubles of pinned host memory
-    double *devScalars = nullptr;  // 64 doubles of device memory for reductions
+    double *devScalars = nullptr;  // 4096 doubles of device memory for reductions
 };
 
 // State of the current device (initialised on first use).
