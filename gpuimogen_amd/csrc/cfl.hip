@@ -124,4 +124,11 @@ int launch_cfl(const double *base, int64_t pitch, int64_t n, const double *cs, d
     return IM_LAUNCHED("cfl_pass2");
 }
 
+int launch_cfl_final(const double *part, int64_t n, int halve, double *devOut, hipStream_t s)
+{
+    if (n < 1 || n > (int64_t)1 << 30) return IM_FAIL(ERROR_INVALID_ARGS, "CFL partial count %lld", (long long)n);
+    hipLaunchKernelGGL(cfl_pass2, dim3(1), dim3(256), 0, s, part, (int)n, halve ? 1.0 : 0.0, devOut);
+    return IM_LAUNCHED("cfl_pass2");
+}
+
 }  // namespace imogen

@@ -6,8 +6,12 @@
 
 namespace imogen {
 
+// The CFL arithmetic is evaluated exactly as written (no FMA contraction), so
+// the limit of a cell is bitwise the same in the CFL kernel and in a sweep
+// that folds the CFL in, whatever the surrounding code lets the compiler fuse.
 __device__ __forceinline__ double cs_of(double rho, double E, double px, double py, double pz, double gg1)
 {
+#pragma clang fp contract(off)
     double rhoinv = 1 / rho;
     double csq = gg1 * (E - .5 * (px * px + py * py + pz * pz) * rhoinv) * rhoinv;
     if (csq < 0.0) csq = 0.0;
@@ -35,6 +39,7 @@ struct CflArgs {
 __device__ __forceinline__ double cfl_cell(const CflArgs &a, double rho, double E, double px, double py,
                                            double pz, double cIn)
 {
+#pragma clang fp contract(off)
     const double c = a.fused ? cs_of(rho, E, px, py, pz, a.gg1) : cIn;
     double u;
     if (!a.xinjin) {
@@ -60,5 +65,17 @@ __device__ __forceinline__ double cfl_cell(const CflArgs &a, double rho, double 
     }
     return u;
 }
+
+// CFL minimum folded into a sweep: each wave of the sweep writes the minimum
+// time-step limit over the cells it stores whose value survives the
+// direction's boundary conditions (cells [lo, hi) of the swept direction;
+// the ghost cells the boundary then overwrites are copies or mirror images
+// of kept cells, with the same limit). part = null: off.
+struct CflFuse {
+    double *part;
+    CflArgs a;
+    int lo, hi;
+    int64_t cap;  // entries available in part
+};
 
 }  // namespace imogen

@@ -182,8 +182,10 @@ __global__ __launch_bounds__(256) void ens_sweep_y(const EnsProblem *__restrict_
     double *col = P.q + (colValid ? x : 0);
     const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nthr = (int64_t)gridDim.x * blockDim.x;
+    const CflFuse nocfl{};
+    double cmin = 0.0;
     march_body<Hllc, 2, LIM>(col, P.pitch, (int64_t)P.nx, N, seglen, j0, L, lambda, hc, rt, defer, gtid, nthr,
-                             sring[wib], lane);
+                             sring[wib], lane, nocfl, cmin);
 }
 
 // constant-extrapolation ghost cells (cudaStatics.cu:1339-1366, 1440-1466 with

@@ -32,7 +32,7 @@ namespace imogen {
 // ---------------------------------------------------------------------------
 template <class M, int LIM>
 static int launch_x(double *q, int64_t pitch, int N, int64_t nlines, double lambda, const double *lamp,
-                    const HydroConsts &hc, int rt, hipStream_t s)
+                    const HydroConsts &hc, int rt, const CflFuse &cf, hipStream_t s)
 {
     int K;
     if (N <= 64) K = 1;
@@ -51,18 +51,27 @@ static int launch_x(double *q, int64_t pitch, int N, int64_t nlines, double lamb
     // takes several line groups and prefetches the next one.
     if (waves > 2048) waves = 2048;
     const int64_t blocks = (waves + 3) / 4;
+    if (cf.part && blocks * 4 > cf.cap) return IM_FAIL(ERROR_INVALID_ARGS, "CFL partial buffer too small");
     dim3 grid((unsigned)blocks), block(256);
     const bool full = (nl * K == N) && (64 % nl == 0);
     double *dummy = (double *)scratch((size_t)blocks * 256 * 5 * sizeof(double));
     if (!dummy) return IM_FAIL(ERROR_NOMEM, "no scratch for the X sweep");
 #define IM_XK(KK)                                                                                       \
     case KK:                                                                                            \
-        if (full)                                                                                       \
-            hipLaunchKernelGGL((sweep_x<M, KK, LIM, true>), grid, block, 0, s, q, pitch, N, nlines, nl,  \
-                               lambda, lamp, hc, rt, dummy);                                            \
-        else                                                                                            \
-            hipLaunchKernelGGL((sweep_x<M, KK, LIM, false>), grid, block, 0, s, q, pitch, N, nlines, nl, \
-                               lambda, lamp, hc, rt, dummy);                                            \
+        if (cf.part) {                                                                                  \
+            if (full)                                                                                   \
+                hipLaunchKernelGGL((sweep_x<M, KK, LIM, true, true>), grid, block, 0, s, q, pitch, N,    \
+                                   nlines, nl, lambda, lamp, hc, rt, dummy, cf);                        \
+            else                                                                                        \
+                hipLaunchKernelGGL((sweep_x<M, KK, LIM, false, true>), grid, block, 0, s, q, pitch, N,   \
+                                   nlines, nl, lambda, lamp, hc, rt, dummy, cf);                        \
+        } else if (full) {                                                                              \
+            hipLaunchKernelGGL((sweep_x<M, KK, LIM, true, false>), grid, block, 0, s, q, pitch, N,       \
+                               nlines, nl, lambda, lamp, hc, rt, dummy, cf);                            \
+        } else {                                                                                        \
+            hipLaunchKernelGGL((sweep_x<M, KK, LIM, false, false>), grid, block, 0, s, q, pitch, N,      \
+                               nlines, nl, lambda, lamp, hc, rt, dummy, cf);                            \
+        }                                                                                               \
         break;
     switch (K) {
         IM_XK(1)
@@ -80,7 +89,7 @@ static int launch_x(double *q, int64_t pitch, int N, int64_t nlines, double lamb
 
 template <class M, int LIM>
 static int launch_march(double *q, int64_t pitch, int nx, int ny, int nz, int dir, double lambda,
-                        const double *lamp, const HydroConsts &hc, int rt, hipStream_t s)
+                        const double *lamp, const HydroConsts &hc, int rt, const CflFuse &cf, hipStream_t s)
 {
     const int N = (dir == 2) ? ny : nz;
     const int64_t sstride = (dir == 2) ? (int64_t)nx : (int64_t)nx * ny;
@@ -99,48 +108,84 @@ static int launch_march(double *q, int64_t pitch, int nx, int ny, int nz, int di
     const int blocksPerRow = (wavesPerRow + 3) / 4;
     dim3 grid((unsigned)blocksPerRow, (unsigned)nother), block(256);
     const int64_t nthr = (int64_t)blocksPerRow * nother * 256;
+    if (cf.part && nthr / 64 > cf.cap) return IM_FAIL(ERROR_INVALID_ARGS, "CFL partial buffer too small");
     // 4 deferred output slots + 1 dummy slot of 5 doubles per lane
     double *defer = (double *)scratch((size_t)nthr * 25 * sizeof(double));
     if (!defer) return IM_FAIL(ERROR_NOMEM, "no scratch for deferred outputs");
-    if (dir == 2)
-        hipLaunchKernelGGL((sweep_march<M, 2, LIM>), grid, block, 0, s, q, pitch, nx, sstride, ostride, N, S,
-                           seglen, wavesPerRow, lambda, lamp, hc, rt, defer);
-    else
-        hipLaunchKernelGGL((sweep_march<M, 3, LIM>), grid, block, 0, s, q, pitch, nx, sstride, ostride, N, S,
-                           seglen, wavesPerRow, lambda, lamp, hc, rt, defer);
+#define IM_MARCH(D, C)                                                                                  \
+    hipLaunchKernelGGL((sweep_march<M, D, LIM, C>), grid, block, 0, s, q, pitch, nx, sstride, ostride, N, S, \
+                       seglen, wavesPerRow, lambda, lamp, hc, rt, defer, cf)
+    if (dir == 2) {
+        if (cf.part) IM_MARCH(2, true);
+        else IM_MARCH(2, false);
+    } else {
+        if (cf.part) IM_MARCH(3, true);
+        else IM_MARCH(3, false);
+    }
+#undef IM_MARCH
     return IM_LAUNCHED("sweep_march");
 }
 
 template <class M, int LIM>
 static int launch_dir(double *q, int64_t pitch, int nx, int ny, int nz, int dir, double lambda,
-                      const double *lamp, const HydroConsts &hc, int rt, hipStream_t s)
+                      const double *lamp, const HydroConsts &hc, int rt, const CflFuse &cf, hipStream_t s)
 {
     const int slot = prof_begin(PROF_SWEEP_X + dir - 1, s);
     int rc;
-    if (dir == 1) rc = launch_x<M, LIM>(q, pitch, nx, (int64_t)ny * nz, lambda, lamp, hc, rt, s);
-    else rc = launch_march<M, LIM>(q, pitch, nx, ny, nz, dir, lambda, lamp, hc, rt, s);
+    if (dir == 1) rc = launch_x<M, LIM>(q, pitch, nx, (int64_t)ny * nz, lambda, lamp, hc, rt, cf, s);
+    else rc = launch_march<M, LIM>(q, pitch, nx, ny, nz, dir, lambda, lamp, hc, rt, cf, s);
     prof_end(slot, s);
     return rc;
 }
 
 template <class M>
 static int launch_method(double *base, int64_t pitch, int nx, int ny, int nz, int dir, double lambda,
-                         const double *lamp, const HydroConsts &hc, int limiter, hipStream_t s)
+                         const double *lamp, const HydroConsts &hc, int limiter, const CflFuse &cf, hipStream_t s)
 {
     switch (limiter) {
-    case LIMITER_OSPRE: return launch_dir<M, 0>(base, pitch, nx, ny, nz, dir, lambda, lamp, hc, 0, s);
-    case LIMITER_VANLEER: return launch_dir<M, 1>(base, pitch, nx, ny, nz, dir, lambda, lamp, hc, 1, s);
+    case LIMITER_OSPRE: return launch_dir<M, 0>(base, pitch, nx, ny, nz, dir, lambda, lamp, hc, 0, cf, s);
+    case LIMITER_VANLEER: return launch_dir<M, 1>(base, pitch, nx, ny, nz, dir, lambda, lamp, hc, 1, cf, s);
     case LIMITER_MINMOD:
     case LIMITER_SUPERBEE:
     case LIMITER_VANALBADA:
     case LIMITER_MC:
-    case LIMITER_ZERO: return launch_dir<M, -1>(base, pitch, nx, ny, nz, dir, lambda, lamp, hc, limiter, s);
+    case LIMITER_ZERO: return launch_dir<M, -1>(base, pitch, nx, ny, nz, dir, lambda, lamp, hc, limiter, cf, s);
     default: return IM_FAIL(ERROR_INVALID_ARGS, "unknown slope limiter %d", limiter);
     }
 }
 
+// The grid arithmetic of launch_x / launch_march, for sizing fused-CFL buffers.
+int64_t sweep_waves(int nx, int ny, int nz, int dir)
+{
+    if (dir == 1) {
+        int K = 16;
+        if (nx <= 64) K = 1;
+        else if (nx <= 128) K = 2;
+        else if (nx <= 192) K = 3;
+        else if (nx <= 256) K = 4;
+        else if (nx <= 384) K = 6;
+        else if (nx <= 512) K = 8;
+        else if (nx <= 768) K = 12;
+        const int nl = (nx + K - 1) / K;
+        const int lpw = 64 / nl;
+        int64_t waves = ((int64_t)ny * nz + lpw - 1) / lpw;
+        if (waves > 2048) waves = 2048;
+        return (waves + 3) / 4 * 4;
+    }
+    const int N = (dir == 2) ? ny : nz;
+    const int nother = (dir == 2) ? nz : ny;
+    const int64_t columns = (int64_t)nx * nother;
+    int S = 1;
+    while (S < 4 && columns * S < 1024 * 64 && N >= 32 * S) S *= 2;
+    const int W = 64 / S;
+    const int wavesPerRow = (nx + W - 1) / W;
+    const int blocksPerRow = (wavesPerRow + 3) / 4;
+    return (int64_t)blocksPerRow * nother * 4;
+}
+
 int launch_fluid_sweep(double *base, int64_t pitch, int nx, int ny, int nz, int dir, int method, double lambda,
-                       double gamma, double rhomin, double ciso, int limiter, hipStream_t s, const double *lamp)
+                       double gamma, double rhomin, double ciso, int limiter, hipStream_t s, const double *lamp,
+                       const CflFuse *cfl)
 {
     if (dir < 1 || dir > 3) return IM_FAIL(ERROR_INVALID_ARGS, "sweep direction %d", dir);
     const int N = (dir == 1) ? nx : (dir == 2 ? ny : nz);
@@ -153,8 +198,10 @@ int launch_fluid_sweep(double *base, int64_t pitch, int nx, int ny, int nz, int 
     hc.emin = 1e-5 / (gamma - 1.0);  // cudaFluidStep.cu:298
     hc.ciso = ciso;
     hc.cisosq = ciso * ciso;
-    if (method == METHOD_HLLC) return launch_method<Hllc>(base, pitch, nx, ny, nz, dir, lambda, lamp, hc, limiter, s);
-    if (method == METHOD_HLL) return launch_method<Hll>(base, pitch, nx, ny, nz, dir, lambda, lamp, hc, limiter, s);
+    const CflFuse cf = cfl ? *cfl : CflFuse{};
+    if (method == METHOD_HLLC)
+        return launch_method<Hllc>(base, pitch, nx, ny, nz, dir, lambda, lamp, hc, limiter, cf, s);
+    if (method == METHOD_HLL) return launch_method<Hll>(base, pitch, nx, ny, nz, dir, lambda, lamp, hc, limiter, cf, s);
     return IM_FAIL(ERROR_INVALID_ARGS, "sweep kernel for method %d", method);
 }
 

@@ -8,6 +8,7 @@
 // always ends where it started (identity), which is what the caller observes.
 #include <cstring>
 
+#include "cfl_device.h"
 #include "imogen_internal.h"
 
 using namespace imogen;
@@ -63,7 +64,7 @@ int sas_code(const MGArray *a, int direction, int side, int *code)
 
 namespace imogen {
 
-int update_1d(MGArray *fluid, const FluidStepParams &params, const double *lampBase)
+int update_1d(MGArray *fluid, const FluidStepParams &params, const double *lampBase, const CflFuse *cfl)
 {
     double *base;
     int64_t pitch;
@@ -83,8 +84,9 @@ int update_1d(MGArray *fluid, const FluidStepParams &params, const double *lampB
     case METHOD_HLL:
         return launch_fluid_sweep(base, pitch, fluid->dim[0], fluid->dim[1], fluid->dim[2], dir,
                                   (int)params.stepMethod, lambda, params.thermoGamma, params.minimumRho,
-                                  params.Cisothermal, params.slopeLimiter, st->stream, lamp);
+                                  params.Cisothermal, params.slopeLimiter, st->stream, lamp, cfl);
     case METHOD_XINJIN:
+        if (cfl) return IM_FAIL(ERROR_NOIMPLEMENT, "the CFL is not folded into Xin-Jin sweeps");
         return launch_xinjin_sweep(base, pitch, fluid->dim[0], fluid->dim[1], fluid->dim[2], dir, lambda,
                                    params.thermoGamma, params.minimumRho, st->stream, lamp);
     default: return IM_FAIL(ERROR_INVALID_ARGS, "unknown fluid method %d", (int)params.stepMethod);
@@ -93,13 +95,63 @@ int update_1d(MGArray *fluid, const FluidStepParams &params, const double *lampB
 
 // flux.cu:27-155 on one rank: the sweep order of (stepNumber, stepDirection),
 // each sweep followed by that direction's boundary conditions.
-int update_3d(MGArray *fluid, const FluidStepParams &fsp, const double *lampBase)
+int last_sweep_dir(const MGArray *fluid, int stepNumber, int stepDirection)
+{
+    if (fluid->dim[1] == 1 && fluid->dim[2] == 1) return 1;
+    int sweep = (stepNumber + 3 * (stepDirection > 0)) % 6;
+    if (sweep < 0) sweep += 6;
+    int last = 0;
+    for (int n = 0; n < 3; n++) {
+        const int d = kFluxcall[n][sweep];
+        if (fluid->dim[d - 1] > 3) last = d;
+    }
+    return last;
+}
+
+int cfl_keep_range(MGArray *fluid, int dir, int *lo, int *hi)
+{
+    const int N = fluid->dim[dir - 1];
+    const int depth = fluid->haloSize > 0 ? fluid->haloSize : 4;
+    *lo = 0;
+    *hi = N;
+    for (int side = 0; side < 2; side++) {
+        const BCModeTypes bt = fluid[0].boundaryConditions.mode[2 * (dir - 1) + side];
+        if (bt == outflow || bt == freebalance) return 1;
+        int code0 = -2;
+        for (int i = 0; i < 5; i++) {
+            int code;
+            if (sas_code(&fluid[i], dir, side, &code)) return 1;
+            if (code == 3) return 1;  // linear extrapolation makes new values
+            if (code0 != -2 && ((code < 0) != (code0 < 0))) return 1;
+            code0 = code;
+        }
+        if (code0 < 0) continue;  // circular: the sweep's values stay
+        if (N < 2 * depth + 2) return 1;
+        if (side == 0) {
+            *lo = depth;  // ghosts [0, depth) copy or mirror cells >= depth
+        } else {
+            // +y/+z const copies the innermost ghost N-depth, which keeps its value
+            const bool keepsInnermost = (dir != 1) && (code0 == 2);
+            *hi = keepsInnermost ? N - depth + 1 : N - depth;
+        }
+    }
+    return 0;
+}
+
+int update_3d(MGArray *fluid, const FluidStepParams &fsp, const double *lampBase, const CflFuse *lastCfl)
 {
     int rc;
     FluidStepParams p = fsp;
+    const int last = lastCfl ? last_sweep_dir(fluid, fsp.stepNumber, fsp.stepDirection) : 0;
+    CflFuse cf;
+    if (lastCfl) {
+        cf = *lastCfl;
+        if (last == 0 || cfl_keep_range(fluid, last, &cf.lo, &cf.hi))
+            return IM_FAIL(ERROR_INVALID_ARGS, "this step's boundaries do not allow a folded CFL");
+    }
     if (fluid->dim[1] == 1 && fluid->dim[2] == 1) {  // flux.cu:57-66
         p.stepDirection = 1;
-        rc = update_1d(fluid, p, lampBase);
+        rc = update_1d(fluid, p, lampBase, lastCfl ? &cf : nullptr);
         if (rc) return rc;
         return setFluidBoundary(fluid, &p.geometry, 1);
     }
@@ -109,7 +161,7 @@ int update_3d(MGArray *fluid, const FluidStepParams &fsp, const double *lampBase
         const int d = kFluxcall[n][sweep];
         if (fluid->dim[d - 1] > 3) {  // flux.cu:102
             p.stepDirection = d;
-            rc = update_1d(fluid, p, lampBase);
+            rc = update_1d(fluid, p, lampBase, (lastCfl && d == last) ? &cf : nullptr);
             if (rc) return rc;
             rc = setFluidBoundary(fluid, &p.geometry, d);
             if (rc) return rc;

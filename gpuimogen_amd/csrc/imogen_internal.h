@@ -9,6 +9,9 @@
 
 namespace imogen {
 
+struct CflFuse;  // cfl_device.h
+
+
 // Records a HIP failure (with where it happened) and returns ERROR_CUDA_BLEW_UP,
 // mirroring CHECK_CUDA_ERROR of the reference (gpuclass/cudaCommon.h:331-337).
 int check_hip(hipError_t e, const char *what, const char *file, int line);
@@ -51,16 +54,28 @@ int fluid_view(MGArray *fluid, double **base, int64_t *pitch);
 
 // performFluidUpdate_1D / _3D without the topology check. lampBase: when not
 // null, the sweeps of direction d read lambda from lampBase[d - 1] (device).
-int update_1d(MGArray *fluid, const FluidStepParams &params, const double *lampBase);
-int update_3d(MGArray *fluid, const FluidStepParams &fsp, const double *lampBase);
+// lastCfl (update_3d): when not null, the last sweep of the step also leaves
+// per-wave CFL minima (its lo/hi are filled in from the boundary modes).
+int update_1d(MGArray *fluid, const FluidStepParams &params, const double *lampBase,
+              const CflFuse *cfl = nullptr);
+int update_3d(MGArray *fluid, const FluidStepParams &fsp, const double *lampBase, const CflFuse *lastCfl = nullptr);
+// Direction swept last by update_3d for (stepNumber, stepDirection), 0 if none.
+int last_sweep_dir(const MGArray *fluid, int stepNumber, int stepDirection);
+// Cells [lo, hi) of direction dir whose swept values survive the direction's
+// boundary conditions, when every other ghost the boundaries write is a copy
+// or mirror image of one of them (modes circular, mirror, const). Returns
+// non-zero when the boundaries do not allow folding the CFL into the sweep.
+int cfl_keep_range(MGArray *fluid, int dir, int *lo, int *hi);
 
 // Launchers (fluid_sweep.hip, xinjin.hip, statics.hip, cfl.hip, drag.hip)
 // method: METHOD_HLLC or METHOD_HLL. rhomin, ciso: HLL floors and isothermal
 // sound speed (-1: adiabatic). lamp: when not null, the kernels read lambda
 // = dt/h from this device address instead of the `lambda` argument.
+// cfl: when not null, the sweep also writes the per-wave CFL minima of the
+// cells it stores (cfl_device.h).
 int launch_fluid_sweep(double *base, int64_t pitch, int nx, int ny, int nz, int dir, int method, double lambda,
                        double gamma, double rhomin, double ciso, int limiter, hipStream_t s,
-                       const double *lamp = nullptr);
+                       const double *lamp = nullptr, const CflFuse *cfl = nullptr);
 int launch_xinjin_sweep(double *base, int64_t pitch, int nx, int ny, int nz, int dir, double lambda,
                         double gamma, double rhomin, hipStream_t s, const double *lamp = nullptr);
 int launch_fluid_bc(double *base, int64_t pitch, int nx, int ny, int nz, int dir, const int *modes10,
@@ -70,5 +85,11 @@ int launch_soundspeed(const double *base, int64_t pitch, int64_t n, double gamma
 int launch_cfl(const double *base, int64_t pitch, int64_t n, const double *cs, double gamma, int fused,
                int dim, int xinjin, int halve, double hx, double hy, double hz, double hmin, double *devOut,
                hipStream_t s);
+// devOut[0] = min(part[0..n)) (x 1/2 when halve): the second pass over
+// per-wave minima a fused sweep left
+int launch_cfl_final(const double *part, int64_t n, int halve, double *devOut, hipStream_t s);
+// Wave count of a sweep launch over an nx x ny x nz HLLC/HLL array in
+// direction dir (the entries a fused CFL needs).
+int64_t sweep_waves(int nx, int ny, int nz, int dir);
 
 }  // namespace imogen

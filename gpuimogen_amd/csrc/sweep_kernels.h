@@ -2,6 +2,7 @@
 // single-array sweeps (fluid_sweep.hip) and the batched ensemble sweeps
 // (ensemble.hip). See fluid_sweep.hip for the data-movement design.
 #pragma once
+#include "cfl_device.h"
 #include "sweep_device.h"
 
 namespace imogen {
@@ -64,6 +65,18 @@ __device__ __forceinline__ void store_cell(double *__restrict__ p, int64_t pitch
     p[M::t2 * pitch] = v.e;
 }
 
+// time-step limit of a stored cell (physical x, y, z momentum order, as the
+// CFL kernel reads the slabs)
+template <class Meth, int DIR>
+__device__ __forceinline__ double cfl_of_cell(const CflArgs &a, const V5 &o)
+{
+    using M = MomSlabs<Meth, DIR>;
+    const double px = (M::n == 2) ? o.c : ((M::t1 == 2) ? o.d : o.e);
+    const double py = (M::n == 3) ? o.c : ((M::t1 == 3) ? o.d : o.e);
+    const double pz = (M::n == 4) ? o.c : ((M::t1 == 4) ? o.d : o.e);
+    return cfl_cell(a, o.a, o.b, px, py, pz, 0.0);
+}
+
 // ---------------------------------------------------------------------------
 // X sweep: lanes own K consecutive cells of a line
 // ---------------------------------------------------------------------------
@@ -118,10 +131,10 @@ __device__ __forceinline__ void x_fluxes(const HydroConsts &hc, int rt, V5 (&U)[
 // operations are unconditional (the compiler can then count them exactly and
 // wait only for the loads it needs); lanes past the last line read a clamped
 // valid line and write into a private dummy slot.
-template <class M, int K, int LIM, bool FULL>
+template <class M, int K, int LIM, bool FULL, bool CFL>
 __global__ __launch_bounds__(256, (K <= 4) ? 2 : 1) void sweep_x(double *__restrict__ q, int64_t pitch, int N, int64_t nlines,
                                                int nl, double lambda, const double *__restrict__ lamp,
-                                               HydroConsts hc, int rt, double *__restrict__ dummy)
+                                               HydroConsts hc, int rt, double *__restrict__ dummy, CflFuse cf)
 {
     if (lamp) lambda = lamp[0];  // time step kept on the device
     const int lane = threadIdx.x & 63;
@@ -159,7 +172,11 @@ __global__ __launch_bounds__(256, (K <= 4) ? 2 : 1) void sweep_x(double *__restr
 
     int64_t line = wave * lpw + (laneUsed ? sub : 0);
     int64_t gbase = wave * lpw;  // first line of this wave's group (uniform)
-    if (gbase >= nlines) return;
+    if (gbase >= nlines) {
+        if (CFL && lane == 0) cf.part[wave] = 1e38;
+        return;
+    }
+    double cmin = 1e38;
     V5 Q[K], Qn[K];
     load_group(line, Q);
 
@@ -184,6 +201,10 @@ __global__ __launch_bounds__(256, (K <= 4) ? 2 : 1) void sweep_x(double *__restr
             if (k < cnt) {
                 const V5 o = avg_update(add5(U1[k], Q[k]), lambda, F[k], k == 0 ? Fm : F[k > 0 ? k - 1 : 0]);
                 store_cell<M, 1>(own ? dst + k : dst, st, o);
+                if (CFL && own && x0 + k >= cf.lo && x0 + k < cf.hi) {
+                    const double u = cfl_of_cell<M, 1>(cf.a, o);
+                    cmin = (u < cmin) ? u : cmin;
+                }
             }
         }
         if (!more) break;
@@ -191,6 +212,10 @@ __global__ __launch_bounds__(256, (K <= 4) ? 2 : 1) void sweep_x(double *__restr
         for (int k = 0; k < K; k++) Q[k] = Qn[k];
         line += groupStride;
         gbase += groupStride;
+    }
+    if (CFL) {
+        cmin = wave_min(cmin);
+        if (lane == 0) cf.part[wave] = cmin;
     }
 }
 
@@ -201,11 +226,11 @@ __global__ __launch_bounds__(256, (K <= 4) ? 2 : 1) void sweep_x(double *__restr
 // walks cells [j0, j0+L) of the column starting at `col` (stride sstride, N
 // cells, periodic), the wave's stencil ring in LDS at `ring`. defer/gtid/nthr
 // address this lane's deferred-output slots. seglen is uniform over the wave.
-template <class M, int DIR, int LIM>
+template <class M, int DIR, int LIM, bool CFL = false>
 __device__ __forceinline__ void march_body(double *__restrict__ col, int64_t pitch, int64_t sstride, int N,
                                            int seglen, int j0, int L, double lambda, const HydroConsts &hc,
                                            int rt, double *__restrict__ defer, int64_t gtid, int64_t nthr,
-                                           double (*ring)[5][64], int lane)
+                                           double (*ring)[5][64], int lane, const CflFuse &cf, double &cmin)
 {
     // wrapped sweep index of the first load (m = j0 - 4)
     int mw = j0 - 4;
@@ -274,6 +299,10 @@ __device__ __forceinline__ void march_body(double *__restrict__ col, int64_t pit
         // outp walks the column one cell per step (it points at output oi);
         // the deferred slots are wave-uniform offsets of this lane's block.
         const int oi = t - 9;  // output index within the segment
+        if (CFL && oi >= 0 && oi < L && j0 + oi >= cf.lo && j0 + oi < cf.hi) {
+            const double u = cfl_of_cell<M, DIR>(cf.a, o);
+            cmin = (u < cmin) ? u : cmin;
+        }
         const bool direct = (oi >= 4) & (oi < L);
         const int slot = ((oi >= 0) & (oi < 4) & (oi < L)) ? oi : 4;
         double *dq = dfr + (int64_t)slot * dstride;
@@ -320,20 +349,24 @@ __device__ __forceinline__ void march_body(double *__restrict__ col, int64_t pit
     }
 }
 
-template <class M, int DIR, int LIM>
+template <class M, int DIR, int LIM, bool CFL>
 __global__ __launch_bounds__(256) void sweep_march(double *__restrict__ q, int64_t pitch, int nx, int64_t sstride,
                                                    int64_t ostride, int N, int S, int seglen, int wavesPerRow,
                                                    double lambda, const double *__restrict__ lamp,
-                                                   HydroConsts hc, int rt, double *__restrict__ defer)
+                                                   HydroConsts hc, int rt, double *__restrict__ defer, CflFuse cf)
 {
     if (lamp) lambda = lamp[0];  // time step kept on the device
     const int lane = threadIdx.x & 63;
     const int wib = threadIdx.x >> 6;
+    const int64_t waveId = ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * (blockDim.x >> 6) + wib;
     const int W = 64 / S;
     const int s = lane / W;
     const int c = lane - s * W;
     const int64_t wrow = (int64_t)blockIdx.x * (blockDim.x >> 6) + wib;  // wave index within this row
-    if (wrow >= wavesPerRow) return;
+    if (wrow >= wavesPerRow) {
+        if (CFL && lane == 0) cf.part[waveId] = 1e38;
+        return;
+    }
     const int64_t x = wrow * W + c;
     const bool colValid = (x < nx) && (s < S);
     const int j0 = s * seglen;
@@ -345,8 +378,13 @@ __global__ __launch_bounds__(256) void sweep_march(double *__restrict__ q, int64
     const int64_t gtid = ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x;
     const int64_t nthr = (int64_t)gridDim.x * gridDim.y * blockDim.x;
     __shared__ double sring[4][4][5][64];  // [wave in block][slot][variable][lane]
-    march_body<M, DIR, LIM>(col, pitch, sstride, N, seglen, j0, L, lambda, hc, rt, defer, gtid, nthr, sring[wib],
-                            lane);
+    double cmin = 1e38;
+    march_body<M, DIR, LIM, CFL>(col, pitch, sstride, N, seglen, j0, L, lambda, hc, rt, defer, gtid, nthr,
+                                 sring[wib], lane, cf, cmin);
+    if (CFL) {
+        cmin = wave_min(cmin);
+        if (lane == 0) cf.part[waveId] = cmin;
+    }
 }
 
 }  // namespace imogen

@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "cfl_device.h"
 #include "imogen_internal.h"
 
 namespace imogen {
@@ -56,6 +57,10 @@ struct TimeloopImpl {
     hipGraphExec_t graph[6] = {nullptr};
     void *scratchAt = nullptr;  // scratch block the graphs were captured with
     size_t scratchBytes = 0;
+    // CFL folded into the last sweep of each iteration: per fluid, whether its
+    // method and boundaries allow it, the per-wave minima and their capacity
+    bool fuse[kMaxFluids] = {false};
+    CflFuse cf[kMaxFluids];
 };
 
 }  // namespace imogen
@@ -100,11 +105,36 @@ static int cfl_of(TimeloopImpl *L, int f, hipStream_t s)
     return rc;
 }
 
-static int iteration(TimeloopImpl *L, int it, hipStream_t s)
+// The CFL args launch_cfl would use for fluid f (cfl_common in fluidstep.hip)
+static CflArgs cfl_args(const TimeloopImpl *L, int f)
+{
+    const GeometryParams &g = L->geo;
+    int spacedim = 0;
+    if (g.globalRez[1] > 1) spacedim = 1;
+    if (g.globalRez[2] > 1) spacedim = 2;
+    const int dim = spacedim + 1;
+    double hmin = g.h[0];
+    if (dim > 1 && g.h[1] < hmin) hmin = g.h[1];
+    if (dim > 2 && g.h[2] < hmin) hmin = g.h[2];
+    const double gamma = L->cflGamma[f];
+    return CflArgs{gamma * (gamma - 1), g.h[0], g.h[1], g.h[2], hmin, 1, dim, 0};
+}
+
+// fusedCfl: take this iteration's CFL from the minima the previous iteration's
+// last sweep left (fluids that allow it); otherwise a full CFL pass.
+static int iteration(TimeloopImpl *L, int it, bool fusedCfl, hipStream_t s)
 {
     int rc;
-    for (int f = 0; f < L->nf; f++)
-        if ((rc = cfl_of(L, f, s))) return rc;
+    for (int f = 0; f < L->nf; f++) {
+        if (fusedCfl && L->fuse[f]) {
+            const MGArray *a = L->fluid[f];
+            const int64_t n = sweep_waves(a->dim[0], a->dim[1], a->dim[2], last_sweep_dir(a, it - 1, -1));
+            rc = launch_cfl_final(L->cf[f].part, n, 1, L->dbuf + kClockDoubles + (int64_t)f * kCflSlot, s);
+        } else {
+            rc = cfl_of(L, f, s);
+        }
+        if (rc) return rc;
+    }
     hipLaunchKernelGGL(clock_advance, dim3(1), dim3(64), 0, s, L->dbuf + kClockDoubles, L->nf, L->cfl, L->tmax,
                        L->geo.h[0], L->geo.h[1], L->geo.h[2], L->dbuf);
     if ((rc = IM_LAUNCHED("clock_advance"))) return rc;
@@ -114,7 +144,8 @@ static int iteration(TimeloopImpl *L, int it, hipStream_t s)
             p.stepNumber = it;
             p.stepDirection = sgn;
             p.dt = 0.0;  // unused: lambda comes from the device clock
-            if ((rc = update_3d(L->fluid[f], p, L->dbuf + 2))) return rc;
+            const CflFuse *cf = (sgn < 0 && L->fuse[f]) ? &L->cf[f] : nullptr;
+            if ((rc = update_3d(L->fluid[f], p, L->dbuf + 2, cf))) return rc;
         }
     return SUCCESSFUL;
 }
@@ -159,9 +190,35 @@ int imogen_timeloop_create(MGArray *fluids, int nfluids, const GeometryParams *g
     L->geo = *geo;
     L->cfl = cfl;
     L->tmax = (tmax > 0) ? tmax : HUGE_VAL;
+    for (int f = 0; f < nfluids; f++) {
+        const int m = (int)L->p[f].stepMethod;
+        MGArray *a = L->fluid[f];
+        bool ok = (m == METHOD_HLLC || m == METHOD_HLL);
+        int64_t cap = 0;
+        for (int d = 1; d <= 3 && ok; d++) {
+            const bool swept = (a->dim[1] == 1 && a->dim[2] == 1) ? (d == 1) : (a->dim[d - 1] > 3);
+            if (!swept) continue;
+            int lo, hi;
+            if (a->dim[d - 1] <= 3 || cfl_keep_range(a, d, &lo, &hi)) ok = false;
+            const int64_t w = sweep_waves(a->dim[0], a->dim[1], a->dim[2], d);
+            cap = (w > cap) ? w : cap;
+        }
+        if (!ok || cap == 0) continue;
+        CflFuse &c = L->cf[f];
+        c = CflFuse{};
+        if (IM_HIP(hipMalloc(&c.part, (size_t)cap * sizeof(double)))) {
+            c.part = nullptr;
+            imogen_timeloop_destroy(T);
+            return ERROR_NOMEM;
+        }
+        c.a = cfl_args(L, f);
+        c.cap = cap;
+        L->fuse[f] = true;
+    }
     const size_t bytes = sizeof(double) * (kClockDoubles + (size_t)nfluids * kCflSlot);
     if (IM_HIP(hipMalloc(&L->dbuf, bytes))) {
-        delete T;
+        L->dbuf = nullptr;
+        imogen_timeloop_destroy(T);
         return ERROR_NOMEM;
     }
     // start at t = 0
@@ -169,8 +226,7 @@ int imogen_timeloop_create(MGArray *fluids, int nfluids, const GeometryParams *g
     int rc = IM_HIP(hipMemcpyAsync(L->dbuf, zeros, sizeof(zeros), hipMemcpyHostToDevice, st->stream));
     if (!rc) rc = IM_HIP(hipStreamSynchronize(st->stream));
     if (rc) {
-        (void)hipFree(L->dbuf);
-        delete T;
+        imogen_timeloop_destroy(T);
         return rc;
     }
     *out = T;
@@ -190,8 +246,11 @@ int imogen_timeloop_run(ImogenTimeloop *T, int firstIteration, int iterations, i
     if (!st) return ERROR_CUDA_BLEW_UP;
     for (int k = 0; k < iterations; k++) {
         const int it = firstIteration + k;
-        if (!useGraph) {
-            int rc = iteration(L, it, st->stream);
+        // The first iteration of a call takes a full CFL pass (the state may
+        // have changed since the last call); later ones read the minima of
+        // the previous iteration's last sweep.
+        if (!useGraph || k == 0) {
+            int rc = iteration(L, it, k > 0, st->stream);
             if (rc) return rc;
             continue;
         }
@@ -204,7 +263,7 @@ int imogen_timeloop_run(ImogenTimeloop *T, int firstIteration, int iterations, i
         if (!g) {
             // eager run of this iteration: sizes the scratch block (no allocation
             // may happen while capturing) and advances the state as the replay would
-            int rc = iteration(L, it, st->stream);
+            int rc = iteration(L, it, true, st->stream);
             if (rc) return rc;
             if (L->scratchAt != st->scratch || L->scratchBytes != st->scratchBytes) {
                 drop_graphs(L);
@@ -218,7 +277,7 @@ int imogen_timeloop_run(ImogenTimeloop *T, int firstIteration, int iterations, i
                 prof_suspend(0);
                 return ERROR_CUDA_BLEW_UP;
             }
-            rc = iteration(L, it, st->stream);
+            rc = iteration(L, it, true, st->stream);
             hipError_t ec = hipStreamEndCapture(st->stream, &graph);
             prof_suspend(0);
             if (rc) return rc;
@@ -264,6 +323,8 @@ int imogen_timeloop_destroy(ImogenTimeloop *T)
     if (st) (void)hipStreamSynchronize(st->stream);
     drop_graphs(&T->impl);
     if (T->impl.dbuf) (void)hipFree(T->impl.dbuf);
+    for (int f = 0; f < T->impl.nf; f++)
+        if (T->impl.cf[f].part) (void)hipFree(T->impl.cf[f].part);
     delete T;
     return SUCCESSFUL;
 }

@@ -2,7 +2,11 @@
 iteration it replaces (imogen.Simulation: imogen_cflTimestepFused per fluid,
 dTime = CFL * min tau, performFluidUpdate_3D(+1), (-1) per fluid), which the
 other GPU tests pin to the oracle. Same kernels, same dTime arithmetic: the
-states and clocks must agree bitwise, eager or replayed from graphs."""
+states and clocks must agree bitwise, eager or replayed from graphs. The loop
+folds the CFL minimum into each iteration's last sweep where the boundaries
+allow it (const with the +y/+z innermost-ghost convention, mirror, circular;
+HLLC and HLL) and takes a full CFL pass otherwise (Xin-Jin) and on the first
+iteration of every run call; the cases cover each."""
 import numpy as np
 import pytest
 
@@ -20,6 +24,14 @@ def make_fluids(kind):
         fl = [gc.FluidManager(gas, 5 / 3, minmass=1e-8, bc_modes=(abi.circular,) * 6),
               gc.FluidManager(dust, 1.01, minmass=1e-8, bc_modes=(abi.circular,) * 6, cisothermal=ciso)]
         return fl, [abi.METHOD_HLLC, abi.METHOD_HLL], gc.GeometryManager((24, 24, 24), h), abi.LIMITER_OSPRE
+    if kind == "mirror2d":
+        from gpuimogen_amd import ensemble as ens
+        _, st = ens.quadrant_problems(1, seed=13, nmin=40, nmax=40)
+        q = np.zeros((5, 1, 40, 56))
+        q[:, :, :, :40] = st[0].reshape(5, 1, 40, 40)
+        q[:, :, :, 40:] = st[0].reshape(5, 1, 40, 40)[:, :, :, :16]
+        fl = [gc.FluidManager(q, 1.4, minmass=1e-8, bc_modes=(abi.mirror,) * 6)]
+        return fl, [abi.METHOD_HLL], gc.GeometryManager((56, 40, 1), (1 / 56, 1 / 40, 1.0)), abi.LIMITER_VANLEER
     if kind == "xinjin2d":
         from gpuimogen_amd import ensemble as ens
         _, st = ens.quadrant_problems(1, seed=11, nmin=48, nmax=48)
@@ -28,7 +40,7 @@ def make_fluids(kind):
     raise ValueError(kind)
 
 
-@pytest.mark.parametrize("kind", ["sedov", "twofluid", "xinjin2d"])
+@pytest.mark.parametrize("kind", ["sedov", "twofluid", "mirror2d", "xinjin2d"])
 @pytest.mark.parametrize("graph", [False, True])
 def test_device_loop_matches_host_loop(kind, graph):
     from gpuimogen_amd import imogen
