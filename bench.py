@@ -176,20 +176,35 @@ def run_gasdust(args):
 def run_ensemble(args):
     """The batched ensemble: `count` independent 2D four-quadrant HLLC problems
     with log-uniform sizes, per-problem CFL 0.4 on the device, one launch per
-    sweep direction over all problems, each iteration a replayed hipGraph. The
-    march segment counts are autotuned on the ensemble itself, then the
-    initial states are reloaded."""
-    from gpuimogen_amd import ensemble as ens, gpuclass as gc
+    sweep direction over all problems, each iteration a replayed hipGraph; the
+    odd problems carry dust (dust-to-gas 0.5) coupled by the learned drag with
+    exact fallback. The march segment counts are autotuned on the ensemble
+    itself, then the initial states are reloaded. cell_updates_per_s counts
+    the problems' cells (2 per cell per iteration)."""
+    from gpuimogen_amd import abi, ensemble as ens, gpuclass as gc, learned_drag as L
 
     sizes, states = ens.quadrant_problems(args.ensemble_count, seed=args.seed)
+    dust = {p: ens.quadrant_dust(states[p], seed=args.seed + 7919 * (p + 1))
+            for p in range(len(sizes)) if p % 2 == 1 and not args.ensemble_no_dust}
     e = ens.Ensemble(sizes, sizes, 1.0 / sizes, 1.0 / sizes)
-    for p, s in enumerate(states):
-        e.set_state(p, s)
+    if dust:
+        params, norm, _ = L.load_model()
+        L.set_model(params, norm)
+        tg, td = L.cold_h2(), L.iron_dust()
+        tg.gamma = 1.4  # the quadrant problems' gas
+        e.set_dust([p in dust for p in range(len(sizes))], tg, td, 1.01, abi.DRAG_LEARNED)
+
+    def load():
+        for p, s in enumerate(states):
+            e.set_state(p, s)
+        for p, s in dust.items():
+            e.set_dust_state(p, s)
+
+    load()
     t0 = time.perf_counter()
     chosen = e.autotune(trials=2)
     tune_s = time.perf_counter() - t0
-    for p, s in enumerate(states):
-        e.set_state(p, s)
+    load()
     e.iteration = 0
     e.run(args.warmup, graph=True)
     gc.synchronize()
@@ -206,6 +221,9 @@ def run_ensemble(args):
     cells = int(np.sum(sizes.astype(np.int64) ** 2))
     times = e.times()
     ok = all(bool(np.all(np.isfinite(e.state(p)))) for p in (0, len(sizes) // 2, len(sizes) - 1))
+    if dust:
+        ok = ok and all(bool(np.all(np.isfinite(e.dust_state(p)))) for p in list(dust)[:3])
+    dust_cells = int(sum(int(sizes[p]) ** 2 for p in dust))
     e.free()
     cu = cells * 2 * args.ensemble_steps
     return {"workload": f"ensemble2d_quadrant_{args.ensemble_count}", "problems": len(sizes),
@@ -215,7 +233,10 @@ def run_ensemble(args):
             "sweep_bytes_GBs": cells * 4 * BYTES_PER_CELL_SWEEP * args.ensemble_steps / el / 1e9,
             "segments_by_ny_class": {"<=128": chosen[0], "<=256": chosen[1], "<=512": chosen[2]},
             "autotune_s": tune_s, "sim_time_range": [float(times.min()), float(times.max())],
-            "finite": ok, "dust": "not carried"}
+            "finite": ok,
+            "dust": ({"problems": len(dust), "dust_to_gas": 0.5, "cells": dust_cells, "drag": "learned + exact fallback",
+                      "fluid_cell_updates_per_s": (cells + dust_cells) * 2 * args.ensemble_steps / el}
+                     if dust else None)}
 
 
 def run_cpu_baseline(args, q0, h):
@@ -291,6 +312,7 @@ def main():
     ap.add_argument("--ensemble-count", type=int, default=512)
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-no-dust", action="store_true", help="ensemble without the dusty half")
     ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args()
     if args.copy_probe:

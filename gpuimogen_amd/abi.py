@@ -156,6 +156,9 @@ SIGNATURES = {
     "imogen_ensemble_times": (c_int, [c_void_p, _DP]),
     "imogen_ensemble_autotune": (c_int, [c_void_p, c_int, _IP]),
     "imogen_ensemble_destroy": (c_int, [c_void_p]),
+    "imogen_ensemble_set_dust": (c_int, [c_void_p, _IP, c_double, _P(ThermoDetails), _P(ThermoDetails), c_int]),
+    "imogen_ensemble_set_dust_state": (c_int, [c_void_p, c_int, _DP]),
+    "imogen_ensemble_get_dust_state": (c_int, [c_void_p, c_int, _DP]),
     "imogen_timeloop_create": (c_int, [_MGP, c_int, _P(GeometryParams), _P(FluidStepParams), _DP, c_double,
                                        c_double, _P(c_void_p)]),
     "imogen_timeloop_run": (c_int, [c_void_p, c_int, c_int, c_int]),

@@ -269,4 +269,23 @@ int launch_drag(double *g, double *d, int64_t pitch, int64_t n, const DragConsts
 int launch_learned_drag(double *g, double *d, int64_t pitch, int64_t n, const DragConsts &c, double dt,
                         hipStream_t s);
 
+// Drag over many gas/dust array pairs (the dusty problems of an ensemble):
+// table entry t = (gas, dust, slab pitch, cells, problem index for dtp), items
+// = 64-cell chunks (t, first cell). Each cell gets dt = frac * dtp[problem].
+// Methods 4, 5, 6 and 7 (learned, with the queue of rejected cells integrated
+// exactly); queue needs one entry per cell of all tables.
+struct EnsDragTable {
+    double *g, *d;
+    int64_t pitch, n;
+    int p, pad;
+};
+struct EnsDragItem {
+    int t, pad;
+    int64_t base;
+};
+int launch_ens_drag(const EnsDragTable *tab, const EnsDragItem *items, int nitems, const double *dtp, double frac,
+                    const DragConsts &c, int method, int64_t *queue, unsigned long long *qcount, hipStream_t s);
+// changes whenever imogen_drag_set_model loads a model (captured graphs hold it by value)
+uint64_t learned_model_generation();
+
 }  // namespace imogen

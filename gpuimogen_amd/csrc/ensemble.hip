@@ -23,16 +23,22 @@
 #include <vector>
 
 #include "cfl_device.h"
+#include "drag_device.h"
 #include "imogen_internal.h"
 #include "sweep_kernels.h"
 
 namespace imogen {
 
+// One fluid array of one problem (a table entry). Entries 0..P-1 are the
+// problems' gas; dusty problems add an entry for their dust.
 struct EnsProblem {
     double *q;
     int64_t pitch;  // doubles between slabs
     int nx, ny;
     double hx, hy;
+    double gamma;
+    int owner;  // problem whose clock and time step this entry follows
+    int dust;   // gas entries: the entry of the problem's dust, -1 if none
 };
 
 struct EnsItem {
@@ -59,12 +65,12 @@ static int x_class_of(int N)
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void ens_cfl_pass1(const EnsProblem *__restrict__ probs, double gg1,
+__global__ __launch_bounds__(256) void ens_cfl_pass1(const EnsProblem *__restrict__ probs,
                                                      double *__restrict__ part)
 {
     const EnsProblem P = probs[blockIdx.y];
     const int64_t n = (int64_t)P.nx * P.ny;
-    CflArgs a{gg1, P.hx, P.hy, 1.0, P.hx < P.hy ? P.hx : P.hy, 1, 2, 0};
+    CflArgs a{P.gamma * (P.gamma - 1), P.hx, P.hy, 1.0, P.hx < P.hy ? P.hx : P.hy, 1, 2, 0};
     double m = 1e37;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const double *c = P.q + i;
@@ -86,25 +92,54 @@ __global__ __launch_bounds__(256) void ens_cfl_pass1(const EnsProblem *__restric
 // lambda per direction, and the problem's clock t += 2 dt
 __global__ __launch_bounds__(64) void ens_cfl_pass2(const EnsProblem *__restrict__ probs, const double *part,
                                                     int nb, double cfl, double *__restrict__ lam,
-                                                    double *__restrict__ clock)
+                                                    double *__restrict__ clock, double *__restrict__ dtp)
 {
     const int p = blockIdx.x;
+    const int de = probs[p].dust;
+    // min over fluids of each fluid's halved limit, as TimeManager takes it
     double m = 1e38;
     for (int k = threadIdx.x; k < nb; k += 64) m = (part[(int64_t)p * nb + k] < m) ? part[(int64_t)p * nb + k] : m;
     m = wave_min(m);
+    double md = 1e38;
+    if (de >= 0) {
+        for (int k = threadIdx.x; k < nb; k += 64)
+            md = (part[(int64_t)de * nb + k] < md) ? part[(int64_t)de * nb + k] : md;
+        md = wave_min(md);
+    }
     if (threadIdx.x == 0) {
-        const double dt = cfl * (m / 2);
+        const double tg = m / 2, td = md / 2;
+        const double tau = (de >= 0 && td < tg) ? td : tg;
+        const double dt = cfl * tau;
         lam[2 * p] = dt / probs[p].hx;
         lam[2 * p + 1] = dt / probs[p].hy;
+        if (de >= 0) {
+            lam[2 * de] = lam[2 * p];
+            lam[2 * de + 1] = lam[2 * p + 1];
+        }
+        dtp[p] = dt;
         clock[p] += 2 * dt;
     }
+}
+
+// the HLLC constants of an entry (launch_fluid_sweep's, from the entry's gamma)
+__device__ __forceinline__ HydroConsts ens_consts(double gamma)
+{
+    HydroConsts hc;
+    hc.gamma = gamma;
+    hc.gm1 = gamma - 1.0;
+    hc.govergm1 = gamma / (gamma - 1.0);
+    hc.rhomin = 0.0;
+    hc.emin = 1e-5 / (gamma - 1.0);
+    hc.ciso = -1.0;
+    hc.cisosq = 1.0;
+    return hc;
 }
 
 // X sweep of whole lines, K cells per lane; one work item (problem, first line) per wave
 template <int K, int LIM>
 __global__ __launch_bounds__(256) void ens_sweep_x(const EnsProblem *__restrict__ probs,
                                                    const EnsItem *__restrict__ work, int nwork,
-                                                   const double *__restrict__ lam, HydroConsts hc, int rt,
+                                                   const double *__restrict__ lam, int rt,
                                                    double *__restrict__ dummy)
 {
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -112,6 +147,7 @@ __global__ __launch_bounds__(256) void ens_sweep_x(const EnsProblem *__restrict_
     const int lane = threadIdx.x & 63;
     const EnsItem it = work[wave];
     const EnsProblem P = probs[it.p];
+    const HydroConsts hc = ens_consts(P.gamma);
     const double lambda = lam[2 * it.p];
     const int N = P.nx;
     const int nl = (N + K - 1) / K;
@@ -156,7 +192,7 @@ __global__ __launch_bounds__(256) void ens_sweep_x(const EnsProblem *__restrict_
 template <int LIM>
 __global__ __launch_bounds__(256) void ens_sweep_y(const EnsProblem *__restrict__ probs,
                                                    const EnsItem *__restrict__ work, int nwork,
-                                                   const double *__restrict__ lam, HydroConsts hc, int rt,
+                                                   const double *__restrict__ lam, int rt,
                                                    double *__restrict__ defer)
 {
     __shared__ double sring[4][4][5][64];
@@ -166,6 +202,7 @@ __global__ __launch_bounds__(256) void ens_sweep_y(const EnsProblem *__restrict_
     const int lane = threadIdx.x & 63;
     const EnsItem it = work[wave];
     const EnsProblem P = probs[it.p];
+    const HydroConsts hc = ens_consts(P.gamma);
     const double lambda = lam[2 * it.p + 1];
     const int S = it.b;
     const int W = 64 / S;
@@ -233,7 +270,8 @@ __global__ __launch_bounds__(256) void ens_bc_const(const EnsProblem *__restrict
 // host side
 // ---------------------------------------------------------------------------
 struct EnsembleImpl {
-    int P = 0;
+    int P = 0;   // problems (gas entries 0..P-1)
+    int NE = 0;  // table entries: gas, then the dusty problems' dust
     double gamma = 1.4, cfl = 0.4;
     int limiter = 0;
     std::vector<EnsProblem> host;
@@ -249,6 +287,17 @@ struct EnsembleImpl {
     int64_t xThreads = 0, yThreads = 0;
     hipGraphExec_t graph[6] = {nullptr};
     int64_t cells = 0;
+    // dust (imogen_ensemble_set_dust)
+    int Pd = 0;
+    double *dpool = nullptr, *dtp = nullptr;
+    EnsDragTable *dragTab = nullptr;
+    EnsDragItem *dragItems = nullptr;
+    int nDragItems = 0;
+    int64_t *queue = nullptr;
+    unsigned long long *qcount = nullptr;
+    DragConsts dc{};
+    int dragMethod = DRAG_LEARNED;
+    uint64_t modelGen = 0;
 };
 
 static void free_graphs(EnsembleImpl *e)
@@ -266,14 +315,14 @@ static int build_work(EnsembleImpl *e)
     // the tables below are replaced while no kernel can still read them
     if (IM_HIP(hipStreamSynchronize(st->stream))) return ERROR_CUDA_BLEW_UP;
     std::vector<EnsItem> xw[6], yw;
-    for (int p = 0; p < e->P; p++) {
+    for (int p = 0; p < e->NE; p++) {
         const EnsProblem &P = e->host[p];
         const int cls = x_class_of(P.nx);
         const int K = kXClassK[cls];
         const int nl = (P.nx + K - 1) / K;
         const int lpw = 64 / nl;
         for (int l = 0; l < P.ny; l += lpw) xw[cls].push_back(EnsItem{p, l, 0, 0});
-        const int S = e->segS[p];
+        const int S = e->segS[e->host[p].owner];
         const int W = 64 / S;
         for (int x = 0; x < P.nx; x += W) yw.push_back(EnsItem{p, x, S, 0});
     }
@@ -309,8 +358,31 @@ static int build_work(EnsembleImpl *e)
     return SUCCESSFUL;
 }
 
+static int ens_bc(EnsembleImpl *e, int dir, hipStream_t s)
+{
+    hipLaunchKernelGGL(ens_bc_const, dim3((unsigned)e->NE), dim3(256), 0, s, e->dprobs, dir, 4);
+    return IM_LAUNCHED("ens_bc_const");
+}
+
+// sources/src0001.m for the dusty problems: two drag calls over dTime*fraction/2
+// each, then every fluid's boundaries (idempotent for the problems without dust)
+static int ens_source(EnsembleImpl *e, double fraction, hipStream_t s)
+{
+    if (!e->Pd) return SUCCESSFUL;
+    for (int call = 0; call < 2; call++) {
+        int rc = launch_ens_drag(e->dragTab, e->dragItems, e->nDragItems, e->dtp, fraction / 2, e->dc, e->dragMethod,
+                                 e->queue, e->qcount, s);
+        if (rc) return rc;
+    }
+    for (int d = 1; d <= 2; d++) {
+        int rc = ens_bc(e, d, s);
+        if (rc) return rc;
+    }
+    return SUCCESSFUL;
+}
+
 template <int LIM>
-static int ens_sweep(EnsembleImpl *e, int dir, const HydroConsts &hc, int rt, hipStream_t s)
+static int ens_sweep(EnsembleImpl *e, int dir, int rt, hipStream_t s)
 {
     if (dir == 1) {
         for (int c = 0; c < 6; c++) {
@@ -318,7 +390,7 @@ static int ens_sweep(EnsembleImpl *e, int dir, const HydroConsts &hc, int rt, hi
             const unsigned blocks = (unsigned)((e->nx_work[c] + 3) / 4);
 #define IM_EX(KK)                                                                                              \
     hipLaunchKernelGGL((ens_sweep_x<KK, LIM>), dim3(blocks), dim3(256), 0, s, e->dprobs, e->xwork[c],           \
-                       e->nx_work[c], e->lam, hc, rt, e->xdummy)
+                       e->nx_work[c], e->lam, rt, e->xdummy)
             switch (kXClassK[c]) {
             case 1: IM_EX(1); break;
             case 2: IM_EX(2); break;
@@ -334,46 +406,40 @@ static int ens_sweep(EnsembleImpl *e, int dir, const HydroConsts &hc, int rt, hi
     } else {
         const unsigned blocks = (unsigned)((e->ny_work + 3) / 4);
         hipLaunchKernelGGL((ens_sweep_y<LIM>), dim3(blocks), dim3(256), 0, s, e->dprobs, e->ywork, e->ny_work,
-                           e->lam, hc, rt, e->ydefer);
+                           e->lam, rt, e->ydefer);
         int rc = IM_LAUNCHED("ens_sweep_y");
         if (rc) return rc;
     }
-    hipLaunchKernelGGL(ens_bc_const, dim3((unsigned)e->P), dim3(256), 0, s, e->dprobs, dir, 4);
-    return IM_LAUNCHED("ens_bc_const");
+    return ens_bc(e, dir, s);
 }
 
-// one iteration: CFL, fluidstep(+1), fluidstep(-1)
+// one iteration (imogen.m:115-139): CFL over all fluids, source(1/2),
+// fluidstep(+1), source(1), fluidstep(-1), source(1/2); the sources act only
+// when some problems carry dust
 static int ens_iteration(EnsembleImpl *e, int it, hipStream_t s)
 {
     static const int kFluxcall[3][6] = {{1, 2, 1, 2, 3, 3}, {3, 1, 2, 3, 1, 2}, {2, 3, 3, 1, 2, 1}};
-    HydroConsts hc;
-    hc.gamma = e->gamma;
-    hc.gm1 = e->gamma - 1.0;
-    hc.govergm1 = e->gamma / (e->gamma - 1.0);
-    hc.rhomin = 0.0;
-    hc.emin = 0.0;
-    hc.ciso = -1.0;
-    hc.cisosq = 1.0;
-    hipLaunchKernelGGL(ens_cfl_pass1, dim3(kEnsCflBlocks, (unsigned)e->P), dim3(256), 0, s, e->dprobs,
-                       e->gamma * (e->gamma - 1.0), e->cflPart);
+    hipLaunchKernelGGL(ens_cfl_pass1, dim3(kEnsCflBlocks, (unsigned)e->NE), dim3(256), 0, s, e->dprobs, e->cflPart);
     int rc = IM_LAUNCHED("ens_cfl_pass1");
     if (rc) return rc;
     hipLaunchKernelGGL(ens_cfl_pass2, dim3((unsigned)e->P), dim3(64), 0, s, e->dprobs, e->cflPart, kEnsCflBlocks,
-                       e->cfl, e->lam, e->clock);
+                       e->cfl, e->lam, e->clock, e->dtp);
     rc = IM_LAUNCHED("ens_cfl_pass2");
     if (rc) return rc;
+    if ((rc = ens_source(e, 0.5, s))) return rc;
     for (int dirn = 1; dirn >= -1; dirn -= 2) {
         int sweep = (it + 3 * (dirn > 0)) % 6;
         for (int n = 0; n < 3; n++) {
             const int d = kFluxcall[n][sweep];
             if (d == 3) continue;  // 2D problems
             switch (e->limiter) {
-            case LIMITER_OSPRE: rc = ens_sweep<0>(e, d, hc, 0, s); break;
-            case LIMITER_VANLEER: rc = ens_sweep<1>(e, d, hc, 1, s); break;
-            default: rc = ens_sweep<-1>(e, d, hc, e->limiter, s); break;
+            case LIMITER_OSPRE: rc = ens_sweep<0>(e, d, 0, s); break;
+            case LIMITER_VANLEER: rc = ens_sweep<1>(e, d, 1, s); break;
+            default: rc = ens_sweep<-1>(e, d, e->limiter, s); break;
             }
             if (rc) return rc;
         }
+        if ((rc = ens_source(e, dirn > 0 ? 1.0 : 0.5, s))) return rc;
     }
     return SUCCESSFUL;
 }
@@ -400,6 +466,7 @@ int imogen_ensemble_create(int nprob, const int *nx, const int *ny, const double
     ImogenEnsemble *E = new ImogenEnsemble();
     EnsembleImpl *e = &E->impl;
     e->P = nprob;
+    e->NE = nprob;
     e->gamma = gamma;
     e->cfl = cfl;
     e->limiter = limiter;
@@ -415,7 +482,7 @@ int imogen_ensemble_create(int nprob, const int *nx, const int *ny, const double
         const int64_t pitch = (n + 31) / 32 * 32;  // 256-byte slabs
         off[p] = total;
         total += 5 * pitch;
-        e->host.push_back(EnsProblem{nullptr, pitch, nx[p], ny[p], hx[p], hy[p]});
+        e->host.push_back(EnsProblem{nullptr, pitch, nx[p], ny[p], hx[p], hy[p], gamma, p, -1});
         e->cells += n;
         e->segS.push_back(ny[p] >= 256 ? 2 : 1);
     }
@@ -428,6 +495,7 @@ int imogen_ensemble_create(int nprob, const int *nx, const int *ny, const double
     if (!rc && IM_HIP(hipMalloc(&e->lam, 2 * nprob * sizeof(double)))) rc = ERROR_CUDA_BLEW_UP;
     if (!rc && IM_HIP(hipMalloc(&e->clock, nprob * sizeof(double)))) rc = ERROR_CUDA_BLEW_UP;
     if (!rc && IM_HIP(hipMemset(e->clock, 0, nprob * sizeof(double)))) rc = ERROR_CUDA_BLEW_UP;
+    if (!rc && IM_HIP(hipMalloc(&e->dtp, nprob * sizeof(double)))) rc = ERROR_CUDA_BLEW_UP;
     if (!rc && IM_HIP(hipMalloc(&e->cflPart, (size_t)nprob * kEnsCflBlocks * sizeof(double)))) rc = ERROR_CUDA_BLEW_UP;
     if (!rc) rc = build_work(e);
     if (rc) {
@@ -468,8 +536,135 @@ int imogen_ensemble_get_state(ImogenEnsemble *E, int p, double *host)
     return IM_HIP(hipStreamSynchronize(st->stream));
 }
 
+int imogen_ensemble_set_dust(ImogenEnsemble *E, const int *hasDust, double dustGamma, const ThermoDetails *gas,
+                             const ThermoDetails *dust, int dragMethod)
+{
+    if (!E || !hasDust || !gas || !dust) return IM_FAIL(ERROR_NULL_POINTER, "null argument");
+    EnsembleImpl *e = &E->impl;
+    if (e->Pd) return IM_FAIL(ERROR_INVALID_ARGS, "the ensemble already carries dust");
+    if (dragMethod != DRAG_LOGTRAP2 && dragMethod != DRAG_LOGTRAP3 && dragMethod != DRAG_EXACT &&
+        dragMethod != DRAG_LEARNED)
+        return IM_FAIL(ERROR_NOIMPLEMENT, "ensemble drag method %d", dragMethod);
+    if (!(dustGamma > 1.0)) return IM_FAIL(ERROR_INVALID_ARGS, "dust gamma %g", dustGamma);
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    if (IM_HIP(hipStreamSynchronize(st->stream))) return ERROR_CUDA_BLEW_UP;
+    std::vector<int> dusty;
+    int64_t total = 0, cells = 0;
+    for (int p = 0; p < e->P; p++)
+        if (hasDust[p]) {
+            dusty.push_back(p);
+            total += 5 * e->host[p].pitch;
+            cells += (int64_t)e->host[p].nx * e->host[p].ny;
+        }
+    if (dusty.empty()) return SUCCESSFUL;
+    const int NE = e->P + (int)dusty.size();
+    double *dpool = nullptr, *lam = nullptr, *part = nullptr;
+    EnsProblem *dprobs = nullptr;
+    EnsDragTable *tab = nullptr;
+    EnsDragItem *items = nullptr;
+    int64_t *queue = nullptr;
+    unsigned long long *qcount = nullptr;
+    std::vector<EnsProblem> host = e->host;
+    std::vector<EnsDragTable> htab;
+    std::vector<EnsDragItem> hitems;
+    int64_t off = 0;
+    int rc = IM_HIP(hipMalloc(&dpool, (size_t)total * sizeof(double)));
+    for (size_t k = 0; k < dusty.size() && !rc; k++) {
+        const int p = dusty[k];
+        EnsProblem D = host[p];
+        D.q = dpool + off;
+        D.gamma = dustGamma;
+        D.owner = p;
+        D.dust = -1;
+        off += 5 * D.pitch;
+        host[p].dust = e->P + (int)k;
+        host.push_back(D);
+        const int64_t n = (int64_t)D.nx * D.ny;
+        htab.push_back(EnsDragTable{host[p].q, D.q, D.pitch, n, p, 0});
+        for (int64_t b = 0; b < n; b += 64) hitems.push_back(EnsDragItem{(int)k, 0, b});
+    }
+    if (!rc) rc = IM_HIP(hipMalloc(&dprobs, NE * sizeof(EnsProblem)));
+    if (!rc) rc = IM_HIP(hipMemcpy(dprobs, host.data(), NE * sizeof(EnsProblem), hipMemcpyHostToDevice));
+    if (!rc) rc = IM_HIP(hipMalloc(&lam, 2 * NE * sizeof(double)));
+    if (!rc) rc = IM_HIP(hipMalloc(&part, (size_t)NE * kEnsCflBlocks * sizeof(double)));
+    if (!rc) rc = IM_HIP(hipMalloc(&tab, htab.size() * sizeof(EnsDragTable)));
+    if (!rc) rc = IM_HIP(hipMemcpy(tab, htab.data(), htab.size() * sizeof(EnsDragTable), hipMemcpyHostToDevice));
+    if (!rc) rc = IM_HIP(hipMalloc(&items, hitems.size() * sizeof(EnsDragItem)));
+    if (!rc) rc = IM_HIP(hipMemcpy(items, hitems.data(), hitems.size() * sizeof(EnsDragItem), hipMemcpyHostToDevice));
+    if (!rc) rc = IM_HIP(hipMalloc(&queue, (size_t)cells * sizeof(int64_t)));
+    if (!rc) rc = IM_HIP(hipMalloc(&qcount, sizeof(unsigned long long)));
+    if (rc) {
+        void *bufs[] = {dpool, dprobs, lam, part, tab, items, queue, qcount};
+        for (void *b : bufs)
+            if (b) (void)hipFree(b);
+        return ERROR_CUDA_BLEW_UP;
+    }
+    void *old[] = {e->dprobs, e->lam, e->cflPart};
+    for (void *b : old)
+        if (b) (void)hipFree(b);
+    e->host = host;
+    e->NE = NE;
+    e->Pd = (int)dusty.size();
+    e->dpool = dpool;
+    e->dprobs = dprobs;
+    e->lam = lam;
+    e->cflPart = part;
+    e->dragTab = tab;
+    e->dragItems = items;
+    e->nDragItems = (int)hitems.size();
+    e->queue = queue;
+    e->qcount = qcount;
+    e->dc = drag_consts(gas, dust);
+    e->dragMethod = dragMethod;
+    e->modelGen = learned_model_generation();
+    return build_work(e);
+}
+
+static int dust_entry(EnsembleImpl *e, int p)
+{
+    if (p < 0 || p >= e->P) return -1;
+    return e->host[p].dust;
+}
+
+int imogen_ensemble_set_dust_state(ImogenEnsemble *E, int p, const double *host)
+{
+    if (!E || !host) return IM_FAIL(ERROR_NULL_POINTER, "null argument");
+    EnsembleImpl *e = &E->impl;
+    const int de = dust_entry(e, p);
+    if (de < 0) return IM_FAIL(ERROR_INVALID_ARGS, "problem %d carries no dust", p);
+    DeviceState *st = state();
+    const EnsProblem &P = e->host[de];
+    const int64_t n = (int64_t)P.nx * P.ny;
+    for (int k = 0; k < 5; k++)
+        if (IM_HIP(hipMemcpyAsync(P.q + k * P.pitch, host + k * n, n * sizeof(double), hipMemcpyHostToDevice,
+                                  st->stream)))
+            return ERROR_CUDA_BLEW_UP;
+    return IM_HIP(hipStreamSynchronize(st->stream));
+}
+
+int imogen_ensemble_get_dust_state(ImogenEnsemble *E, int p, double *host)
+{
+    if (!E || !host) return IM_FAIL(ERROR_NULL_POINTER, "null argument");
+    EnsembleImpl *e = &E->impl;
+    const int de = dust_entry(e, p);
+    if (de < 0) return IM_FAIL(ERROR_INVALID_ARGS, "problem %d carries no dust", p);
+    DeviceState *st = state();
+    const EnsProblem &P = e->host[de];
+    const int64_t n = (int64_t)P.nx * P.ny;
+    for (int k = 0; k < 5; k++)
+        if (IM_HIP(hipMemcpyAsync(host + k * n, P.q + k * P.pitch, n * sizeof(double), hipMemcpyDeviceToHost,
+                                  st->stream)))
+            return ERROR_CUDA_BLEW_UP;
+    return IM_HIP(hipStreamSynchronize(st->stream));
+}
+
 int imogen_ensemble_run(ImogenEnsemble *E, int firstIteration, int iterations, int useGraph)
 {
+    if (E && E->impl.Pd && E->impl.dragMethod == DRAG_LEARNED && E->impl.modelGen != learned_model_generation()) {
+        free_graphs(&E->impl);  // the graphs hold the previous model by value
+        E->impl.modelGen = learned_model_generation();
+    }
     if (!E) return IM_FAIL(ERROR_NULL_POINTER, "null ensemble");
     EnsembleImpl *e = &E->impl;
     DeviceState *st = state();
@@ -568,7 +763,8 @@ int imogen_ensemble_destroy(ImogenEnsemble *E)
     free_graphs(e);
     for (int c = 0; c < 6; c++)
         if (e->xwork[c]) (void)hipFree(e->xwork[c]);
-    void *bufs[] = {e->pool, e->dprobs, e->lam, e->clock, e->cflPart, e->ywork, e->xdummy, e->ydefer};
+    void *bufs[] = {e->pool,   e->dprobs,   e->lam,       e->clock, e->cflPart, e->ywork, e->xdummy,
+                    e->ydefer, e->dpool,    e->dtp,       e->dragTab, e->dragItems, e->queue, e->qcount};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     delete E;

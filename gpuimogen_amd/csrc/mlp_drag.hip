@@ -225,64 +225,74 @@ __device__ __forceinline__ void drag_features(const DragConsts &c, const DragCel
 // Cells the surrogate does not accept are appended to a queue (one atomic
 // per wave) and integrated by fallback_drag_kernel in dense waves, so a few
 // rejected lanes do not hold their whole wave in the adaptive integrator.
+// One wave of 64 cells [base, base+64) of an n-cell range: features, network,
+// residual check; accepted cells are updated, rejected ones appended to the
+// fallback queue as qtag | index (one atomic per wave).
+__device__ __forceinline__ void learned_cells(double *__restrict__ g, double *__restrict__ d, int64_t pitch,
+                                              int64_t base, int64_t n, const DragConsts &c, double dt,
+                                              const MlpDev &m, int64_t *__restrict__ queue,
+                                              unsigned long long *__restrict__ qcount, int64_t qtag)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t i = base + lane;
+    const bool live = i < n;
+    const int64_t ic = live ? i : n - 1;  // lanes past the end work on a valid cell and store nothing
+    const DragCell s = drag_load(g + ic, d + ic, pitch);
+    const bool moving = s.s0 >= 1e-9;
+    DragRate r0 = {0.0, 1.0, 1.0, 0.0};
+    if (moving) r0 = drag_k_parts(c, s.rg, s.rd, s.s0, s.u0);
+    float f[5];
+    drag_features(c, s, r0, dt, f);
+    bool inside = moving;
+    float z[5];
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+        inside = inside && (f[j] >= m.norm.flo[j]) && (f[j] <= m.norm.fhi[j]);
+        z[j] = inside ? (f[j] - m.norm.fmean[j]) / m.norm.fstd[j] : 0.0f;
+    }
+    float o[3];
+    mlp_wave(m, z, o);
+    double fac = 1.0;
+    bool reject = false;
+    if (moving) {
+        const double tau = dt * r0.k;
+        const double q0 = inside ? (double)(o[0] * m.norm.tstd[0] + m.norm.tmean[0]) : 0.0;
+        const double q1 = inside ? (double)(o[1] * m.norm.tstd[1] + m.norm.tmean[1]) : 0.0;
+        const double q2 = inside ? (double)(o[2] * m.norm.tstd[2] + m.norm.tmean[2]) : 0.0;
+        const double kbar = r0.k * (1.0 + tau * (.5 * r0.g + tau * q0));
+        const double kmid = r0.k * (1.0 + tau * (.5 * r0.g + tau * q1));
+        const double kend = r0.k * (1.0 + tau * (r0.g + tau * q2));
+        // end state implied by kbar; its true rate checks the prediction
+        const double s1 = s.s0 * exp(-kbar * dt);
+        const double u1 = s.u0 + .5 * s.rd / (s.rg + s.rd) * (s.s0 * s.s0 - s1 * s1);
+        const double k1 = drag_k(c, s.rg, s.rd, s1, u1);
+        const double ksimp = (r0.k + 4.0 * kmid + k1) * (1.0 / 6.0);
+        const double e1 = fabs(dt * (k1 - kend));
+        const double e2 = fabs(dt * (kbar - ksimp));
+        const bool ok = (e1 <= m.norm.residTol) && (e2 <= m.norm.residTol) && kbar > 0.0;
+        if (ok) fac = exp(-kbar * dt);
+        else reject = live;
+    }
+    const unsigned long long mask = __ballot(reject);
+    if (mask) {
+        const int leader = __ffsll((long long)mask) - 1;
+        unsigned long long slot = 0;
+        if (lane == leader) slot = atomicAdd(qcount, (unsigned long long)__popcll(mask));
+        slot = __shfl(slot, leader, 64);
+        if (reject) queue[slot + __popcll(mask & ((1ull << lane) - 1ull))] = qtag | i;
+    }
+    if (live && !reject) drag_apply(g + i, d + i, pitch, s, fac);
+}
+
 __global__ __launch_bounds__(256) void learned_drag_kernel(double *__restrict__ g, double *__restrict__ d,
                                                            int64_t pitch, int64_t n, DragConsts c, double dt,
                                                            MlpDev m, int64_t *__restrict__ queue,
                                                            unsigned long long *__restrict__ qcount)
 {
-    const int lane = threadIdx.x & 63;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-    for (int64_t base = wave * 64; base < n; base += nwaves * 64) {
-        const int64_t i = base + lane;
-        const bool live = i < n;
-        const int64_t ic = live ? i : n - 1;  // lanes past the end work on a valid cell and store nothing
-        const DragCell s = drag_load(g + ic, d + ic, pitch);
-        const bool moving = s.s0 >= 1e-9;
-        DragRate r0 = {0.0, 1.0, 1.0, 0.0};
-        if (moving) r0 = drag_k_parts(c, s.rg, s.rd, s.s0, s.u0);
-        float f[5];
-        drag_features(c, s, r0, dt, f);
-        bool inside = moving;
-        float z[5];
-#pragma unroll
-        for (int j = 0; j < 5; j++) {
-            inside = inside && (f[j] >= m.norm.flo[j]) && (f[j] <= m.norm.fhi[j]);
-            z[j] = inside ? (f[j] - m.norm.fmean[j]) / m.norm.fstd[j] : 0.0f;
-        }
-        float o[3];
-        mlp_wave(m, z, o);
-        double fac = 1.0;
-        bool reject = false;
-        if (moving) {
-            const double tau = dt * r0.k;
-            const double q0 = inside ? (double)(o[0] * m.norm.tstd[0] + m.norm.tmean[0]) : 0.0;
-            const double q1 = inside ? (double)(o[1] * m.norm.tstd[1] + m.norm.tmean[1]) : 0.0;
-            const double q2 = inside ? (double)(o[2] * m.norm.tstd[2] + m.norm.tmean[2]) : 0.0;
-            const double kbar = r0.k * (1.0 + tau * (.5 * r0.g + tau * q0));
-            const double kmid = r0.k * (1.0 + tau * (.5 * r0.g + tau * q1));
-            const double kend = r0.k * (1.0 + tau * (r0.g + tau * q2));
-            // end state implied by kbar; its true rate checks the prediction
-            const double s1 = s.s0 * exp(-kbar * dt);
-            const double u1 = s.u0 + .5 * s.rd / (s.rg + s.rd) * (s.s0 * s.s0 - s1 * s1);
-            const double k1 = drag_k(c, s.rg, s.rd, s1, u1);
-            const double ksimp = (r0.k + 4.0 * kmid + k1) * (1.0 / 6.0);
-            const double e1 = fabs(dt * (k1 - kend));
-            const double e2 = fabs(dt * (kbar - ksimp));
-            const bool ok = (e1 <= m.norm.residTol) && (e2 <= m.norm.residTol) && kbar > 0.0;
-            if (ok) fac = exp(-kbar * dt);
-            else reject = live;
-        }
-        const unsigned long long mask = __ballot(reject);
-        if (mask) {
-            const int leader = __ffsll((long long)mask) - 1;
-            unsigned long long slot = 0;
-            if (lane == leader) slot = atomicAdd(qcount, (unsigned long long)__popcll(mask));
-            slot = __shfl(slot, leader, 64);
-            if (reject) queue[slot + __popcll(mask & ((1ull << lane) - 1ull))] = i;
-        }
-        if (live && !reject) drag_apply(g + i, d + i, pitch, s, fac);
-    }
+    for (int64_t base = wave * 64; base < n; base += nwaves * 64)
+        learned_cells(g, d, pitch, base, n, c, dt, m, queue, qcount, 0);
 }
 
 __global__ __launch_bounds__(256) void fallback_drag_kernel(double *__restrict__ g, double *__restrict__ d,
@@ -306,6 +316,56 @@ __global__ __launch_bounds__(256) void fallback_drag_kernel(double *__restrict__
 __global__ void zero_counter(unsigned long long *p)
 {
     *p = 0;
+}
+
+// ---------------------------------------------------------------------------
+// drag over the dusty problems of an ensemble: one wave per 64-cell item,
+// dt = frac * dTime of the item's problem (read on the device)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void ens_drag_kernel(const EnsDragTable *__restrict__ tab,
+                                                       const EnsDragItem *__restrict__ items, int nitems,
+                                                       const double *__restrict__ dtp, double frac, DragConsts c,
+                                                       int method, MlpDev m, int64_t *__restrict__ queue,
+                                                       unsigned long long *__restrict__ qcount)
+{
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (wave >= nitems) return;
+    const EnsDragItem it = items[wave];
+    const EnsDragTable T = tab[it.t];
+    const double dt = frac * dtp[T.p];
+    if (method == DRAG_LEARNED) {
+        learned_cells(T.g, T.d, T.pitch, it.base, T.n, c, dt, m, queue, qcount, (int64_t)it.t << 32);
+        return;
+    }
+    const int64_t i = it.base + (threadIdx.x & 63);
+    if (i >= T.n) return;
+    const DragCell s = drag_load(T.g + i, T.d + i, T.pitch);
+    double f;
+    if (method == DRAG_EXACT) {
+        int k;
+        f = drag_exact_ratio(c, s.rg, s.rd, s.s0, s.u0, dt, &k);
+    } else {
+        f = exp(-dt * drag_logtrap_rate(c, s.rg, s.rd, s.s0, s.u0, dt, method == DRAG_LOGTRAP3 ? 3 : 2));
+    }
+    drag_apply(T.g + i, T.d + i, T.pitch, s, f);
+}
+
+__global__ __launch_bounds__(256) void ens_fallback_kernel(const EnsDragTable *__restrict__ tab,
+                                                           const double *__restrict__ dtp, double frac, DragConsts c,
+                                                           const int64_t *__restrict__ queue,
+                                                           const unsigned long long *__restrict__ qcount)
+{
+    const unsigned long long nq = *qcount;
+    for (unsigned long long t = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; t < nq;
+         t += (unsigned long long)gridDim.x * blockDim.x) {
+        const int64_t e = queue[t];
+        const EnsDragTable T = tab[(int)(e >> 32)];
+        const int64_t i = e & 0xffffffffll;
+        const DragCell s = drag_load(T.g + i, T.d + i, T.pitch);
+        int steps;
+        const double fac = drag_exact_ratio(c, s.rg, s.rd, s.s0, s.u0, frac * dtp[T.p], &steps);
+        drag_apply(T.g + i, T.d + i, T.pitch, s, fac);
+    }
 }
 
 // standalone BF16 inference (tests)
@@ -584,6 +644,7 @@ struct HostModel {
     unsigned long long *counts = nullptr;  // device [unused, fallbacks, queue length, unused]
 };
 static HostModel g_model;
+static uint64_t g_modelGeneration = 0;  // bumped by every imogen_drag_set_model
 static int64_t g_learnedCells = 0;  // cells through the learned drag since the last stats read
 
 static int ensure_counts()
@@ -623,6 +684,32 @@ int launch_learned_drag(double *g, double *d, int64_t pitch, int64_t n, const Dr
     if (rc) return rc;
     g_learnedCells += n;
     return SUCCESSFUL;
+}
+
+int launch_ens_drag(const EnsDragTable *tab, const EnsDragItem *items, int nitems, const double *dtp, double frac,
+                    const DragConsts &c, int method, int64_t *queue, unsigned long long *qcount, hipStream_t s)
+{
+    if (method != DRAG_LOGTRAP2 && method != DRAG_LOGTRAP3 && method != DRAG_EXACT && method != DRAG_LEARNED)
+        return IM_FAIL(ERROR_NOIMPLEMENT, "ensemble drag method %d", method);
+    if (method == DRAG_LEARNED && !g_model.loaded)
+        return IM_FAIL(ERROR_INVALID_ARGS, "learned drag: no model (imogen_drag_set_model)");
+    if (nitems <= 0) return SUCCESSFUL;
+    if (method == DRAG_LEARNED) {
+        hipLaunchKernelGGL(zero_counter, dim3(1), dim3(1), 0, s, qcount);
+        int rc = IM_LAUNCHED("zero_counter");
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(ens_drag_kernel, dim3((unsigned)((nitems + 3) / 4)), dim3(256), 0, s, tab, items, nitems, dtp,
+                       frac, c, method, g_model.dev, queue, qcount);
+    int rc = IM_LAUNCHED("ens_drag_kernel");
+    if (rc || method != DRAG_LEARNED) return rc;
+    hipLaunchKernelGGL(ens_fallback_kernel, dim3(1024), dim3(256), 0, s, tab, dtp, frac, c, queue, qcount);
+    return IM_LAUNCHED("ens_fallback_kernel");
+}
+
+uint64_t learned_model_generation()
+{
+    return g_modelGeneration;
 }
 
 }  // namespace imogen
@@ -676,6 +763,7 @@ int imogen_drag_set_model(const float *params, const DragModelNorm *norm)
     g_model.dev.b3 = bd + 128;
     g_model.dev.norm = *norm;
     g_model.loaded = true;
+    g_modelGeneration++;
     return SUCCESSFUL;
 }
 

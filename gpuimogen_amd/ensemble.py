@@ -4,7 +4,8 @@ The workload of the batched-ensemble benchmark: `count` four-quadrant Riemann
 problems on the unit square, N_k x N_k cells with N_k = round(nmin (nmax /
 nmin)^u), u ~ U[0, 1] (log-uniform sizes), quadrant states rho, P ~ U[0.1, 2]
 and v_x, v_y ~ U[-1, 1], gamma = 1.4, constant-extrapolation boundaries,
-per-problem CFL 0.4.
+per-problem CFL 0.4. Odd-numbered problems may carry dust at dust-to-gas
+ratio 0.5 (quadrant_dust, Ensemble.set_dust) coupled by the drag source.
 """
 from __future__ import annotations
 
@@ -14,6 +15,22 @@ import numpy as np
 
 from . import abi
 from .problems import conserved
+
+
+def quadrant_dust(gas_state, seed, eps=0.5, dust_gamma=1.01, dust_cs=0.3):
+    """Dust for a quadrant problem: rho_d = eps rho_g, its own quadrant
+    velocities (U[-1, 1]), a cold gamma = 1.01 fluid with sound speed dust_cs."""
+    rng = np.random.default_rng(seed)
+    n = gas_state.shape[-1]
+    ny = gas_state.shape[-2]
+    rho = eps * gas_state[0]
+    vx, vy = rng.uniform(-1, 1, 4), rng.uniform(-1, 1, 4)
+    h, hy = n // 2, ny // 2
+    q_idx = np.zeros((ny, n), dtype=int)
+    q_idx[:hy, h:] = 1
+    q_idx[hy:, :h] = 2
+    q_idx[hy:, h:] = 3
+    return conserved(rho, vx[q_idx], vy[q_idx], np.zeros((ny, n)), rho * dust_cs ** 2 / dust_gamma, dust_gamma)
 
 
 def quadrant_problems(count=512, seed=0, nmin=64, nmax=512, gamma=1.4):
@@ -58,6 +75,27 @@ class Ensemble:
     def state(self, p):
         out = np.empty((5, self.ny[p], self.nx[p]))
         abi.call("imogen_ensemble_get_state", self.h, int(p), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return out
+
+    def set_dust(self, has_dust, gas_thermo, dust_thermo, dust_gamma=1.01, method=abi.DRAG_LEARNED):
+        """Problems with has_dust[p] carry a dust fluid coupled by the drag source."""
+        flags = np.ascontiguousarray(np.asarray(has_dust, dtype=bool).astype(np.int32))
+        if flags.shape != (self.count,):
+            raise ValueError("one dust flag per problem")
+        abi.call("imogen_ensemble_set_dust", self.h, flags.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                 float(dust_gamma), ctypes.byref(gas_thermo), ctypes.byref(dust_thermo), int(method))
+        self.dusty = flags.astype(bool)
+        self.dust_cells = int(np.sum((self.nx.astype(np.int64) * self.ny)[self.dusty]))
+
+    def set_dust_state(self, p, state):
+        s = np.ascontiguousarray(state, dtype=np.float64)
+        if s.shape != (5, self.ny[p], self.nx[p]):
+            raise ValueError(f"problem {p} expects shape (5, {self.ny[p]}, {self.nx[p]})")
+        abi.call("imogen_ensemble_set_dust_state", self.h, int(p), s.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+
+    def dust_state(self, p):
+        out = np.empty((5, self.ny[p], self.nx[p]))
+        abi.call("imogen_ensemble_get_dust_state", self.h, int(p), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
         return out
 
     def run(self, iterations, graph=True):

@@ -337,6 +337,16 @@ int imogen_ensemble_times(ImogenEnsemble *e, double *t);
  * march; chosen[3] (optional) receives them. The ensemble state advances
  * during tuning and the clocks are reset to 0 afterwards. */
 int imogen_ensemble_autotune(ImogenEnsemble *e, int trials, int *chosen);
+/* Gives the problems with hasDust[p] != 0 a second fluid, dust (HLLC with
+ * gamma dustGamma, same grid and boundaries), coupled to the gas by the
+ * two-fluid drag source of sourcefunction_2FluidDrag (method 4, 5, 6 or 7;
+ * gas and dust microphysics as there) applied as sources/src0001.m does
+ * around the fluid steps of imogen.m:115-139; the time step of a dusty
+ * problem is the minimum over both fluids. Call once, before running. */
+int imogen_ensemble_set_dust(ImogenEnsemble *e, const int *hasDust, double dustGamma, const ThermoDetails *gas,
+                             const ThermoDetails *dust, int dragMethod);
+int imogen_ensemble_set_dust_state(ImogenEnsemble *e, int p, const double *host);
+int imogen_ensemble_get_dust_state(ImogenEnsemble *e, int p, double *host);
 int imogen_ensemble_destroy(ImogenEnsemble *e);
 
 /* ======================================================================== */

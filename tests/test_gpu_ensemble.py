@@ -82,3 +82,59 @@ def test_graph_replay_equals_eager_and_autotune_runs():
     assert np.abs(a.state(3) - ref).max() <= 1e-12 * np.abs(ref).max()
     a.free()
     b.free()
+
+
+DUSTY_SIZES = [64, 77, 100, 130]
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_ensemble_with_dust_matches_single_problems(graph):
+    """Odd problems carry dust (dust-to-gas ratio 0.5, learned drag with exact
+    fallback); every problem must match the same problem advanced alone by the
+    host-driven loop with the same fluids and drag source (imogen.Simulation)."""
+    from gpuimogen_amd import abi, ensemble as ens, gpuclass as gc, imogen, learned_drag as L
+    params, norm, _ = L.load_model()
+    L.set_model(params, norm)
+    tg, td = L.cold_h2(), L.iron_dust()
+    tg.gamma = 1.4  # the quadrant problems' gas
+    rng = np.random.default_rng(21)
+    gas, dust = [], {}
+    for p, n in enumerate(DUSTY_SIZES):
+        _, st = ens.quadrant_problems(1, seed=int(rng.integers(1 << 30)), nmin=n, nmax=n)
+        gas.append(st[0])
+        if p % 2:
+            dust[p] = ens.quadrant_dust(st[0], seed=100 + p)
+    sizes = np.array(DUSTY_SIZES)
+    e = ens.Ensemble(sizes, sizes, 1.0 / sizes, 1.0 / sizes)
+    for p, s in enumerate(gas):
+        e.set_state(p, s)
+    e.set_dust([p % 2 == 1 for p in range(len(sizes))], tg, td, 1.01, abi.DRAG_LEARNED)
+    for p, s in dust.items():
+        e.set_dust_state(p, s)
+    iters = 3
+    e.run(iters, graph=graph)
+    times = e.times()
+    for p, n in enumerate(DUSTY_SIZES):
+        print(f"problem {p}: {n}^2, dust={p in dust}, graph={graph}", flush=True)
+        gm = gc.GeometryManager((n, n, 1), (1.0 / n, 1.0 / n, 1.0))
+        fl = [gc.FluidManager(gas[p].reshape(5, 1, n, n), 1.4, minmass=1e-8, bc_modes=(abi.extrapConstant,) * 6,
+                              thermoDetails=tg)]
+        if p in dust:
+            fl.append(gc.FluidManager(dust[p].reshape(5, 1, n, n), 1.01, minmass=1e-8,
+                                      bc_modes=(abi.extrapConstant,) * 6, thermoDetails=td))
+        sim = imogen.Simulation(fl, [abi.METHOD_HLLC] * len(fl), gm, cfl=0.4,
+                                drag_method=abi.DRAG_LEARNED if p in dust else None)
+        for _ in range(iters):
+            sim.step()
+        pairs = [(e.state(p), fl[0].state.reshape(5, n, n))]
+        if p in dust:
+            pairs.append((e.dust_state(p), fl[1].state.reshape(5, n, n)))
+        for got, ref in pairs:
+            for s in range(5):
+                d = np.abs(got[s] - ref[s]).sum()
+                scale = max(np.abs(ref[s]).sum(), 1e-300)
+                assert d / scale <= 1e-12, (p, s, d / scale)
+        assert times[p] == pytest.approx(sim.time, rel=1e-13)
+        for f in fl:
+            f.free()
+    e.free()
