@@ -4,6 +4,8 @@
 //   |dv| and u_0 (cukern_prepareForERK_h0, gpuclass/cudaSource2FluidDrag.cu:1883-1908),
 //   a decay factor f of the relative velocity over dt, from
 //     LogTrap2 / LogTrap3 (cukern_LogTrapSolve, :1532-1630), or
+//     exponential Euler with the rate frozen at t = 0 (solveDragETDRK1,
+//     :477-577), or
 //     the explicit midpoint / RK4 integrators of |dv| (solveDragEMP,
 //     solveDragRK4, :305-467; these rebuild the state with
 //     cukern_applyFinalDeltaV, :1160-1237), or
@@ -27,6 +29,10 @@ __global__ __launch_bounds__(256) void drag_kernel(double *__restrict__ g, doubl
     unsigned long long steps = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         DragCell s = drag_load(g + i, d + i, pitch);
+        if (METHOD == DRAG_ETDRK1) {
+            drag_etd1_apply(g + i, d + i, pitch, dt, drag_k(c, s.rg, s.rd, s.s0, s.u0));
+            continue;
+        }
         double f;
         if (METHOD == DRAG_EXACT) {
             int k = 0;
@@ -92,6 +98,10 @@ int launch_drag(double *g, double *d, int64_t pitch, int64_t n, const DragConsts
         hipLaunchKernelGGL((drag_kernel<DRAG_EXACT>), dim3((unsigned)blocks), dim3(256), 0, s, g, d, pitch, n, c,
                            dt, work);
         break;
+    case DRAG_ETDRK1:
+        hipLaunchKernelGGL((drag_kernel<DRAG_ETDRK1>), dim3((unsigned)blocks), dim3(256), 0, s, g, d, pitch, n, c,
+                           dt, work);
+        break;
     case DRAG_EMP:
         hipLaunchKernelGGL((drag_rk_kernel<DRAG_EMP>), dim3((unsigned)blocks), dim3(256), 0, s, g, d, pitch, n, c, dt);
         break;
@@ -134,9 +144,9 @@ int sourcefunction_2FluidDrag(MGArray *fluidA, MGArray *fluidB, GeometryParams *
     case DRAG_LOGTRAP3:
     case DRAG_EXACT:
     case DRAG_EMP:
-    case DRAG_RK4: rc = launch_drag(g, d, pg, n, c, dt, method, nullptr, st->stream); break;
+    case DRAG_RK4:
+    case DRAG_ETDRK1: rc = launch_drag(g, d, pg, n, c, dt, method, nullptr, st->stream); break;
     case DRAG_LEARNED: rc = launch_learned_drag(g, d, pg, n, c, dt, st->stream); break;
-    case DRAG_ETDRK1:
     case DRAG_ETDRK2: rc = IM_FAIL(ERROR_NOIMPLEMENT, "drag method %d is not built", method); break;
     default: rc = IM_FAIL(ERROR_INVALID_ARGS, "unknown drag method %d", method);
     }

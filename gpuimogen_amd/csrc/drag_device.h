@@ -80,6 +80,32 @@ __device__ __forceinline__ void drag_apply(double *g, double *d, int64_t p, cons
     g[p] -= .5 * pdustsq / s.rd;
 }
 
+// Exponential Euler update (cukern_ETDRK1, :1426-1530, with ACCOUNT_GRADP
+// undefined, :63): every dv component decays by exp(-t k0) about the
+// barycentric velocity, in the reference's driven form dpdt/k + (dv - dpdt/k)
+// e^{-kt} with dpdt = 0 (NaN when k0 = 0, as there)
+__device__ __forceinline__ void drag_etd1_apply(double *g, double *d, int64_t p, double t, double k)
+{
+    const double rhoginv = 1.0 / g[0];
+    const double mu = g[0] * d[0] / (g[0] + d[0]);
+    double pdustsq = 0.0;
+#pragma unroll
+    for (int c = 2; c <= 4; c++) {
+        const double pg = g[c * p], pd = d[c * p];
+        pdustsq -= pd * pd;
+        const double vstick = (pg + pd) / (g[0] + d[0]);
+        const double dv_i = pg * rhoginv - pd / d[0];
+        double dpdt = 0.0;
+        dpdt *= mu * rhoginv;
+        const double dv_t = dpdt / k + (dv_i - dpdt / k) * exp(-t * k);
+        g[c * p] = g[0] * vstick + dv_t * mu;
+        const double q = d[0] * vstick - dv_t * mu;
+        d[c * p] = q;
+        pdustsq += q * q;
+    }
+    g[p] -= .5 * pdustsq / d[0];
+}
+
 // path-averaged drag rate of LogTrap2 / LogTrap3 (cukern_LogTrapSolve, :1546-1630)
 __device__ __forceinline__ double drag_logtrap_rate(const DragConsts &c, double rhog, double rhod, double dv0,
                                                     double u0, double t, int order)

@@ -11,7 +11,9 @@
  * midpoint and classical RK4 integrators of |dv| (solveDragEMP :305-370,
  * solveDragRK4 :372-467: cukern_findInitialDeltaV :1014-1056,
  * cukern_GasDustDrag_GeneralAccel :898-946, cukern_SolveRK_single/final
- * :1062-1100, cukern_applyFinalDeltaV :1160-1237).
+ * :1062-1100, cukern_applyFinalDeltaV :1160-1237), and exponential Euler
+ * (solveDragETDRK1 :477-577 with ACCOUNT_GRADP undefined (:63): the drag
+ * rate frozen at t = 0, cukern_ETDRK1 :1426-1530).
  *
  * The "exact" integrator (method 6) has no counterpart in the reference's
  * compiled code; it integrates the same scalar relaxation problem that the
@@ -280,11 +282,35 @@ double orc_drag_rk_magdv(const double *P, double rhog, double rhod, double s0, d
     return s0 + (dt / 6.0) * (acc + 1.0 * k);
 }
 
+/* cukern_ETDRK1 (:1426-1530) without the pressure-gradient drive: each dv
+ * component decays by exp(-t k0) about the barycentric velocity; the
+ * reference's driven form dpdt/k + (dv - dpdt/k) e^{-kt} is kept (it is NaN
+ * when k0 = 0, i.e. below the |dv| cut-off). */
+static void etdrk1_apply(double *g, double *d, long pitch, double t, double k)
+{
+    double rhoginv = 1.0 / g[0];
+    double mu = g[0] * d[0] / (g[0] + d[0]);
+    double pdustsq = 0.0;
+    for (int c = 2; c <= 4; c++) {
+        pdustsq -= d[c * pitch] * d[c * pitch];
+        double vstick = (g[c * pitch] + d[c * pitch]) / (g[0] + d[0]);
+        double dv_i = (g[c * pitch]) * rhoginv - d[c * pitch] / d[0];
+        double dpdt = 0.0;
+        dpdt *= mu * rhoginv;
+        double dv_t = dpdt / k + (dv_i - dpdt / k) * exp(-t * k);
+        g[c * pitch] = g[0] * vstick + dv_t * mu;
+        double q = d[0] * vstick - dv_t * mu;
+        d[c * pitch] = q;
+        pdustsq += q * q;
+    }
+    g[pitch] -= .5 * pdustsq / d[0];
+}
+
 int orc_drag(double *gas, double *dust, const orc_grid *g, const orc_thermo *tg, const orc_thermo *td, double dt,
              int method, double tol, long *work)
 {
     if (method != ORC_DRAG_LOGTRAP2 && method != ORC_DRAG_LOGTRAP3 && method != ORC_DRAG_EXACT &&
-        method != ORC_DRAG_EMP && method != ORC_DRAG_RK4)
+        method != ORC_DRAG_EMP && method != ORC_DRAG_RK4 && method != ORC_DRAG_ETDRK1)
         return -105;
     double P[9];
     orc_drag_params(tg, td, P);
@@ -302,6 +328,10 @@ int orc_drag(double *gas, double *dust, const orc_grid *g, const orc_thermo *tg,
             continue;
         }
         prepare(gc, dc, pitch, &s0, &u0);
+        if (method == ORC_DRAG_ETDRK1) {
+            etdrk1_apply(gc, dc, pitch, dt, orc_drag_k(P, gc[0], dc[0], s0, u0));
+            continue;
+        }
         double f;
         if (method == ORC_DRAG_EXACT) {
             int steps;

@@ -98,6 +98,7 @@ typedef struct {
 enum orc_drag_method {
     ORC_DRAG_EMP = 0,
     ORC_DRAG_RK4 = 1,
+    ORC_DRAG_ETDRK1 = 2,
     ORC_DRAG_LOGTRAP2 = 4,
     ORC_DRAG_LOGTRAP3 = 5,
     ORC_DRAG_EXACT = 6

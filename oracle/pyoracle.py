@@ -37,7 +37,7 @@ class Thermo(ctypes.Structure):
                 ("sigma0", ctypes.c_double), ("sigmaTindex", ctypes.c_double)]
 
 
-DRAG_EMP, DRAG_RK4, DRAG_LOGTRAP2, DRAG_LOGTRAP3, DRAG_EXACT = 0, 1, 4, 5, 6
+DRAG_EMP, DRAG_RK4, DRAG_ETDRK1, DRAG_LOGTRAP2, DRAG_LOGTRAP3, DRAG_EXACT = 0, 1, 2, 4, 5, 6
 
 _lib = None
 

@@ -9,6 +9,8 @@ What pins what:
                             conservation (DustyBoxAnalyzer.m:135-140)
   LogTrap2 / LogTrap3       their convergence orders 2 and 3 against the exact
                             solution (the reference's tsDustybox.m criterion)
+  ETDRK1                    exact in the linear (Stokes, no heating) limit;
+                            local truncation order 2 in general
   EMP / RK4 on |dv|         local truncation orders 3 and 5 (global 2 and 4)
                             against the exact solution; in the linear-drag
                             limit the Taylor polynomials of exp(-x) to 2nd
@@ -126,7 +128,7 @@ def test_logtrap_convergence_orders():
         assert np.all(orders < order + 0.5), (method, errs, orders)
 
 
-@pytest.mark.parametrize("method", [po.DRAG_LOGTRAP2, po.DRAG_LOGTRAP3, po.DRAG_EXACT])
+@pytest.mark.parametrize("method", [po.DRAG_LOGTRAP2, po.DRAG_LOGTRAP3, po.DRAG_EXACT, po.DRAG_ETDRK1])
 def test_drag_invariants(method):
     tg, td = gas_dust_thermo()
     rng = np.random.default_rng(4)
@@ -145,6 +147,12 @@ def test_drag_invariants(method):
     g0, d0 = g.copy(), d.copy()
     dv0 = g0[2:] / g0[0] - d0[2:] / d0[0]
     po.drag(g, d, tg, td, 0.02, method)
+    if method == po.DRAG_ETDRK1:
+        # the reference's driven form 0/k + (dv - 0/k) e^{-kt} is NaN where the
+        # rate is cut to 0 (|dv| < 1e-9); compare the moving cells only
+        assert np.all(np.isnan(g[2:, 0, 0, :2])) and np.all(np.isnan(d[2:, 0, 0, :2]))
+        for a, b in ((g, g0), (d, d0)):
+            a[:, 0, 0, :2] = b[:, 0, 0, :2]
     # momentum per cell
     assert np.allclose(g[2:] + d[2:], g0[2:] + d0[2:], rtol=0, atol=1e-14)
     # total energy per cell: gas E + dust kinetic energy
@@ -250,3 +258,30 @@ def test_emp_rk4_cell_update(method):
         u0 = (g0[1][idx] - .5 * (g0[2:][(slice(None),) + idx] ** 2).sum() / g0[0][idx]) / g0[0][idx]
         want = po.drag_rk_magdv(P9, g0[0][idx], d0[0][idx], n0[idx], u0, dt, method)
         assert n1[idx] == pytest.approx(want, rel=1e-12)
+
+
+def test_etdrk1_decay_and_local_order():
+    """Exponential Euler (solveDragETDRK1): one step scales |dv| by exactly
+    exp(-k0 dt), k0 the rate of the initial state (so it is exact whenever the
+    rate is constant along the path); against the exact relaxation its
+    one-step error shrinks as dt^2 (a wrong rate state or a dropped heating
+    term would not)."""
+    tg, td = gas_dust_thermo()
+    P9 = po.drag_params(tg, td)
+    rhog, rhod, Pg = 1.0, 0.4, 1.0
+    g0, d0 = one_cell(rhog, [0.4, -0.1, 0.2], Pg, rhod, [-0.3, 0.1, 0.0])
+    s0 = np.linalg.norm(rel_dv(g0, d0))
+    u0 = Pg / ((tg.gamma - 1) * rhog)
+    k0 = po.drag_k(P9, rhog, rhod, s0, u0)
+    for x in (0.3, 2.0, 9.0):
+        g, d = g0.copy(), d0.copy()
+        po.drag(g, d, tg, td, x / k0, po.DRAG_ETDRK1)
+        assert np.linalg.norm(rel_dv(g, d)) == pytest.approx(s0 * np.exp(-x), rel=1e-12)
+    errs = []
+    for x in (0.2, 0.1, 0.05, 0.025):
+        exact, _ = po.drag_exact_ratio(P9, rhog, rhod, s0, u0, x / k0, 1e-14)
+        g, d = g0.copy(), d0.copy()
+        po.drag(g, d, tg, td, x / k0, po.DRAG_ETDRK1)
+        errs.append(abs(np.linalg.norm(rel_dv(g, d)) / s0 - exact))
+    orders = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert np.all(orders > 1.7) and np.all(orders < 2.4), (errs, orders)
