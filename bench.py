@@ -187,12 +187,6 @@ def run_ensemble(args):
     dust = {p: ens.quadrant_dust(states[p], seed=args.seed + 7919 * (p + 1))
             for p in range(len(sizes)) if p % 2 == 1 and not args.ensemble_no_dust}
     e = ens.Ensemble(sizes, sizes, 1.0 / sizes, 1.0 / sizes)
-    if dust:
-        params, norm, _ = L.load_model()
-        L.set_model(params, norm)
-        tg, td = L.cold_h2(), L.iron_dust()
-        tg.gamma = 1.4  # the quadrant problems' gas
-        e.set_dust([p in dust for p in range(len(sizes))], tg, td, 1.01, abi.DRAG_LEARNED)
 
     def load():
         for p, s in enumerate(states):
@@ -200,10 +194,18 @@ def run_ensemble(args):
         for p, s in dust.items():
             e.set_dust_state(p, s)
 
+    # the march segments are tuned on the hydrodynamics alone (the drag
+    # source would only add noise to the timings), then the dust is attached
     load()
     t0 = time.perf_counter()
     chosen = e.autotune(trials=2)
     tune_s = time.perf_counter() - t0
+    if dust:
+        params, norm, _ = L.load_model()
+        L.set_model(params, norm)
+        tg, td = L.cold_h2(), L.iron_dust()
+        tg.gamma = 1.4  # the quadrant problems' gas
+        e.set_dust([p in dust for p in range(len(sizes))], tg, td, 1.01, abi.DRAG_LEARNED)
     load()
     e.iteration = 0
     e.run(args.warmup, graph=True)
