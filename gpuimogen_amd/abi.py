@@ -162,6 +162,7 @@ SIGNATURES = {
     "imogen_timeloop_create": (c_int, [_MGP, c_int, _P(GeometryParams), _P(FluidStepParams), _DP, c_double,
                                        c_double, _P(c_void_p)]),
     "imogen_timeloop_run": (c_int, [c_void_p, c_int, c_int, c_int]),
+    "imogen_timeloop_set_drag": (c_int, [c_void_p, _P(ThermoDetails), _P(ThermoDetails), c_int]),
     "imogen_timeloop_clock": (c_int, [c_void_p, _DP, _DP]),
     "imogen_timeloop_set_time": (c_int, [c_void_p, c_double]),
     "imogen_timeloop_destroy": (c_int, [c_void_p]),

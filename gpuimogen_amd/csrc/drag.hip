@@ -23,9 +23,10 @@ namespace imogen {
 
 template <int METHOD>
 __global__ __launch_bounds__(256) void drag_kernel(double *__restrict__ g, double *__restrict__ d, int64_t pitch,
-                                                   int64_t n, DragConsts c, double dt,
+                                                   int64_t n, DragConsts c, DragDt dtv,
                                                    unsigned long long *__restrict__ work)
 {
+    const double dt = drag_dt(dtv);
     unsigned long long steps = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         DragCell s = drag_load(g + i, d + i, pitch);
@@ -53,8 +54,9 @@ __global__ __launch_bounds__(256) void drag_kernel(double *__restrict__ g, doubl
 // explicit midpoint (method 0) / RK4 (method 1): 72 bytes read, 64 written per cell
 template <int METHOD>
 __global__ __launch_bounds__(256) void drag_rk_kernel(double *__restrict__ g, double *__restrict__ d, int64_t pitch,
-                                                      int64_t n, DragConsts c, double dt)
+                                                      int64_t n, DragConsts c, DragDt dtv)
 {
+    const double dt = drag_dt(dtv);
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const DragCell s = drag_rk_load(g + i, d + i, pitch);
         drag_rk_apply(g + i, d + i, pitch, drag_rk_magdv<METHOD>(c, s.rg, s.rd, s.s0, s.u0, dt));
@@ -79,7 +81,7 @@ DragConsts drag_consts(const ThermoDetails *tg, const ThermoDetails *td)
     return c;
 }
 
-int launch_drag(double *g, double *d, int64_t pitch, int64_t n, const DragConsts &c, double dt, int method,
+int launch_drag(double *g, double *d, int64_t pitch, int64_t n, const DragConsts &c, DragDt dt, int method,
                 unsigned long long *work, hipStream_t s)
 {
     int64_t blocks = (n + 255) / 256;
@@ -145,8 +147,8 @@ int sourcefunction_2FluidDrag(MGArray *fluidA, MGArray *fluidB, GeometryParams *
     case DRAG_EXACT:
     case DRAG_EMP:
     case DRAG_RK4:
-    case DRAG_ETDRK1: rc = launch_drag(g, d, pg, n, c, dt, method, nullptr, st->stream); break;
-    case DRAG_LEARNED: rc = launch_learned_drag(g, d, pg, n, c, dt, st->stream); break;
+    case DRAG_ETDRK1: rc = launch_drag(g, d, pg, n, c, host_dt(dt), method, nullptr, st->stream); break;
+    case DRAG_LEARNED: rc = launch_learned_drag(g, d, pg, n, c, host_dt(dt), nullptr, st->stream); break;
     case DRAG_ETDRK2: rc = IM_FAIL(ERROR_NOIMPLEMENT, "drag method %d is not built", method); break;
     default: rc = IM_FAIL(ERROR_INVALID_ARGS, "unknown drag method %d", method);
     }

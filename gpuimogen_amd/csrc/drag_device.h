@@ -289,11 +289,29 @@ __device__ __forceinline__ void drag_rk_apply(double *g, double *d, int64_t p, d
     d[p] = .5 * dustmomfin / rhod + 1e-4 * rhod;
 }
 
+// The time interval of a drag call: a host value, or frac * dTime read on the
+// device (loops that keep dTime in device memory). Scaling by frac = 1/4 or
+// 1/2 is exact, so both forms give the same dt for the same dTime.
+struct DragDt {
+    double dt;
+    const double *p;
+    double frac;
+};
+__device__ __forceinline__ double drag_dt(const DragDt &t)
+{
+    return t.p ? t.frac * t.p[0] : t.dt;
+}
+inline DragDt host_dt(double dt)
+{
+    return DragDt{dt, nullptr, 0.0};
+}
+
 DragConsts drag_consts(const ThermoDetails *tg, const ThermoDetails *td);
-int launch_drag(double *g, double *d, int64_t pitch, int64_t n, const DragConsts &c, double dt, int method,
+int launch_drag(double *g, double *d, int64_t pitch, int64_t n, const DragConsts &c, DragDt dt, int method,
                 unsigned long long *work, hipStream_t s);
-int launch_learned_drag(double *g, double *d, int64_t pitch, int64_t n, const DragConsts &c, double dt,
-                        hipStream_t s);
+// queue: one int64 per cell, or null for the library's scratch block
+int launch_learned_drag(double *g, double *d, int64_t pitch, int64_t n, const DragConsts &c, DragDt dt,
+                        int64_t *queue, hipStream_t s);
 
 // Drag over many gas/dust array pairs (the dusty problems of an ensemble):
 // table entry t = (gas, dust, slab pitch, cells, problem index for dtp), items
@@ -309,8 +327,9 @@ struct EnsDragItem {
     int t, pad;
     int64_t base;
 };
-int launch_ens_drag(const EnsDragTable *tab, const EnsDragItem *items, int nitems, const double *dtp, double frac,
-                    const DragConsts &c, int method, int64_t *queue, unsigned long long *qcount, hipStream_t s);
+int launch_ens_drag(const EnsDragTable *tab, const EnsDragItem *items, int nitems, int64_t ncells,
+                    const double *dtp, double frac, const DragConsts &c, int method, int64_t *queue,
+                    unsigned long long *qcount, hipStream_t s);
 // changes whenever imogen_drag_set_model loads a model (captured graphs hold it by value)
 uint64_t learned_model_generation();
 

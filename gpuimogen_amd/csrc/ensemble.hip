@@ -293,6 +293,7 @@ struct EnsembleImpl {
     EnsDragTable *dragTab = nullptr;
     EnsDragItem *dragItems = nullptr;
     int nDragItems = 0;
+    int64_t dragCells = 0;  // cells of the dusty problems
     int64_t *queue = nullptr;
     unsigned long long *qcount = nullptr;
     DragConsts dc{};
@@ -370,8 +371,8 @@ static int ens_source(EnsembleImpl *e, double fraction, hipStream_t s)
 {
     if (!e->Pd) return SUCCESSFUL;
     for (int call = 0; call < 2; call++) {
-        int rc = launch_ens_drag(e->dragTab, e->dragItems, e->nDragItems, e->dtp, fraction / 2, e->dc, e->dragMethod,
-                                 e->queue, e->qcount, s);
+        int rc = launch_ens_drag(e->dragTab, e->dragItems, e->nDragItems, e->dragCells, e->dtp, fraction / 2, e->dc,
+                                 e->dragMethod, e->queue, e->qcount, s);
         if (rc) return rc;
     }
     for (int d = 1; d <= 2; d++) {
@@ -613,6 +614,7 @@ int imogen_ensemble_set_dust(ImogenEnsemble *E, const int *hasDust, double dustG
     e->dragTab = tab;
     e->dragItems = items;
     e->nDragItems = (int)hitems.size();
+    e->dragCells = cells;
     e->queue = queue;
     e->qcount = qcount;
     e->dc = drag_consts(gas, dust);

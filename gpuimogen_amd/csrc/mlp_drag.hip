@@ -285,10 +285,11 @@ __device__ __forceinline__ void learned_cells(double *__restrict__ g, double *__
 }
 
 __global__ __launch_bounds__(256) void learned_drag_kernel(double *__restrict__ g, double *__restrict__ d,
-                                                           int64_t pitch, int64_t n, DragConsts c, double dt,
+                                                           int64_t pitch, int64_t n, DragConsts c, DragDt dtv,
                                                            MlpDev m, int64_t *__restrict__ queue,
                                                            unsigned long long *__restrict__ qcount)
 {
+    const double dt = drag_dt(dtv);
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
     for (int64_t base = wave * 64; base < n; base += nwaves * 64)
@@ -296,13 +297,17 @@ __global__ __launch_bounds__(256) void learned_drag_kernel(double *__restrict__ 
 }
 
 __global__ __launch_bounds__(256) void fallback_drag_kernel(double *__restrict__ g, double *__restrict__ d,
-                                                            int64_t pitch, DragConsts c, double dt,
+                                                            int64_t pitch, int64_t n, DragConsts c, DragDt dtv,
                                                             const int64_t *__restrict__ queue,
                                                             const unsigned long long *__restrict__ qcount,
                                                             unsigned long long *__restrict__ counts)
 {
+    const double dt = drag_dt(dtv);
     const unsigned long long nq = *qcount;
-    if (blockIdx.x == 0 && threadIdx.x == 0 && nq) atomicAdd(counts + 1, nq);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        atomicAdd(counts, (unsigned long long)n);
+        if (nq) atomicAdd(counts + 1, nq);
+    }
     for (unsigned long long t = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; t < nq;
          t += (unsigned long long)gridDim.x * blockDim.x) {
         const int64_t i = queue[t];
@@ -353,9 +358,14 @@ __global__ __launch_bounds__(256) void ens_drag_kernel(const EnsDragTable *__res
 __global__ __launch_bounds__(256) void ens_fallback_kernel(const EnsDragTable *__restrict__ tab,
                                                            const double *__restrict__ dtp, double frac, DragConsts c,
                                                            const int64_t *__restrict__ queue,
-                                                           const unsigned long long *__restrict__ qcount)
+                                                           const unsigned long long *__restrict__ qcount,
+                                                           int64_t ncells, unsigned long long *__restrict__ counts)
 {
     const unsigned long long nq = *qcount;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        atomicAdd(counts, (unsigned long long)ncells);
+        if (nq) atomicAdd(counts + 1, nq);
+    }
     for (unsigned long long t = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; t < nq;
          t += (unsigned long long)gridDim.x * blockDim.x) {
         const int64_t e = queue[t];
@@ -641,11 +651,10 @@ struct HostModel {
     void *buf = nullptr;  // device: w1t | w2t | w3t | b1 | b2 | b3
     MlpDev dev{};
     bool loaded = false;
-    unsigned long long *counts = nullptr;  // device [unused, fallbacks, queue length, unused]
+    unsigned long long *counts = nullptr;  // device [learned cells, fallbacks, queue length, unused]
 };
 static HostModel g_model;
 static uint64_t g_modelGeneration = 0;  // bumped by every imogen_drag_set_model
-static int64_t g_learnedCells = 0;  // cells through the learned drag since the last stats read
 
 static int ensure_counts()
 {
@@ -658,15 +667,15 @@ static int ensure_counts()
     return IM_HIP(hipStreamSynchronize(st->stream));
 }
 
-int launch_learned_drag(double *g, double *d, int64_t pitch, int64_t n, const DragConsts &c, double dt,
-                        hipStream_t s)
+int launch_learned_drag(double *g, double *d, int64_t pitch, int64_t n, const DragConsts &c, DragDt dt,
+                        int64_t *queue, hipStream_t s)
 {
     if (!g_model.loaded) return IM_FAIL(ERROR_INVALID_ARGS, "learned drag: no model (imogen_drag_set_model)");
     if (ensure_counts()) return ERROR_CUDA_BLEW_UP;
     int64_t waves = (n + 63) / 64;
     if (waves > 8192) waves = 8192;
     const int64_t blocks = (waves + 3) / 4;
-    int64_t *queue = (int64_t *)scratch((size_t)n * sizeof(int64_t));
+    if (!queue) queue = (int64_t *)scratch((size_t)n * sizeof(int64_t));
     if (!queue) return IM_FAIL(ERROR_NOMEM, "no scratch for the drag fallback queue");
     unsigned long long *qcount = g_model.counts + 2;
     const int slot = prof_begin(PROF_MLP, s);
@@ -677,17 +686,15 @@ int launch_learned_drag(double *g, double *d, int64_t pitch, int64_t n, const Dr
                        g_model.dev, queue, qcount);
     rc = IM_LAUNCHED("learned_drag_kernel");
     if (rc) return rc;
-    hipLaunchKernelGGL(fallback_drag_kernel, dim3(1024), dim3(256), 0, s, g, d, pitch, c, dt, queue, qcount,
+    hipLaunchKernelGGL(fallback_drag_kernel, dim3(1024), dim3(256), 0, s, g, d, pitch, n, c, dt, queue, qcount,
                        g_model.counts);
     prof_end(slot, s);
-    rc = IM_LAUNCHED("fallback_drag_kernel");
-    if (rc) return rc;
-    g_learnedCells += n;
-    return SUCCESSFUL;
+    return IM_LAUNCHED("fallback_drag_kernel");
 }
 
-int launch_ens_drag(const EnsDragTable *tab, const EnsDragItem *items, int nitems, const double *dtp, double frac,
-                    const DragConsts &c, int method, int64_t *queue, unsigned long long *qcount, hipStream_t s)
+int launch_ens_drag(const EnsDragTable *tab, const EnsDragItem *items, int nitems, int64_t ncells,
+                    const double *dtp, double frac, const DragConsts &c, int method, int64_t *queue,
+                    unsigned long long *qcount, hipStream_t s)
 {
     if (method != DRAG_LOGTRAP2 && method != DRAG_LOGTRAP3 && method != DRAG_EXACT && method != DRAG_LEARNED)
         return IM_FAIL(ERROR_NOIMPLEMENT, "ensemble drag method %d", method);
@@ -703,7 +710,9 @@ int launch_ens_drag(const EnsDragTable *tab, const EnsDragItem *items, int nitem
                        frac, c, method, g_model.dev, queue, qcount);
     int rc = IM_LAUNCHED("ens_drag_kernel");
     if (rc || method != DRAG_LEARNED) return rc;
-    hipLaunchKernelGGL(ens_fallback_kernel, dim3(1024), dim3(256), 0, s, tab, dtp, frac, c, queue, qcount);
+    if (ensure_counts()) return ERROR_CUDA_BLEW_UP;
+    hipLaunchKernelGGL(ens_fallback_kernel, dim3(1024), dim3(256), 0, s, tab, dtp, frac, c, queue, qcount, ncells,
+                       g_model.counts);
     return IM_LAUNCHED("ens_fallback_kernel");
 }
 
@@ -778,9 +787,8 @@ int imogen_drag_learned_stats(int64_t *cells, int64_t *fallbacks)
         return ERROR_CUDA_BLEW_UP;
     if (IM_HIP(hipMemsetAsync(g_model.counts, 0, sizeof(h), st->stream))) return ERROR_CUDA_BLEW_UP;
     if (IM_HIP(hipStreamSynchronize(st->stream))) return ERROR_CUDA_BLEW_UP;
-    if (cells) *cells = g_learnedCells;
+    if (cells) *cells = (int64_t)h[0];
     if (fallbacks) *fallbacks = (int64_t)h[1];
-    g_learnedCells = 0;
     return SUCCESSFUL;
 }
 

@@ -1,12 +1,16 @@
 // timeloop.hip -- Imogen iterations with the time step kept on the device.
 //
-// One iteration is what imogen.m:115-139 runs for hydrodynamics:
+// One iteration is what imogen.m:115-139 runs:
 //     TimeManager.update (time/TimeManager.m:124-186): tau = min over fluids of
 //         the CFL limit (cflTimestep.cu:107-377); dTime = CFL * tau, clamped so
 //         that t + 2 dTime does not pass tMax; t += 2 dTime
-//     fluidstep(+1) for every fluid, then fluidstep(-1) for every fluid
-//         (flux.cu:27-155: three directional sweeps, each followed by its
-//         boundary conditions)
+//     source(1/2); fluidstep(+1) for every fluid; source(1); fluidstep(-1) for
+//         every fluid; source(1/2)
+//         (fluidstep: flux.cu:27-155, three directional sweeps, each followed
+//         by its boundary conditions; source(f): sources/src0001.m, two
+//         gas-dust drag calls over f dTime / 2 each, then the boundary
+//         conditions of both fluids in every direction -- only when a drag
+//         method is set, imogen_timeloop_set_drag)
 // The host-driven form of this loop reads tau back after every CFL reduction
 // (cflTimestep.cu:214 synchronises), so the GPU idles while the host issues
 // the next sweeps. Here the CFL partial minima, dTime and the per-direction
@@ -18,6 +22,7 @@
 #include <cstring>
 
 #include "cfl_device.h"
+#include "drag_device.h"
 #include "imogen_internal.h"
 
 namespace imogen {
@@ -61,6 +66,10 @@ struct TimeloopImpl {
     // method and boundaries allow it, the per-wave minima and their capacity
     bool fuse[kMaxFluids] = {false};
     CflFuse cf[kMaxFluids];
+    // drag source coupling fluid 0 (gas) and fluid 1 (dust); -1 = none
+    int dragMethod = -1;
+    DragConsts dc{};
+    uint64_t modelGen = 0;  // learned model the graphs were captured with
 };
 
 }  // namespace imogen
@@ -120,6 +129,33 @@ static CflArgs cfl_args(const TimeloopImpl *L, int f)
     return CflArgs{gamma * (gamma - 1), g.h[0], g.h[1], g.h[2], hmin, 1, dim, 0};
 }
 
+// sources/src0001.m: the drag over fraction * dTime as two calls of half that
+// interval (dt read from the device clock), then the boundary conditions of
+// both fluids in the directions FluidManager.setBoundaries visits
+static int source(TimeloopImpl *L, double fraction, hipStream_t s)
+{
+    if (L->dragMethod < 0) return SUCCESSFUL;
+    double *g, *d;
+    int64_t pg, pd;
+    int rc = fluid_view(L->fluid[0], &g, &pg);
+    if (!rc) rc = fluid_view(L->fluid[1], &d, &pd);
+    if (rc) return rc;
+    const int64_t n = (int64_t)L->fluid[0][0].dim[0] * L->fluid[0][0].dim[1] * L->fluid[0][0].dim[2];
+    const DragDt dt{0.0, L->dbuf + 1, fraction / 2};
+    for (int call = 0; call < 2; call++) {
+        const int slot = prof_begin(PROF_DRAG, s);
+        rc = (L->dragMethod == DRAG_LEARNED) ? launch_learned_drag(g, d, pg, n, L->dc, dt, nullptr, s)
+                                             : launch_drag(g, d, pg, n, L->dc, dt, L->dragMethod, nullptr, s);
+        prof_end(slot, s);
+        if (rc) return rc;
+    }
+    for (int f = 0; f < 2; f++)
+        for (int dir = 1; dir <= 3; dir++)
+            if (L->geo.globalRez[dir - 1] > 1 || dir == 1)
+                if ((rc = setFluidBoundary(L->fluid[f], &L->geo, dir))) return rc;
+    return SUCCESSFUL;
+}
+
 // fusedCfl: take this iteration's CFL from the minima the previous iteration's
 // last sweep left (fluids that allow it); otherwise a full CFL pass.
 static int iteration(TimeloopImpl *L, int it, bool fusedCfl, hipStream_t s)
@@ -138,7 +174,8 @@ static int iteration(TimeloopImpl *L, int it, bool fusedCfl, hipStream_t s)
     hipLaunchKernelGGL(clock_advance, dim3(1), dim3(64), 0, s, L->dbuf + kClockDoubles, L->nf, L->cfl, L->tmax,
                        L->geo.h[0], L->geo.h[1], L->geo.h[2], L->dbuf);
     if ((rc = IM_LAUNCHED("clock_advance"))) return rc;
-    for (int sgn = 1; sgn >= -1; sgn -= 2)
+    if ((rc = source(L, 0.5, s))) return rc;
+    for (int sgn = 1; sgn >= -1; sgn -= 2) {
         for (int f = 0; f < L->nf; f++) {
             FluidStepParams p = L->p[f];
             p.stepNumber = it;
@@ -147,6 +184,8 @@ static int iteration(TimeloopImpl *L, int it, bool fusedCfl, hipStream_t s)
             const CflFuse *cf = (sgn < 0 && L->fuse[f]) ? &L->cf[f] : nullptr;
             if ((rc = update_3d(L->fluid[f], p, L->dbuf + 2, cf))) return rc;
         }
+        if ((rc = source(L, sgn > 0 ? 1.0 : 0.5, s))) return rc;
+    }
     return SUCCESSFUL;
 }
 
@@ -259,6 +298,10 @@ int imogen_timeloop_run(ImogenTimeloop *T, int firstIteration, int iterations, i
             L->scratchAt = st->scratch;
             L->scratchBytes = st->scratchBytes;
         }
+        if (L->dragMethod == DRAG_LEARNED && L->modelGen != learned_model_generation()) {
+            drop_graphs(L);  // the graphs hold the network's weights by value
+            L->modelGen = learned_model_generation();
+        }
         hipGraphExec_t &g = L->graph[((it % 6) + 6) % 6];
         if (!g) {
             // eager run of this iteration: sizes the scratch block (no allocation
@@ -289,6 +332,47 @@ int imogen_timeloop_run(ImogenTimeloop *T, int firstIteration, int iterations, i
         }
         if (IM_HIP(hipGraphLaunch(g, st->stream))) return ERROR_CUDA_BLEW_UP;
     }
+    return SUCCESSFUL;
+}
+
+int imogen_timeloop_set_drag(ImogenTimeloop *T, const ThermoDetails *gas, const ThermoDetails *dust, int method)
+{
+    if (!T) return IM_FAIL(ERROR_NULL_POINTER, "null time loop");
+    TimeloopImpl *L = &T->impl;
+    if (method < 0) {
+        L->dragMethod = -1;
+        drop_graphs(L);
+        return SUCCESSFUL;
+    }
+    if (!gas || !dust) return IM_FAIL(ERROR_NULL_POINTER, "null ThermoDetails");
+    if (L->nf != 2) return IM_FAIL(ERROR_INVALID_ARGS, "the drag source couples exactly two fluids (got %d)", L->nf);
+    switch (method) {
+    case DRAG_EMP:
+    case DRAG_RK4:
+    case DRAG_ETDRK1:
+    case DRAG_LOGTRAP2:
+    case DRAG_LOGTRAP3:
+    case DRAG_EXACT:
+    case DRAG_LEARNED: break;
+    case DRAG_ETDRK2: return IM_FAIL(ERROR_NOIMPLEMENT, "drag method %d is not built", method);
+    default: return IM_FAIL(ERROR_INVALID_ARGS, "unknown drag method %d", method);
+    }
+    double *g, *d;
+    int64_t pg, pd;
+    int rc = fluid_view(L->fluid[0], &g, &pg);
+    if (!rc) rc = fluid_view(L->fluid[1], &d, &pd);
+    if (rc) return rc;
+    for (int i = 0; i < 3; i++)
+        if (L->fluid[0][0].dim[i] != L->fluid[1][0].dim[i])
+            return IM_FAIL(ERROR_INVALID_ARGS, "gas and dust grids differ in dimension %d", i + 1);
+    if (pg != pd) return IM_FAIL(ERROR_INVALID_ARGS, "gas and dust slab pitches differ");
+    L->dragMethod = method;
+    L->dc = drag_consts(gas, dust);
+    L->modelGen = learned_model_generation();
+    // the last source changes the state after the last sweep, so the next
+    // CFL limit needs its own pass
+    for (int f = 0; f < L->nf; f++) L->fuse[f] = false;
+    drop_graphs(L);
     return SUCCESSFUL;
 }
 

@@ -67,13 +67,16 @@ class Simulation:
 
 
 class DeviceLoop:
-    """Hydrodynamic iterations of Simulation (no sources) with dTime kept on
-    the device and each iteration replayed from a HIP graph
-    (imogen_timeloop_*). Bitwise the same results as Simulation.step()."""
+    """Iterations of Simulation (with the drag source when drag_method is
+    set) with dTime kept on the device and each iteration replayed from a HIP
+    graph (imogen_timeloop_*). Bitwise the same results as Simulation.step()."""
 
-    def __init__(self, fluids, methods, geometry, cfl=0.4, limiter=abi.LIMITER_OSPRE, t_max=None):
+    def __init__(self, fluids, methods, geometry, cfl=0.4, limiter=abi.LIMITER_OSPRE, t_max=None,
+                 drag_method=None):
         if len(methods) != len(fluids):
             raise ValueError("one fluid method per fluid")
+        if drag_method is not None and len(fluids) != 2:
+            raise ValueError("the drag source couples exactly two fluids (gas, dust)")
         self.fluids = list(fluids)
         nf = len(fluids)
         arr = (abi.MGArray * (5 * nf))()
@@ -91,6 +94,9 @@ class DeviceLoop:
                  float(t_max) if t_max is not None else 0.0, ctypes.byref(h))
         self.h = h
         self.iteration = 0
+        if drag_method is not None:
+            abi.call("imogen_timeloop_set_drag", self.h, ctypes.byref(fluids[0].thermoDetails),
+                     ctypes.byref(fluids[1].thermoDetails), int(drag_method))
 
     def run(self, iterations, graph=True):
         abi.call("imogen_timeloop_run", self.h, self.iteration, int(iterations), 1 if graph else 0)

@@ -353,9 +353,10 @@ int imogen_ensemble_destroy(ImogenEnsemble *e);
 /* Time loop on the device                                                  */
 /* ======================================================================== */
 
-/* The iteration of imogen.m:115-139 for hydrodynamics (TimeManager.update,
+/* The iteration of imogen.m:115-139 (TimeManager.update,
  * time/TimeManager.m:124-186, then fluidstep(+1) and fluidstep(-1) of every
- * fluid, flux.cu:27-155) with dTime kept in device memory: no host round trip
+ * fluid, flux.cu:27-155, and the drag source when imogen_timeloop_set_drag
+ * adds it) with dTime kept in device memory: no host round trip
  * per CFL reduction (cflTimestep.cu:214), one replayed HIP graph per
  * iteration. The host-driven equivalent is imogen_cflTimestepFused on every
  * fluid, dTime = cfl * min(tau) (clamped so t + 2 dTime <= tmax when
@@ -375,6 +376,15 @@ int imogen_timeloop_create(MGArray *fluids, int nfluids, const GeometryParams *g
  * picks the sweep order as stepNumber does); returns without waiting. */
 int imogen_timeloop_run(ImogenTimeloop *loop, int firstIteration, int iterations, int useGraph);
 /* Simulation time and the last dTime (waits). */
+/* Adds the gas-dust drag source of sources/src0001.m to every iteration:
+ * source(1/2), fluidstep(+1), source(1), fluidstep(-1), source(1/2)
+ * (imogen.m:118-127), each source two sourcefunction_2FluidDrag calls over
+ * half its interval followed by setFluidBoundary of both fluids. Fluid 0 is
+ * the gas, fluid 1 the dust; method as in sourcefunction_2FluidDrag (the
+ * learned method needs imogen_drag_set_model first; graphs are re-captured
+ * when the model changes). method < 0 removes the source. The results are
+ * bitwise those of the host-driven loop with the same calls. */
+int imogen_timeloop_set_drag(ImogenTimeloop *loop, const ThermoDetails *gas, const ThermoDetails *dust, int method);
 int imogen_timeloop_clock(ImogenTimeloop *loop, double *time, double *dt);
 int imogen_timeloop_set_time(ImogenTimeloop *loop, double time);
 int imogen_timeloop_destroy(ImogenTimeloop *loop);

@@ -6,7 +6,10 @@ states and clocks must agree bitwise, eager or replayed from graphs. The loop
 folds the CFL minimum into each iteration's last sweep where the boundaries
 allow it (const with the +y/+z innermost-ghost convention, mirror, circular;
 HLLC and HLL) and takes a full CFL pass otherwise (Xin-Jin) and on the first
-iteration of every run call; the cases cover each."""
+iteration of every run call; the cases cover each. With a drag source the
+loop runs imogen.m's source(1/2), fluidstep(+1), source(1), fluidstep(-1),
+source(1/2) with the drag interval read from the device clock, for every
+drag method that is built."""
 import numpy as np
 import pytest
 
@@ -60,6 +63,73 @@ def test_device_loop_matches_host_loop(kind, graph):
         assert np.array_equal(a.state, b.state)
     loop.free()
     for f in fa + fb:
+        f.free()
+
+
+def make_gasdust(n=16, seed=6, grain_mass=1.0, isothermal_dust=False):
+    from gpuimogen_amd import abi, gpuclass as gc, learned_drag as L, problems
+    gas, dust, h, ciso = problems.gasdust3d(n, seed=seed)
+    tg, td = L.cold_h2(), L.iron_dust()
+    td.m *= grain_mass
+    fl = [gc.FluidManager(gas, tg.gamma, minmass=1e-8, bc_modes=(abi.circular,) * 6, thermoDetails=tg),
+          gc.FluidManager(dust, td.gamma, minmass=1e-8, bc_modes=(abi.circular,) * 6, thermoDetails=td,
+                          cisothermal=ciso if isothermal_dust else -1.0)]
+    methods = [abi.METHOD_HLLC, abi.METHOD_HLL if isothermal_dust else abi.METHOD_HLLC]
+    return fl, methods, gc.GeometryManager((n, n, n), h)
+
+
+@pytest.mark.parametrize("method", ["EMP", "RK4", "ETDRK1", "LOGTRAP2", "LOGTRAP3", "EXACT", "LEARNED"])
+@pytest.mark.parametrize("graph", [False, True])
+def test_device_loop_with_drag_matches_host_loop(method, graph):
+    """source(1/2), fluidstep(+1), source(1), fluidstep(-1), source(1/2) with
+    the drag interval read from the device clock: the same kernels and the
+    same dt as the host loop's src0001 calls, so bitwise the same states."""
+    from gpuimogen_amd import abi, imogen, learned_drag as L
+    drag = getattr(abi, "DRAG_" + method)
+    if drag == abi.DRAG_LEARNED:
+        params, norm, _ = L.load_model()
+        L.set_model(params, norm)
+    iters = 7
+    # EMP and RK4 rebuild the dust energy as its kinetic energy + 1e-4 rho
+    # (cukern_applyFinalDeltaV: pressureless dust), which an HLLC dust sweep
+    # turns negative; they are run with the isothermal HLL dust. At 16^3 the
+    # drag rate reaches k dt ~ 3, beyond these explicit integrators: they get
+    # a 20x heavier grain (20x smaller rate). The oracle shows the same.
+    explicit = drag in (abi.DRAG_EMP, abi.DRAG_RK4)
+    mass = 20.0 if explicit else 1.0
+    fa, methods, gm = make_gasdust(grain_mass=mass, isothermal_dust=explicit)
+    fb, _, _ = make_gasdust(grain_mass=mass, isothermal_dust=explicit)
+    print(f"gas+dust 16^3, drag {method}: {iters} iterations, graph={graph}", flush=True)
+    sim = imogen.Simulation(fa, methods, gm, cfl=0.4, drag_method=drag)
+    for _ in range(iters):
+        sim.step()
+    L.learned_stats()
+    loop = imogen.DeviceLoop(fb, methods, gm, cfl=0.4, drag_method=drag)
+    loop.run(2, graph=graph)
+    loop.run(iters - 2, graph=graph)
+    t, _ = loop.clock()
+    assert t == sim.time
+    for a, b in zip(fa, fb):
+        assert np.all(np.isfinite(a.state))
+        assert np.array_equal(a.state, b.state)
+    if drag == abi.DRAG_LEARNED:
+        # every source call is two drag calls over all cells, replays included
+        cells, _ = L.learned_stats()
+        assert cells == 6 * iters * 16 ** 3
+    loop.free()
+    for f in fa + fb:
+        f.free()
+
+
+def test_device_loop_drag_needs_two_fluids():
+    from gpuimogen_amd import abi, imogen
+    fl, methods, gm, lim = make_fluids("sedov")
+    with pytest.raises(ValueError):
+        imogen.DeviceLoop(fl, methods, gm, drag_method=abi.DRAG_LOGTRAP2)
+    fa, methods, gm = make_gasdust(8)
+    with pytest.raises(abi.ImogenError):
+        imogen.DeviceLoop(fa, methods, gm, drag_method=abi.DRAG_ETDRK2)
+    for f in fl + fa:
         f.free()
 
 
