@@ -212,7 +212,7 @@ __global__ __launch_bounds__(256) void ens_sweep_y(const EnsProblem *__restrict_
     const bool colValid = (x < P.nx) && (s < S);
     const int N = P.ny;
     const int seglen = (N + S - 1) / S;
-    const int j0 = s * seglen;
+    const int j0 = colValid ? s * seglen : 0;  // lanes of no column: zero-length segment at row 0
     int L = N - j0;
     L = L < seglen ? L : seglen;
     if (!colValid) L = 0;

@@ -30,6 +30,34 @@ namespace imogen {
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
+
+// Segments per column for the Y/Z march, S in {1, 2, 4} (W = 64/S columns per
+// wave, so every lane has a column slot). The kernel's time follows the
+// busiest SIMD: with w waves over 1024 SIMDs (256 CUs x 4, up to 2 resident
+// waves each) it runs ceil(w / 1024) waves of seglen + 9 steps, and a launch
+// with fewer waves than SIMDs runs about 1.8x slower per step (idle SIMDs,
+// no latency hiding). Segments shorter than 16 cells are not used. Measured
+// Y / Z times (HLLC, ms) and the rule's pick:
+//   96^3  S=1 0.140  S=2 0.077  S=4 0.051  -> 4     320^3 S=1 1.04  S=2 0.79  S=4 0.79  -> 4
+//   160^3 S=1 0.216  S=2 0.201  S=4 0.125  -> 4     384^3 S=1 1.37  S=2 1.24  S=4 1.26  -> 4
+//   192^3 S=1 0.249  S=2 0.236  S=4 0.203  -> 4     256^3 S=1 0.315 S=2 0.309..0.329 S=4 0.324 -> 1
+static int march_segments(int64_t columns, int N)
+{
+    int best = 1;
+    double bestCost = 0.0;
+    for (int S = 1; S <= 4; S *= 2) {
+        if (S > 1 && N < 16 * S) break;
+        const int64_t waves = (columns * S + 63) / 64;
+        const int64_t rounds = (waves + 1023) / 1024;
+        const double cost = (double)rounds * ((N + S - 1) / S + 9) * (waves < 1024 ? 1.8 : 1.0);
+        if (S == 1 || cost < bestCost) {
+            best = S;
+            bestCost = cost;
+        }
+    }
+    return best;
+}
+
 template <class M, int LIM>
 static int launch_x(double *q, int64_t pitch, int N, int64_t nlines, double lambda, const double *lamp,
                     const HydroConsts &hc, int rt, const CflFuse &cf, hipStream_t s)
@@ -96,12 +124,7 @@ static int launch_march(double *q, int64_t pitch, int nx, int ny, int nz, int di
     const int64_t ostride = (dir == 2) ? (int64_t)nx * ny : (int64_t)nx;
     const int nother = (dir == 2) ? nz : ny;
     const int64_t columns = (int64_t)nx * nother;
-    // Segments per column: enough for one wave per SIMD (256 CUs x 4 SIMDs).
-    // More segments cost more than the occupancy gains: each adds halo
-    // re-reads and deferred outputs, and at 256^3 one segment per column
-    // measured 0.311 ms (Z) against 0.351 ms with two.
-    int S = 1;
-    while (S < 4 && columns * S < 1024 * 64 && N >= 32 * S) S *= 2;
+    const int S = march_segments(columns, N);
     const int seglen = (N + S - 1) / S;
     const int W = 64 / S;
     const int wavesPerRow = (nx + W - 1) / W;
@@ -175,8 +198,7 @@ int64_t sweep_waves(int nx, int ny, int nz, int dir)
     const int N = (dir == 2) ? ny : nz;
     const int nother = (dir == 2) ? nz : ny;
     const int64_t columns = (int64_t)nx * nother;
-    int S = 1;
-    while (S < 4 && columns * S < 1024 * 64 && N >= 32 * S) S *= 2;
+    const int S = march_segments(columns, N);
     const int W = 64 / S;
     const int wavesPerRow = (nx + W - 1) / W;
     const int blocksPerRow = (wavesPerRow + 3) / 4;

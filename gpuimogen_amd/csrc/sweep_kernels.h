@@ -232,9 +232,8 @@ __device__ __forceinline__ void march_body(double *__restrict__ col, int64_t pit
                                            int rt, double *__restrict__ defer, int64_t gtid, int64_t nthr,
                                            double (*ring)[5][64], int lane, const CflFuse &cf, double &cmin)
 {
-    // wrapped sweep index of the first load (m = j0 - 4)
-    int mw = j0 - 4;
-    if (mw < 0) mw += N;
+    // wrapped sweep index of the first load (m = j0 - 4), in [0, N)
+    int mw = (j0 - 4) % N;
     if (mw < 0) mw += N;
     double *outp = col + (int64_t)(j0 - 9) * sstride;  // address of output t - 9
     double *const dfr = defer + gtid;
@@ -369,7 +368,10 @@ __global__ __launch_bounds__(256) void sweep_march(double *__restrict__ q, int64
     }
     const int64_t x = wrow * W + c;
     const bool colValid = (x < nx) && (s < S);
-    const int j0 = s * seglen;
+    // lanes of no column (x past the grid, or s >= S when S does not divide
+    // 64) march a zero-length segment from row 0: their unconditional loads
+    // then stay inside the column
+    const int j0 = colValid ? s * seglen : 0;
     int L = N - j0;
     L = L < seglen ? L : seglen;
     if (!colValid) L = 0;
