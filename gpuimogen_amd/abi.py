@@ -96,9 +96,23 @@ class DragModelNorm(ctypes.Structure):
                 ("residTol", ctypes.c_float)]
 
 
+class GravityData(ctypes.Structure):
+    _fields_ = [("phi", ctypes.POINTER(MGArray)), ("spaceOrder", c_int), ("timeOrder", c_int)]
+
+
+class GridFluid(ctypes.Structure):
+    _fields_ = [("DataHolder", MGArray), ("data", MGArray * 5), ("thermo", ThermoDetails),
+                ("bcond", BCSettings * 5), ("rhoMin", c_double)]
+
+
+class ParametricRadiation(ctypes.Structure):
+    _fields_ = [("exponent", c_double), ("prefactor", c_double), ("minTemperature", c_double)]
+
+
 STRUCTS = {"BCSettings": BCSettings, "MGArray": MGArray, "GeometryParams": GeometryParams,
            "ThermoDetails": ThermoDetails, "FluidStepParams": FluidStepParams,
-           "ParallelTopology": ParallelTopology, "DragModelNorm": DragModelNorm}
+           "ParallelTopology": ParallelTopology, "DragModelNorm": DragModelNorm, "GravityData": GravityData,
+           "GridFluid": GridFluid, "ParametricRadiation": ParametricRadiation}
 
 _P = ctypes.POINTER
 _DP = _P(c_double)
@@ -136,6 +150,8 @@ SIGNATURES = {
     "calculateSoundspeed": (c_int, [_MGP, _MGP, _MGP, c_double]),
     "computeLocalCFLTimestep": (c_int, [_MGP, _MGP, _P(GeometryParams), c_int, _DP, _DP]),
     "imogen_cflTimestepFused": (c_int, [_MGP, _P(GeometryParams), c_int, c_double, _DP, _DP]),
+    "performSourceFunctions": (c_int, [c_int, _P(GridFluid), c_int, FluidStepParams, _P(ParallelTopology),
+                                       _P(GravityData), _P(ParametricRadiation), _MGP]),
     "sourcefunction_2FluidDrag": (c_int, [_MGP, _MGP, _P(GeometryParams), _P(ThermoDetails),
                                           _P(ThermoDetails), c_double, c_int]),
     "imogen_drag_set_model": (c_int, [_FP, _P(DragModelNorm)]),

@@ -149,11 +149,34 @@ int sourcefunction_2FluidDrag(MGArray *fluidA, MGArray *fluidB, GeometryParams *
     case DRAG_RK4:
     case DRAG_ETDRK1: rc = launch_drag(g, d, pg, n, c, host_dt(dt), method, nullptr, st->stream); break;
     case DRAG_LEARNED: rc = launch_learned_drag(g, d, pg, n, c, host_dt(dt), nullptr, st->stream); break;
-    case DRAG_ETDRK2: rc = IM_FAIL(ERROR_NOIMPLEMENT, "drag method %d is not built", method); break;
+    case DRAG_ETDRK2:  // listed as not implemented by the reference itself (classes/ImogenManager.m:181)
+        rc = IM_FAIL(ERROR_NOIMPLEMENT, "drag method %d is not built", method);
+        break;
     default: rc = IM_FAIL(ERROR_INVALID_ARGS, "unknown drag method %d", method);
     }
     prof_end(slot, st->stream);
     return rc;
+}
+
+int performSourceFunctions(int srcType, GridFluid *fluids, int nFluids, FluidStepParams fsp, ParallelTopology *topo,
+                           GravityData *gravdata, ParametricRadiation *prad, MGArray *tmpStorage)
+{
+    (void)gravdata;
+    (void)tmpStorage;
+    if (!prad) return IM_FAIL(ERROR_NULL_POINTER, "null ParametricRadiation");
+    int rc = check_topology(topo);
+    if (rc) return rc;
+    if (srcType < 0 || srcType > 15) return IM_FAIL(ERROR_INVALID_ARGS, "source type %d", srcType);
+    if (srcType != 0 && srcType != 8)
+        return IM_FAIL(ERROR_NOIMPLEMENT, "source type %d (cylindrical / rotating-frame / gravity terms) is not built",
+                       srcType);
+    if (prad->prefactor > 0) return IM_FAIL(ERROR_NOIMPLEMENT, "optically thin radiation is not built");
+    if (srcType == 0) return SUCCESSFUL;  // srcBlank, sourceStep.cu:293-297
+    // src___D without radiation (sourceStep.cu:300-326): one full-dt drag call
+    if (!fluids) return IM_FAIL(ERROR_NULL_POINTER, "null fluids");
+    if (nFluids < 2) return IM_FAIL(ERROR_INVALID_ARGS, "two-fluid drag needs 2 fluids (got %d)", nFluids);
+    return sourcefunction_2FluidDrag(&fluids[0].data[0], &fluids[1].data[0], &fsp.geometry, &fluids[0].thermo,
+                                     &fluids[1].thermo, fsp.dt, fsp.multifluidDragMethod);
 }
 
 }  // extern "C"

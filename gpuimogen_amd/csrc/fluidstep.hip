@@ -31,12 +31,7 @@ int imogen::fluid_view(MGArray *fluid, double **base, int64_t *pitch)
     return SUCCESSFUL;
 }
 
-namespace {
-
-// fluxcall table of gpuclass/flux.cu:46: direction swept at position n of sweep s
-const int kFluxcall[3][6] = {{1, 2, 1, 2, 3, 3}, {3, 1, 2, 3, 1, 2}, {2, 3, 3, 1, 2, 1}};
-
-int check_topology(const ParallelTopology *t)
+int imogen::check_topology(const ParallelTopology *t)
 {
     if (!t) return SUCCESSFUL;
     for (int i = 0; i < 3; i++)
@@ -44,6 +39,11 @@ int check_topology(const ParallelTopology *t)
             return IM_FAIL(ERROR_NOIMPLEMENT, "multi-rank topologies are not built (nproc[%d] = %d)", i, t->nproc[i]);
     return SUCCESSFUL;
 }
+
+namespace {
+
+// fluxcall table of gpuclass/flux.cu:46: direction swept at position n of sweep s
+const int kFluxcall[3][6] = {{1, 2, 1, 2, 3, 3}, {3, 1, 2, 3, 1, 2}, {2, 3, 3, 1, 2, 1}};
 
 // sas code of one array for one face (cudaStatics.cu:226-250)
 int sas_code(const MGArray *a, int direction, int side, int *code)

@@ -51,6 +51,8 @@ void *scratch(size_t bytes);
 // Validates a 5-array fluid canister (slab views of one 5-slab owner, XYZ
 // order, one GPU) and returns its base pointer and slab pitch in doubles.
 int fluid_view(MGArray *fluid, double **base, int64_t *pitch);
+// ERROR_NOIMPLEMENT for topologies of more than one rank (null: one rank).
+int check_topology(const ParallelTopology *t);
 
 // performFluidUpdate_1D / _3D without the topology check. lampBase: when not
 // null, the sweeps of direction d read lambda from lampBase[d - 1] (device).

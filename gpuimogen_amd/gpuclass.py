@@ -288,6 +288,46 @@ def cudaSource2FluidDrag(fluids, geometry, params):
          int(params[1]))
 
 
+def sourceStep(fluids, geometry, dTime, params, multifluidDragMethod=abi.DRAG_LOGTRAP2, radiation=None):
+    """sourceStep(run, fluid, bx, by, bz, xyvector, [order, step, method, tFraction])
+    (gpuclass/sourceStep.cu:71-193, the MEX form of performSourceFunctions):
+    dt = dTime * tFraction, srcType = cylindrical + 2 rotatingFrame + 8 (more
+    than one fluid) -- no gravity potential here -- then
+    performSourceFunctions (sourceStep.cu:200-226). With two fluids that is the
+    drag source over the whole dt with multifluidDragMethod (fluids[0] gas,
+    fluids[1] dust). radiation: (exponent, strength) or None; any strength
+    raises ERROR_NOIMPLEMENT, as do the cylindrical and rotating-frame types."""
+    if len(params) != 4:
+        raise ValueError("need [order, step #, step method, tFraction]")
+    order, step, method, tFraction = params
+    dt = float(dTime) * float(tFraction)
+    n = len(fluids)
+    gf = (abi.GridFluid * n)()
+    for i, f in enumerate(fluids):
+        arr = f.arrays()
+        gf[i].DataHolder = arr[0]
+        for k in range(5):
+            gf[i].data[k] = arr[k]
+            gf[i].bcond[k] = arr[k].boundaryConditions
+        gf[i].thermo = f.thermoDetails
+        gf[i].rhoMin = f.MINMASS
+    f0 = fluids[0]
+    fsp = step_params(dt, int(order), int(step), int(method), geometry, f0.gamma, f0.MINMASS, f0.Cisothermal)
+    fsp.multifluidDragMethod = int(multifluidDragMethod) if n > 1 else 0
+    g = geometry.params()
+    cyl = g.shape in (abi.CYLINDRICAL, abi.RZCYLINDRICAL)
+    srcType = 1 * int(cyl) + 2 * int(g.frameOmega != 0) + 8 * int(n > 1)
+    rad = abi.ParametricRadiation()
+    if radiation is not None:
+        rad.exponent = float(radiation[0])
+        rad.minTemperature = 1.05  # sourceStep.cu:122
+        rad.prefactor = float(radiation[1]) * dt
+    topo = abi.ParallelTopology()
+    for i in range(3):
+        topo.nproc[i] = 1
+    call("performSourceFunctions", srcType, gf, n, fsp, ctypes.byref(topo), None, ctypes.byref(rad), None)
+
+
 def src0001(fluids, geometry, dTime, tFraction, dragMethod):
     """sources/src0001.m: drag over dTime*tFraction as two half steps, then the
     boundary conditions of every fluid in every direction."""

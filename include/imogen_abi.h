@@ -259,10 +259,44 @@ typedef enum DragMethods {
 
 /* cudaSource2FluidDrag.cu:189 (header cudaSource2FluidDrag.h). fluidA = gas,
  * fluidB = dust canisters (5 slab views each, same grid). Updates gas E and
- * momenta and dust momenta in place over dt. Built: methods 4-7; 0-3 return
- * ERROR_NOIMPLEMENT. */
+ * momenta and dust momenta in place over dt. Built: methods 0-2 and 4-7;
+ * method 3 (ETDRK2, listed as not implemented by the reference itself,
+ * classes/ImogenManager.m:181) returns ERROR_NOIMPLEMENT. */
 int sourcefunction_2FluidDrag(MGArray *fluidA, MGArray *fluidB, GeometryParams *geo, ThermoDetails *thermogas,
                               ThermoDetails *thermodust, double dt, int method);
+
+/* ---- compiled source step: gpuclass/sourceStep.h, cudaCommon.h:216-234,
+ *      cudaFreeRadiation.h:11-15 ------------------------------------------- */
+typedef struct GravityData {
+    MGArray *phi;  /* gravity potential */
+    int spaceOrder, timeOrder;
+} GravityData;
+
+typedef struct GridFluid {
+    MGArray DataHolder;  /* the 5-slab allocation */
+    MGArray data[5];     /* rho, E, px, py, pz slab views */
+    ThermoDetails thermo;
+    BCSettings bcond[5];
+    double rhoMin;
+} GridFluid;
+
+typedef struct ParametricRadiation {
+    double exponent;
+    double prefactor;  /* 0 = no radiation */
+    double minTemperature;
+} ParametricRadiation;
+
+/* sourceStep.cu:200-226. srcType = cyl + 2 rotatingFrame + 4 potential
+ * + 8 twoFluid (the table at sourceStep.cu:228-250). Built: 0 (nothing
+ * without radiation) and 8 (src___D, sourceStep.cu:300-326: one
+ * sourcefunction_2FluidDrag call of fluids[0] (gas) and fluids[1] (dust) over
+ * fsp.dt with fsp.multifluidDragMethod; no boundary update, as there).
+ * Radiation (prad->prefactor > 0) and srcType 1-7, 9-15 (cylindrical,
+ * rotating-frame and gravity terms) return ERROR_NOIMPLEMENT; other values
+ * ERROR_INVALID_ARGS. gravdata and tmpStorage are not used by the built
+ * types and may be NULL; prad must not be NULL (the reference reads it). */
+int performSourceFunctions(int srcType, GridFluid *fluids, int nFluids, FluidStepParams fsp, ParallelTopology *topo,
+                           GravityData *gravdata, ParametricRadiation *prad, MGArray *tmpStorage);
 
 /* ---- learned drag surrogate (no counterpart in the reference) -------------
  * Network 5 -> 64 -> 64 -> 3, GELU (tanh form), parameters in FP32 as

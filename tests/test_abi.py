@@ -101,3 +101,27 @@ def test_too_many_gpus_rejected():
     m = abi.MGArray()
     rc = L.deserializeTagToMGArray(tag.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), ctypes.byref(m))
     assert rc == abi.ERROR_DESERIALIZE_GPUTAG_FAILED
+
+
+def test_source_step_dispatch_errors_without_device_work():
+    """performSourceFunctions (sourceStep.cu:200-226) rejects what is not
+    built before touching any array: types with cylindrical, rotating-frame or
+    gravity terms, radiation, unknown types, a one-fluid drag and a missing
+    radiation record. Type 0 without radiation does nothing."""
+    L = abi.lib()
+    fsp = abi.FluidStepParams()
+    rad = abi.ParametricRadiation()
+    gf = (abi.GridFluid * 2)()
+    P = ctypes.byref
+
+    def call(src, n=2, r=rad):
+        return L.performSourceFunctions(src, gf, n, fsp, None, None, P(r) if r is not None else None, None)
+
+    assert call(0) == 0
+    for src in (1, 2, 3, 4, 5, 6, 7, 9, 12, 15):
+        assert call(src) == abi.ERROR_NOIMPLEMENT, src
+    assert call(16) == abi.ERROR_INVALID_ARGS and call(-1) == abi.ERROR_INVALID_ARGS
+    assert call(8, n=1) == abi.ERROR_INVALID_ARGS
+    assert call(8, r=None) == abi.ERROR_NULL_POINTER
+    rad.prefactor = 1.0
+    assert call(0) == abi.ERROR_NOIMPLEMENT and call(8) == abi.ERROR_NOIMPLEMENT

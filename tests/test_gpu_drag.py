@@ -122,6 +122,51 @@ def test_unbuilt_drag_methods_fail_loudly():
             run_gpu_drag(g, d, 0.01, m, sh)
 
 
+@pytest.mark.parametrize("method", [4, 5])
+def test_source_step_two_fluids_is_one_full_dt_drag_call(method):
+    """performSourceFunctions type 8 without radiation (src___D,
+    sourceStep.cu:300-326) is one sourcefunction_2FluidDrag call over
+    dTime * tFraction: bitwise the direct call, and the oracle to 1e-12."""
+    sh = (4, 6, 40)
+    g, d = random_pair(sh, seed=30 + method)
+    nz, ny, nx = sh
+    dTime, frac = 0.04, 0.5
+    print(f"sourceStep drag method {method} shape {sh}", flush=True)
+    a = abi()
+    tg, td = thermo_pair()
+    geo = gc().GeometryManager((nx, ny, nz), (0.1, 0.1, 0.1))
+    fg = gc().FluidManager(g, tg.gamma, thermoDetails=tg)
+    fd = gc().FluidManager(d, td.gamma, thermoDetails=td)
+    gc().sourceStep([fg, fd], geo, dTime, [1, 0, a.METHOD_HLLC, frac], multifluidDragMethod=method)
+    got_g, got_d = fg.state, fd.state
+    fg.free()
+    fd.free()
+    ref_g, ref_d = run_gpu_drag(g, d, dTime * frac, method, sh)
+    assert np.all(np.isfinite(got_g)) and np.all(np.isfinite(got_d))
+    assert np.array_equal(got_g, ref_g) and np.array_equal(got_d, ref_d)
+    og, od = gas_dust_thermo()
+    rg, rd = g.copy(), d.copy()
+    po.drag(rg, rd, og, od, dTime * frac, method)
+    compare_slabs(got_g, rg, 1e-12, f"gas m{method}")
+    compare_slabs(got_d, rd, 1e-12, f"dust m{method}")
+
+
+def test_source_step_one_fluid_and_unbuilt_terms():
+    a = abi()
+    sh = (2, 3, 8)
+    g, _ = random_pair(sh, seed=2)
+    tg, _ = thermo_pair()
+    geo = gc().GeometryManager((8, 3, 2), (0.1, 0.1, 0.1))
+    fg = gc().FluidManager(g, tg.gamma, thermoDetails=tg)
+    gc().sourceStep([fg], geo, 0.01, [1, 0, a.METHOD_HLLC, 1.0])  # type 0: nothing to do
+    assert np.array_equal(fg.state, g)
+    with pytest.raises(a.ImogenError) as e:
+        gc().sourceStep([fg], geo, 0.01, [1, 0, a.METHOD_HLLC, 1.0], radiation=(0.5, 1.0))
+    assert e.value.code == a.ERROR_NOIMPLEMENT
+    assert np.array_equal(fg.state, g)
+    fg.free()
+
+
 # ---------------------------------------------------------------- network kernels
 def _dev(x, dtype=np.float32):
     return ld().DeviceArray.from_host(np.asarray(x, dtype=dtype))
