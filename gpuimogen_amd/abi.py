@@ -150,6 +150,9 @@ SIGNATURES = {
     "calculateSoundspeed": (c_int, [_MGP, _MGP, _MGP, c_double]),
     "computeLocalCFLTimestep": (c_int, [_MGP, _MGP, _P(GeometryParams), c_int, _DP, _DP]),
     "imogen_cflTimestepFused": (c_int, [_MGP, _P(GeometryParams), c_int, c_double, _DP, _DP]),
+    "alterArrayMetadata": (c_int, [_MGP, _MGP, c_int]),
+    "flipArrayIndices": (c_int, [_MGP, _P(_MGP), c_int, c_int, c_void_p, _MGP]),
+    "exchange_MPI_Halos": (c_int, [_MGP, c_int, _P(ParallelTopology), c_int]),
     "performSourceFunctions": (c_int, [c_int, _P(GridFluid), c_int, FluidStepParams, _P(ParallelTopology),
                                        _P(GravityData), _P(ParametricRadiation), _MGP]),
     "sourcefunction_2FluidDrag": (c_int, [_MGP, _MGP, _P(GeometryParams), _P(ThermoDetails),

@@ -16,6 +16,8 @@
 // The reference stages |dv|, u and k through three temporary slabs and
 // several launches; here one thread carries a cell through the whole method
 // in registers: 72 bytes read and 56 bytes written per cell.
+#include <cstring>
+
 #include "drag_device.h"
 #include "imogen_internal.h"
 
@@ -135,6 +137,9 @@ int sourcefunction_2FluidDrag(MGArray *fluidA, MGArray *fluidB, GeometryParams *
     for (int i = 0; i < 3; i++)
         if (fluidA->dim[i] != fluidB->dim[i])
             return IM_FAIL(ERROR_INVALID_ARGS, "gas and dust grids differ in dimension %d", i + 1);
+    if (is_xyz(fluidA) != is_xyz(fluidB) ||
+        (!is_xyz(fluidA) && memcmp(fluidA->currentPermutation, fluidB->currentPermutation, sizeof(int) * 3)))
+        return IM_FAIL(ERROR_INVALID_ARGS, "gas and dust arrays are in different memory orders");
     if (pg != pd) return IM_FAIL(ERROR_INVALID_ARGS, "gas and dust slab pitches differ");
     const int64_t n = (int64_t)fluidA->dim[0] * fluidA->dim[1] * fluidA->dim[2];
     DeviceState *st = state();

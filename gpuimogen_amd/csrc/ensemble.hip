@@ -228,7 +228,11 @@ __global__ __launch_bounds__(256) void ens_sweep_y(const EnsProblem *__restrict_
 // constant-extrapolation ghost cells (cudaStatics.cu:1339-1366, 1440-1466 with
 // the +y innermost-ghost convention), depth d, both faces of one direction;
 // one block per problem
-__global__ __launch_bounds__(256) void ens_bc_const(const EnsProblem *__restrict__ probs, int dir, int d)
+// xrules: the +y face copies the last interior cell, as the fluid step sets
+// its boundaries (flux.cu:99-107, the swept direction rotated into memory x);
+// 0: the y kernel's innermost-ghost copy (cudaStatics.cu:1461-1466), as
+// src0001.m's boundary calls on unrotated arrays
+__global__ __launch_bounds__(256) void ens_bc_const(const EnsProblem *__restrict__ probs, int dir, int d, int xrules)
 {
     const EnsProblem P = probs[blockIdx.x];
     const int nx = P.nx, ny = P.ny;
@@ -259,7 +263,7 @@ __global__ __launch_bounds__(256) void ens_bc_const(const EnsProblem *__restrict
                 const double f = a[(int64_t)d * nx];
                 for (int g = 0; g < d; g++) a[(int64_t)g * nx] = f;
             } else {
-                const double f = a[(int64_t)(ny - d) * nx];
+                const double f = a[(int64_t)(xrules ? ny - d - 1 : ny - d) * nx];
                 for (int g = 0; g < d; g++) a[(int64_t)(ny - d + g) * nx] = f;
             }
         }
@@ -359,9 +363,9 @@ static int build_work(EnsembleImpl *e)
     return SUCCESSFUL;
 }
 
-static int ens_bc(EnsembleImpl *e, int dir, hipStream_t s)
+static int ens_bc(EnsembleImpl *e, int dir, hipStream_t s, int xrules)
 {
-    hipLaunchKernelGGL(ens_bc_const, dim3((unsigned)e->NE), dim3(256), 0, s, e->dprobs, dir, 4);
+    hipLaunchKernelGGL(ens_bc_const, dim3((unsigned)e->NE), dim3(256), 0, s, e->dprobs, dir, 4, xrules);
     return IM_LAUNCHED("ens_bc_const");
 }
 
@@ -376,7 +380,7 @@ static int ens_source(EnsembleImpl *e, double fraction, hipStream_t s)
         if (rc) return rc;
     }
     for (int d = 1; d <= 2; d++) {
-        int rc = ens_bc(e, d, s);
+        int rc = ens_bc(e, d, s, 0);
         if (rc) return rc;
     }
     return SUCCESSFUL;
@@ -411,7 +415,7 @@ static int ens_sweep(EnsembleImpl *e, int dir, int rt, hipStream_t s)
         int rc = IM_LAUNCHED("ens_sweep_y");
         if (rc) return rc;
     }
-    return ens_bc(e, dir, s);
+    return ens_bc(e, dir, s, 1);
 }
 
 // one iteration (imogen.m:115-139): CFL over all fluids, source(1/2),

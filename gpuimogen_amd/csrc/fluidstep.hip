@@ -18,17 +18,24 @@ int imogen::fluid_view(MGArray *fluid, double **base, int64_t *pitch)
     if (!fluid) return IM_FAIL(ERROR_NULL_POINTER, "null fluid");
     if (fluid->nGPUs != 1)
         return IM_FAIL(ERROR_NOIMPLEMENT, "only single-GPU fluids are supported (nGPUs = %d)", fluid->nGPUs);
-    if (fluid->permtag != 1 && fluid->permtag != 0)
-        return IM_FAIL(ERROR_NOIMPLEMENT, "fluid arrays must be in XYZ memory order (permtag %d)", fluid->permtag);
     const int64_t p = fluid->slabPitch[0] / (int64_t)sizeof(double);
     for (int i = 1; i < 5; i++) {
         if (fluid[i].devicePtr[0] != fluid[0].devicePtr[0] + i * p)
             return IM_FAIL(ERROR_INVALID_ARGS, "fluid array %d is not slab %d of the canister", i, i);
+        // the slabs are rotated one by one (flux.m xchgIndices): all must agree
+        if (is_xyz(&fluid[i]) != is_xyz(fluid) ||
+            (!is_xyz(fluid) && memcmp(fluid[i].currentPermutation, fluid->currentPermutation, sizeof(int) * 3)))
+            return IM_FAIL(ERROR_INVALID_ARGS, "fluid array %d is in another memory order than array 0", i);
     }
     if (!fluid->devicePtr[0]) return IM_FAIL(ERROR_NULL_POINTER, "null device pointer");
     *base = fluid->devicePtr[0];
     *pitch = p;
     return SUCCESSFUL;
+}
+
+bool imogen::is_xyz(const MGArray *a)
+{
+    return a->permtag == 0 || a->permtag == 1;
 }
 
 int imogen::check_topology(const ParallelTopology *t)
@@ -64,12 +71,96 @@ int sas_code(const MGArray *a, int direction, int side, int *code)
 
 namespace imogen {
 
+// Runs fn on an XYZ-ordered copy of a canister whose arrays were rotated
+// (flipArrayIndices) and writes the result back in the rotated order. The
+// sweeps walk each physical direction's own stride, so a rotated canister is
+// stepped exactly as its unrotated image: the reference's 1D update of memory-x
+// (always the swept direction once flux.m has rotated the arrays) is the sweep
+// of the memory axis that holds the physical direction.
+template <class Fn>
+static int on_xyz_copy(MGArray *fluid, Fn &&fn)
+{
+    double *base;
+    int64_t pitch;
+    int rc = fluid_view(fluid, &base, &pitch);
+    if (rc) return rc;
+    const int *perm = fluid->currentPermutation;
+    int pdim[3], toXyz[3], back[3];
+    for (int a = 0; a < 3; a++) {
+        if (perm[a] < 1 || perm[a] > 3) return IM_FAIL(ERROR_INVALID_ARGS, "currentPermutation[%d] = %d", a, perm[a]);
+        pdim[perm[a] - 1] = fluid->dim[a];
+        toXyz[perm[a] - 1] = a;  // XYZ axis perm[a]-1 comes from memory axis a
+        back[a] = perm[a] - 1;   // memory axis a comes from XYZ axis perm[a]-1
+    }
+    const int64_t numel = (int64_t)pdim[0] * pdim[1] * pdim[2];
+    const int64_t xpitch = ((numel * 8 + 255) / 256) * 256 / 8;
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    double *x = nullptr;
+    if (IM_HIP(hipMalloc(&x, (size_t)xpitch * 5 * sizeof(double)))) return ERROR_NOMEM;
+    MGArray xf[5];
+    for (int k = 0; k < 5 && !rc; k++) {
+        rc = launch_permute_copy(base + k * pitch, x + k * xpitch, fluid->dim, toXyz, st->stream);
+        xf[k] = fluid[k];
+        for (int a = 0; a < 3; a++) {
+            xf[k].dim[a] = pdim[a];
+            xf[k].currentPermutation[a] = a + 1;
+        }
+        xf[k].permtag = 1;
+        xf[k].numel = numel;
+        xf[k].partNumel[0] = (int)numel;
+        xf[k].slabPitch[0] = xpitch * (int64_t)sizeof(double);
+        xf[k].devicePtr[0] = x + k * xpitch;
+        xf[k].numSlabs = (k == 0) ? 5 : -k;
+    }
+    if (!rc) rc = fn(xf);
+    for (int k = 0; k < 5 && !rc; k++) rc = launch_permute_copy(x + k * xpitch, base + k * pitch, pdim, back, st->stream);
+    (void)hipStreamSynchronize(st->stream);
+    (void)hipFree(x);
+    return rc;
+}
+
+int fluid_boundary(MGArray *fluid, int direction, int xrules)
+{
+    double *base;
+    int64_t pitch;
+    int rc = fluid_view(fluid, &base, &pitch);
+    if (rc) return rc;
+    if (direction < 1 || direction > 3)
+        return IM_FAIL(ERROR_INVALID_ARGS, "direction %d is not one of 1 (X) 2 (Y) 3 (Z)", direction);
+    int modes10[10];
+    for (int side = 0; side < 2; side++) {
+        const BCModeTypes bt = fluid[0].boundaryConditions.mode[2 * (direction - 1) + side];
+        if (bt == outflow || bt == freebalance)
+            return IM_FAIL(ERROR_NOIMPLEMENT, "outflow/freebalance boundaries are not built");
+        for (int i = 0; i < 5; i++) {
+            rc = sas_code(&fluid[i], direction, side, &modes10[2 * i + side]);
+            if (rc) return rc;
+        }
+    }
+    // modes and mirror signs follow the physical direction, the kernels the
+    // memory axis that holds it (cudaStatics.cu:230)
+    const int memdir = is_xyz(fluid) ? direction : MGA_dir2memdir(fluid->currentPermutation, direction);
+    if (memdir < 1) return IM_FAIL(ERROR_INVALID_ARGS, "no memory axis holds direction %d", direction);
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    const int slot = prof_begin(PROF_BC, st->stream);
+    rc = launch_fluid_bc(base, pitch, fluid->dim[0], fluid->dim[1], fluid->dim[2], memdir, modes10,
+                         fluid->haloSize > 0 ? fluid->haloSize : 4, st->stream, xrules);
+    prof_end(slot, st->stream);
+    return rc;
+}
+
 int update_1d(MGArray *fluid, const FluidStepParams &params, const double *lampBase, const CflFuse *cfl)
 {
     double *base;
     int64_t pitch;
     int rc = fluid_view(fluid, &base, &pitch);
     if (rc) return rc;
+    if (!is_xyz(fluid)) {
+        if (lampBase || cfl) return IM_FAIL(ERROR_NOIMPLEMENT, "device-loop sweeps need XYZ-ordered arrays");
+        return on_xyz_copy(fluid, [&](MGArray *xf) { return update_1d(xf, params, nullptr, nullptr); });
+    }
     if (params.geometry.shape != SQUARE)
         return IM_FAIL(ERROR_NOIMPLEMENT, "only SQUARE (cartesian) geometry is built");
     if (params.onlyHydro != 1) return IM_FAIL(ERROR_NOIMPLEMENT, "MHD is not built (dead in the reference)");
@@ -127,13 +218,14 @@ int cfl_keep_range(MGArray *fluid, int dir, int *lo, int *hi)
         }
         if (code0 < 0) continue;  // circular: the sweep's values stay
         if (N < 2 * depth + 2) return 1;
-        if (side == 0) {
-            *lo = depth;  // ghosts [0, depth) copy or mirror cells >= depth
-        } else {
-            // +y/+z const copies the innermost ghost N-depth, which keeps its value
-            const bool keepsInnermost = (dir != 1) && (code0 == 2);
-            *hi = keepsInnermost ? N - depth + 1 : N - depth;
-        }
+        // a mirror face copies cells up to 2 depth from the face: they must lie
+        // in the kept range
+        if ((code0 == 0 || code0 == 1) && N <= 3 * depth) return 1;
+        // the fluid step's boundaries (x conventions along every direction)
+        // overwrite the ghosts [0, depth) and [N - depth, N) with copies or
+        // mirror images of cells in [depth, N - depth)
+        if (side == 0) *lo = depth;
+        else *hi = N - depth;
     }
     return 0;
 }
@@ -141,6 +233,11 @@ int cfl_keep_range(MGArray *fluid, int dir, int *lo, int *hi)
 int update_3d(MGArray *fluid, const FluidStepParams &fsp, const double *lampBase, const CflFuse *lastCfl)
 {
     int rc;
+    if (!fluid) return IM_FAIL(ERROR_NULL_POINTER, "null fluid");
+    if (!is_xyz(fluid)) {
+        if (lampBase || lastCfl) return IM_FAIL(ERROR_NOIMPLEMENT, "device-loop sweeps need XYZ-ordered arrays");
+        return on_xyz_copy(fluid, [&](MGArray *xf) { return update_3d(xf, fsp, nullptr, nullptr); });
+    }
     FluidStepParams p = fsp;
     const int last = lastCfl ? last_sweep_dir(fluid, fsp.stepNumber, fsp.stepDirection) : 0;
     CflFuse cf;
@@ -153,7 +250,7 @@ int update_3d(MGArray *fluid, const FluidStepParams &fsp, const double *lampBase
         p.stepDirection = 1;
         rc = update_1d(fluid, p, lampBase, lastCfl ? &cf : nullptr);
         if (rc) return rc;
-        return setFluidBoundary(fluid, &p.geometry, 1);
+        return fluid_boundary(fluid, 1, 1);
     }
     int sweep = (fsp.stepNumber + 3 * (fsp.stepDirection > 0)) % 6;
     if (sweep < 0) sweep += 6;
@@ -163,7 +260,9 @@ int update_3d(MGArray *fluid, const FluidStepParams &fsp, const double *lampBase
             p.stepDirection = d;
             rc = update_1d(fluid, p, lampBase, (lastCfl && d == last) ? &cf : nullptr);
             if (rc) return rc;
-            rc = setFluidBoundary(fluid, &p.geometry, d);
+            // the reference sets these boundaries with direction d rotated into
+            // memory x (flux.cu:99-107): the x kernels' conventions along d
+            rc = fluid_boundary(fluid, d, 1);
             if (rc) return rc;
         }
     }
@@ -177,29 +276,7 @@ extern "C" {
 int setFluidBoundary(MGArray *fluid, GeometryParams *geo, int direction)
 {
     (void)geo;
-    double *base;
-    int64_t pitch;
-    int rc = fluid_view(fluid, &base, &pitch);
-    if (rc) return rc;
-    if (direction < 1 || direction > 3)
-        return IM_FAIL(ERROR_INVALID_ARGS, "direction %d is not one of 1 (X) 2 (Y) 3 (Z)", direction);
-    int modes10[10];
-    for (int side = 0; side < 2; side++) {
-        const BCModeTypes bt = fluid[0].boundaryConditions.mode[2 * (direction - 1) + side];
-        if (bt == outflow || bt == freebalance)
-            return IM_FAIL(ERROR_NOIMPLEMENT, "outflow/freebalance boundaries are not built");
-        for (int i = 0; i < 5; i++) {
-            rc = sas_code(&fluid[i], direction, side, &modes10[2 * i + side]);
-            if (rc) return rc;
-        }
-    }
-    DeviceState *st = state();
-    if (!st) return ERROR_CUDA_BLEW_UP;
-    const int slot = prof_begin(PROF_BC, st->stream);
-    rc = launch_fluid_bc(base, pitch, fluid->dim[0], fluid->dim[1], fluid->dim[2], direction, modes10,
-                         fluid->haloSize > 0 ? fluid->haloSize : 4, st->stream);
-    prof_end(slot, st->stream);
-    return rc;
+    return fluid_boundary(fluid, direction, 0);
 }
 
 int setArrayBoundaryConditions(MGArray *phi, GeometryParams *geo, int direction, int side)
@@ -218,8 +295,10 @@ int setArrayBoundaryConditions(MGArray *phi, GeometryParams *geo, int direction,
     modes10[side] = code;  // slab 0 of a one-slab launch at phi's own pointer
     DeviceState *st = state();
     if (!st) return ERROR_CUDA_BLEW_UP;
+    const int memdir = is_xyz(phi) ? direction : MGA_dir2memdir(phi->currentPermutation, direction);
+    if (memdir < 1) return IM_FAIL(ERROR_INVALID_ARGS, "no memory axis holds direction %d", direction);
     // A single array is addressed as slab 0; the other 4 table entries are inert.
-    return launch_fluid_bc(phi->devicePtr[0], 0, phi->dim[0], phi->dim[1], phi->dim[2], direction, modes10,
+    return launch_fluid_bc(phi->devicePtr[0], 0, phi->dim[0], phi->dim[1], phi->dim[2], memdir, modes10,
                            phi->haloSize > 0 ? phi->haloSize : 4, st->stream);
 }
 

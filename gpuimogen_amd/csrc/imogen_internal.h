@@ -53,6 +53,14 @@ void *scratch(size_t bytes);
 int fluid_view(MGArray *fluid, double **base, int64_t *pitch);
 // ERROR_NOIMPLEMENT for topologies of more than one rank (null: one rank).
 int check_topology(const ParallelTopology *t);
+// Whether an array is in XYZ memory order (permtag 0 or 1: never rotated, or
+// rotated back). fluid_view accepts rotated canisters (all five arrays in the
+// same order); the sweeps and boundaries map directions through
+// currentPermutation.
+bool is_xyz(const MGArray *a);
+// Copies an sdim[0] x sdim[1] x sdim[2] array so that destination axis a walks
+// source axis ax[a] (rotate.hip).
+int launch_permute_copy(const double *src, double *dst, const int sdim[3], const int ax[3], hipStream_t s);
 
 // performFluidUpdate_1D / _3D without the topology check. lampBase: when not
 // null, the sweeps of direction d read lambda from lampBase[d - 1] (device).
@@ -61,12 +69,16 @@ int check_topology(const ParallelTopology *t);
 int update_1d(MGArray *fluid, const FluidStepParams &params, const double *lampBase,
               const CflFuse *cfl = nullptr);
 int update_3d(MGArray *fluid, const FluidStepParams &fsp, const double *lampBase, const CflFuse *lastCfl = nullptr);
+// setFluidBoundary with a choice of kernel conventions (launch_fluid_bc's
+// xrules); update_3d uses xrules = 1 after every sweep.
+int fluid_boundary(MGArray *fluid, int direction, int xrules);
 // Direction swept last by update_3d for (stepNumber, stepDirection), 0 if none.
 int last_sweep_dir(const MGArray *fluid, int stepNumber, int stepDirection);
-// Cells [lo, hi) of direction dir whose swept values survive the direction's
-// boundary conditions, when every other ghost the boundaries write is a copy
-// or mirror image of one of them (modes circular, mirror, const). Returns
-// non-zero when the boundaries do not allow folding the CFL into the sweep.
+// Cells [lo, hi) of direction dir whose swept values survive the boundary
+// conditions update_3d sets after the sweep, when every other ghost they write
+// is a copy or mirror image of one of them (modes circular, mirror, const).
+// Returns non-zero when the boundaries do not allow folding the CFL into the
+// sweep.
 int cfl_keep_range(MGArray *fluid, int dir, int *lo, int *hi);
 
 // Launchers (fluid_sweep.hip, xinjin.hip, statics.hip, cfl.hip, drag.hip)
@@ -80,8 +92,11 @@ int launch_fluid_sweep(double *base, int64_t pitch, int nx, int ny, int nz, int 
                        const double *lamp = nullptr, const CflFuse *cfl = nullptr);
 int launch_xinjin_sweep(double *base, int64_t pitch, int nx, int ny, int nz, int dir, double lambda,
                         double gamma, double rhomin, hipStream_t s, const double *lamp = nullptr);
+// xrules: apply the memory-x kernels' conventions along direction dir (the
+// boundaries the fluid step sets after each sweep); 0: each memory axis's own
+// kernels (setFluidBoundary / setArrayBoundaryConditions).
 int launch_fluid_bc(double *base, int64_t pitch, int nx, int ny, int nz, int dir, const int *modes10,
-                    int depth, hipStream_t s);
+                    int depth, hipStream_t s, int xrules = 0);
 int launch_soundspeed(const double *base, int64_t pitch, int64_t n, double gamma, double *cs,
                       hipStream_t s);
 int launch_cfl(const double *base, int64_t pitch, int64_t n, const double *cs, double gamma, int fused,

@@ -6,9 +6,14 @@
 //   * z-minus mirror reflects about cell 4.5 (index 2*depth+1-q), x/y about 4;
 //   * +y and +z 'const' copy the innermost ghost cell (n-depth), +x copies the
 //     last interior cell (n-depth-1).
-// One launch handles both faces of one direction for all 5 slabs when the
-// faces are disjoint; for lines shorter than 2*depth+2 the minus face runs
-// first, then the plus face, the reference's order.
+// The fluid step sets each sweep's boundaries with the memory-x kernels'
+// conventions along the swept direction (xrules: the reference rotates that
+// direction into memory x first, flux.cu:99-107), so these quirks act only in
+// setFluidBoundary / setArrayBoundaryConditions calls on unrotated arrays.
+// One launch handles both faces of one direction for all 5 slabs when neither
+// face reads a cell the other writes (lines of at least 3*depth+2 cells);
+// otherwise the minus face runs first, then the plus face, the reference's
+// order.
 #include "imogen_internal.h"
 
 namespace imogen {
@@ -51,9 +56,14 @@ __global__ void bc_face_x(double *__restrict__ q, int64_t pitch, int nx, int64_t
     }
 }
 
-// memdir 2 (y) and 3 (z): one thread per (face cell, slab), looping over depth
+// memdir 2 (y) and 3 (z): one thread per (face cell, slab), looping over depth.
+// xrules: the memory-x kernels' index conventions along this axis (what the
+// reference's fluid step applies: it rotates the swept direction into memory
+// x before setting its boundaries, flux.cu:99-107), i.e. no z-minus mirror
+// offset, +side const from the last interior cell, and -side linear.
 __global__ void bc_face_yz(double *__restrict__ q, int64_t pitch, int nx, int64_t n1, int64_t n2,
-                           int64_t stride, int64_t ostride, int N, int zdir, int side0, int d, SasTable t)
+                           int64_t stride, int64_t ostride, int N, int zdir, int side0, int d, SasTable t,
+                           int xrules)
 {
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t face = (int64_t)nx * n2;
@@ -67,7 +77,7 @@ __global__ void bc_face_yz(double *__restrict__ q, int64_t pitch, int nx, int64_
     const int64_t o = tid / nx;
     double *c = q + slab * pitch + x + o * ostride;
     if (side == 0) {
-        const int off = zdir ? 2 * d + 1 : 2 * d;  // z-minus mirror quirk (:1496-1500)
+        const int off = (zdir && !xrules) ? 2 * d + 1 : 2 * d;  // z-minus mirror quirk (:1496-1500)
         switch (sas) {
         case 0:
             for (int g = 0; g < d; g++) c[g * stride] = c[(off - g) * stride];
@@ -79,6 +89,10 @@ __global__ void bc_face_yz(double *__restrict__ q, int64_t pitch, int nx, int64_
             const double f = c[d * stride];
             for (int g = 0; g < d; g++) c[g * stride] = f;
         } break;
+        case 3: {  // -x linear extrapolation (:1372-1380), xrules only
+            const double f0 = c[d * stride], f1 = c[(d + 1) * stride];
+            for (int g = 0; g < d; g++) c[g * stride] = f0 + (d - g) * (f0 - f1);
+        } break;
         }
     } else {
         switch (sas) {
@@ -89,7 +103,9 @@ __global__ void bc_face_yz(double *__restrict__ q, int64_t pitch, int nx, int64_
             for (int g = 0; g < d; g++) c[(N - 1 - g) * stride] = -c[(N - 1 - 2 * d + g) * stride];
             break;
         case 2: {
-            const double f = c[(N - d) * stride];  // innermost ghost (:1461-1466, :1524-1532)
+            // y/z kernels copy the innermost ghost (:1461-1466, :1524-1532), the
+            // x kernel the last interior cell (:1361-1366)
+            const double f = c[(xrules ? N - d - 1 : N - d) * stride];
             for (int g = 0; g < d; g++) c[(N - d + g) * stride] = f;
         } break;
         }
@@ -97,7 +113,7 @@ __global__ void bc_face_yz(double *__restrict__ q, int64_t pitch, int nx, int64_
 }
 
 int launch_fluid_bc(double *base, int64_t pitch, int nx, int ny, int nz, int dir, const int *modes10,
-                    int depth, hipStream_t s)
+                    int depth, hipStream_t s, int xrules)
 {
     const int N = (dir == 1) ? nx : (dir == 2 ? ny : nz);
     SasTable t;
@@ -110,12 +126,14 @@ int launch_fluid_bc(double *base, int64_t pitch, int nx, int ny, int nz, int dir
         if ((t.s[k] == 0 || t.s[k] == 1) && N < 2 * depth + 2)
             return IM_FAIL(ERROR_INVALID_ARGS, "mirror BC needs >= %d cells (got %d)", 2 * depth + 2, N);
         if (N < depth + 2) return IM_FAIL(ERROR_INVALID_ARGS, "BC needs >= %d cells (got %d)", depth + 2, N);
-        if (t.s[k] == 3 && (dir != 1 || side != 0))
+        if (t.s[k] == 3 && ((dir != 1 && !xrules) || side != 0))
             return IM_FAIL(ERROR_NOIMPLEMENT, "linear extrapolation BC exists only on the -x face");
     }
-    // Both faces in one launch when they cannot touch each other's cells
-    // (N >= 2*depth+2); otherwise side 0 then side 1, as the reference orders them.
-    const bool together = any[0] && any[1] && N >= 2 * depth + 2;
+    // Both faces in one launch when neither reads a cell the other writes:
+    // a mirror face reads up to 2 depth (z-minus: 2 depth + 1) cells in from
+    // its edge, so the faces are disjoint for N >= 3 depth + 2. Otherwise
+    // side 0, then side 1, the reference's order.
+    const bool together = any[0] && any[1] && N >= 3 * depth + 2;
     for (int side = 0; side < 2; side++) {
         if (!any[side]) continue;
         const int nsides = together ? 2 : 1;
@@ -130,7 +148,8 @@ int launch_fluid_bc(double *base, int64_t pitch, int nx, int ny, int nz, int dir
             const int64_t n2 = (dir == 2) ? nz : ny;
             const int64_t face = (int64_t)nx * n2;
             hipLaunchKernelGGL(bc_face_yz, dim3((unsigned)((face + 255) / 256), 5 * nsides), dim3(256), 0, s, base,
-                               pitch, nx, (int64_t)N, n2, stride, ostride, N, dir == 3 ? 1 : 0, side, depth, t);
+                               pitch, nx, (int64_t)N, n2, stride, ostride, N, dir == 3 ? 1 : 0, side, depth, t,
+                               xrules);
         }
         int rc = IM_LAUNCHED("boundary condition kernel");
         if (rc) return rc;

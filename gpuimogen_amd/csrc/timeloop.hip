@@ -208,6 +208,7 @@ int imogen_timeloop_create(MGArray *fluids, int nfluids, const GeometryParams *g
         int64_t pitch;
         int rc = fluid_view(fluids + 5 * f, &base, &pitch);
         if (rc) return rc;
+        if (!is_xyz(fluids + 5 * f)) return IM_FAIL(ERROR_NOIMPLEMENT, "the device loop needs XYZ-ordered fluids");
         if (params[f].geometry.shape != SQUARE) return IM_FAIL(ERROR_NOIMPLEMENT, "only SQUARE geometry is built");
         if (params[f].onlyHydro != 1) return IM_FAIL(ERROR_NOIMPLEMENT, "MHD is not built");
         const int m = (int)params[f].stepMethod;

@@ -235,6 +235,93 @@ def cudaFluidStep(params, fluid, geometry, limiter=abi.LIMITER_OSPRE):
     call("performFluidUpdate_1D", fluid.arrays(), p, ctypes.byref(topo), None)
 
 
+def cudaArrayRotateB(array, code):
+    """cudaArrayRotateB(tag, code) in place (gpuclass/cudaArrayRotateB.cu:56-88):
+    flipArrayIndices on one array, then its tag is rewritten (dims and
+    permtag). code 2 = X<->Y, 3 = X<->Z, 4 = Y<->Z, 5 = XYZ -> YZX, 6 = XYZ -> ZXY."""
+    m = array.mgarray()
+    call("flipArrayIndices", ctypes.byref(m), None, 1, int(code), None, None)
+    abi.lib().serializeMGArrayToTag(ctypes.byref(m), array._tp())
+
+
+def cudaHaloExchange(array, direction, topology=None):
+    """cudaHaloExchange(tag, dir, topology, circ) (cudaHaloExchange.cu:113-133):
+    exchange_MPI_Halos on one array; nothing to exchange on one rank."""
+    m = array.mgarray()
+    topo = topology if topology is not None else abi.ParallelTopology()
+    if topology is None:
+        for i in range(3):
+            topo.nproc[i] = 1
+    call("exchange_MPI_Halos", ctypes.byref(m), 1, ctypes.byref(topo), int(direction))
+
+
+def cudaStatics(array, geometry, direction, vector_component=0, bc_modes=None):
+    """cudaStatics(array, 8, geometry, dir) (cudaStatics.cu:85-104): both faces
+    of one array in physical direction dir (mapped to the memory axis that
+    holds it)."""
+    m = array.mgarray()
+    m.vectorComponent = vector_component
+    if bc_modes is not None:
+        for k in range(6):
+            m.boundaryConditions.mode[k] = int(bc_modes[k])
+    g = geometry.params()
+    for side in (0, 1):
+        call("setArrayBoundaryConditions", ctypes.byref(m), ctypes.byref(g), int(direction), side)
+
+
+def xchgIndices(fluid, code):
+    """flux/flux.m:xchgIndices: rotate the 5 arrays of a fluid (code 0: nothing)."""
+    if code == 0:
+        return
+    for g in [fluid.mass, fluid.ener] + fluid.mom:
+        cudaArrayRotateB(g, code)
+
+
+def relaxingFluid(fluid, geometry, direction, dt, method, limiter=abi.LIMITER_OSPRE):
+    """fluid/relaxingFluid.m:25-31: cudaFluidStep in physical direction X on
+    the (rotated) arrays, then cudaStatics of each array in X."""
+    p = [dt, 1, fluid.gamma, fluid.MINMASS, method, direction]
+    if fluid.Cisothermal != -1:
+        p.append(fluid.Cisothermal)
+    cudaFluidStep(p, fluid, geometry, limiter=limiter)
+    for i, g in enumerate([fluid.mass, fluid.ener] + fluid.mom):
+        cudaStatics(g, geometry, direction, vector_component=0 if i < 2 else i - 1, bc_modes=fluid.bcModes)
+
+
+# flux/flux.m:29-40
+_PREPERM = [0, 2, 0, 2, 3, 3]
+_FLUXCALL = [[1, 2, 1, 2, 3, 3], [3, 1, 2, 3, 1, 2], [2, 3, 3, 1, 2, 1]]
+_PERMCALL = [[3, 2, 2, 3, 3, 2], [2, 3, 3, 5, 2, 6], [6, 3, 5, 0, 2, 0]]
+
+
+def flux(fluids, geometry, order, iteration, dt, method, limiter=abi.LIMITER_OSPRE):
+    """flux/flux.m: the MATLAB-side fluid step of one direction order (+1
+    forward, -1 backward) that rotates the arrays so every sweep is along
+    memory x: xchgIndices(preperm), then per position n relaxingFluid of
+    fluxcall(n) (directions of more than 2 cells), the (one-rank) halo
+    exchange, xchgIndices(permcall(n)). The tables end every sequence in XYZ
+    order. 2D runs take sweep = mod(iteration + (order < 0), 2) + 1 as flux.m
+    does; 1D runs one X sweep."""
+    dims = geometry.globalDomainRez
+    if sum(1 for d in dims if d > 2) == 1:
+        for f in fluids:
+            relaxingFluid(f, geometry, 1, dt, method, limiter)
+        return
+    if dims[2] > 2:
+        sweep = (iteration + 3 * (order > 0)) % 6
+    else:
+        sweep = (iteration + (order < 0)) % 2
+    for f in fluids:
+        xchgIndices(f, _PREPERM[sweep])
+        for n in range(3):
+            d = _FLUXCALL[n][sweep]
+            if dims[d - 1] > 2:
+                relaxingFluid(f, geometry, d, dt, method, limiter)
+                for g in [f.mass, f.ener] + f.mom:
+                    cudaHaloExchange(g, d)
+            xchgIndices(f, _PERMCALL[n][sweep])
+
+
 def cudaSoundspeed(fluid, gamma):
     """cs = cudaSoundspeed(mass, ener, momX, momY, momZ, gamma) (cudaSoundspeed.cu:54-100)."""
     arr = fluid.arrays()

@@ -182,6 +182,31 @@ int MGA_uploadArrayToGPU(double *p, MGArray *g, int partitionTo);
 /* cudaCommon.cu:2048-2217: device -> host; *p == NULL -> malloc'd for the caller */
 int MGA_downloadArrayToCPU(MGArray *g, double **p, int partitionFrom);
 
+/* ---- index exchanges and halos (gpuclass/cudaArrayRotateB.cu, cudaHaloExchange.h) */
+/* cudaArrayRotateB.cu:583-640: the metadata of an index exchange (dims,
+ * currentPermutation, permtag, partitionDir, partNumel); exchangeCode 2 = X<->Y,
+ * 3 = X<->Z, 4 = Y<->Z, 5 = rotate left (XYZ -> YZX), 6 = rotate right
+ * (XYZ -> ZXY); other codes ERROR_INVALID_ARGS. src may equal dst. */
+int alterArrayMetadata(MGArray *src, MGArray *dst, int exchangeCode);
+/* cudaArrayRotateB.cu:103-300 (the work of the cudaArrayRotateB MEX,
+ * :56-88): transposes each of nArrays arrays in memory by exchangeCode and
+ * rewrites its metadata. newArrays == NULL: in place (the arrays keep their
+ * pointers; slab views of a fluid canister may be rotated one by one, as
+ * flux.m does). newArrays != NULL: new owner arrays are allocated (*newArrays
+ * == NULL: a malloc'd vector of nArrays MGArrays for the caller to free).
+ * Exchanges that do not change the memory order (a unit dimension) only
+ * rewrite the metadata (:551-579). streamPtrs and tempStorage are accepted
+ * for the reference's signature and not used. A rotated fluid canister can
+ * be passed to every fluid entry point except the device loop: the fluid
+ * steps sweep the memory axis that holds the requested physical direction,
+ * the boundaries map it through currentPermutation (cudaStatics.cu:230). */
+int flipArrayIndices(MGArray *phi, MGArray **newArrays, int nArrays, int exchangeCode, void *streamPtrs,
+                     MGArray *tempStorage);
+/* cudaHaloExchange.h:13 (cudaHaloExchange.cu:140): one rank has no halos to
+ * exchange (periodic lines wrap inside the sweeps); more ranks return
+ * ERROR_NOIMPLEMENT. */
+int exchange_MPI_Halos(MGArray *phi, int nArrays, ParallelTopology *topo, int xchgDir);
+
 /* ======================================================================== */
 /* MEX-level tag primitives (gpuclass/GPU_*.cu) with host buffers and int64  */
 /* tags in place of mxArrays. Tags are GPU_TAG_LENGTH + 2 int64.             */

@@ -861,6 +861,64 @@ int orc_set_fluid_boundary(double *q, const orc_grid *g, int dir, const int *mod
     return ORC_OK;
 }
 
+/* The boundaries the fluid step sets after sweeping direction dir. flux.cu
+ * rotates the arrays so that the swept direction is memory x
+ * (cudaArrayRotateB before each sweep, the preperm / permcall tables of
+ * flux.cu:46-48) and then calls setFluidBoundary (flux.cu:99-107), which picks
+ * its kernels by memory axis (cudaStatics.cu:230): the memory-x kernels'
+ * conventions (cudaStatics.cu:1308-1380) apply along dir, whatever dir is. */
+static int array_face_bc_swept(double *phi, const orc_grid *g, int side, int dir, int sas, int d)
+{
+    const long n3[3] = {g->nx, g->ny, g->nz};
+    const long s3[3] = {1, (long)g->nx, (long)g->nx * g->ny};
+    const int a1 = (dir == 1) ? 1 : 0, a2 = (dir == 3) ? 1 : 2; /* the two other axes */
+    const long N = n3[dir - 1], st = s3[dir - 1];
+    long i, j;
+    int q;
+    if ((sas == 0 || sas == 1) && N < 2 * d + 2) return ORC_EINVAL;
+    if (N < d + 2) return ORC_EINVAL;
+    if (sas == 3 && side == 1) return ORC_ENOIMPL;
+    for (j = 0; j < n3[a2]; j++)
+        for (i = 0; i < n3[a1]; i++) {
+            double *r = phi + i * s3[a1] + j * s3[a2];
+            if (side == 0) {
+                if (sas == 0) for (q = 0; q < d; q++) r[(d - 1 - q) * st] = r[(d + 1 + q) * st];
+                else if (sas == 1) for (q = 0; q < d; q++) r[(d - 1 - q) * st] = -r[(d + 1 + q) * st];
+                else if (sas == 2) for (q = 0; q < d; q++) r[q * st] = r[d * st];
+                else {
+                    double f0 = r[d * st], f1 = r[(d + 1) * st];
+                    for (q = 0; q < d; q++) r[q * st] = f0 + (d - q) * (f0 - f1);
+                }
+            } else {
+                if (sas == 0) for (q = 0; q < d; q++) r[(N - d + q) * st] = r[(N - d - 2 - q) * st];
+                else if (sas == 1) for (q = 0; q < d; q++) r[(N - d + q) * st] = -r[(N - d - 2 - q) * st];
+                else for (q = 0; q < d; q++) r[(N - d + q) * st] = r[(N - d - 1) * st];
+            }
+        }
+    return ORC_OK;
+}
+
+int orc_fluid_boundary_swept(double *q, const orc_grid *g, int dir, const int *modes, int depth)
+{
+    if (dir < 1 || dir > 3) return ORC_EINVAL;
+    int side, s;
+    for (side = 0; side < 2; side++) {
+        int bt = modes[2 * (dir - 1) + side];
+        if (bt == ORC_BC_OUTFLOW || bt == ORC_BC_FREEBALANCE || bt == ORC_BC_WALL) return ORC_ENOIMPL;
+        if (bt == ORC_BC_CIRCULAR || bt == ORC_BC_STATIONARY) continue;
+        for (s = 0; s < 5; s++) {
+            int sas;
+            int veccomp = (s < 2) ? 0 : s - 1;
+            if (bt == ORC_BC_MIRROR) sas = (veccomp == dir);
+            else if (bt == ORC_BC_CONST) sas = 2;
+            else sas = 3;
+            int rc = array_face_bc_swept(q + (long)s * g->pitch, g, side, dir, sas, depth);
+            if (rc) return rc;
+        }
+    }
+    return ORC_OK;
+}
+
 /* ------------------------------------------------------------------------ */
 /* fluidstep: gpuclass/flux.cu:27-155                                        */
 /* ------------------------------------------------------------------------ */
@@ -881,7 +939,7 @@ int orc_fluidstep(double *q, const orc_grid *g, const double *h, double dt, int 
     if (g->ny == 1 && g->nz == 1) {
         rc = orc_fluid_update_1d(q, g, 1, method, dt / h[0], p);
         if (rc) return rc;
-        return orc_set_fluid_boundary(q, g, 1, modes, depth);
+        return orc_fluid_boundary_swept(q, g, 1, modes, depth);
     }
     int sweep = (stepNumber + 3 * (stepDirection > 0)) % 6;
     int dims[3] = {g->nx, g->ny, g->nz};
@@ -891,7 +949,7 @@ int orc_fluidstep(double *q, const orc_grid *g, const double *h, double dt, int 
         if (dims[d - 1] > 3) {
             rc = orc_fluid_update_1d(q, g, d, method, dt / h[d - 1], p);
             if (rc) return rc;
-            rc = orc_set_fluid_boundary(q, g, d, modes, depth);
+            rc = orc_fluid_boundary_swept(q, g, d, modes, depth);
             if (rc) return rc;
         }
     }

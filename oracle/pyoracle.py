@@ -62,6 +62,8 @@ def lib():
         L.orc_xj_predict_1d.restype = ctypes.c_int
         L.orc_set_fluid_boundary.argtypes = [dp, ctypes.POINTER(_Grid), ctypes.c_int, ip, ctypes.c_int]
         L.orc_set_fluid_boundary.restype = ctypes.c_int
+        L.orc_fluid_boundary_swept.argtypes = [dp, ctypes.POINTER(_Grid), ctypes.c_int, ip, ctypes.c_int]
+        L.orc_fluid_boundary_swept.restype = ctypes.c_int
         L.orc_fluidstep.argtypes = [dp, ctypes.POINTER(_Grid), dp, ctypes.c_double, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_int, ctypes.POINTER(_Params), ip, ctypes.c_int]
         L.orc_fluidstep.restype = ctypes.c_int
@@ -143,6 +145,16 @@ def set_fluid_boundary(q, direction, modes, depth=4):
                                       m.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), depth)
     if rc:
         raise RuntimeError(f"orc_set_fluid_boundary returned {rc}")
+
+
+def fluid_boundary_swept(q, direction, modes, depth=4):
+    """The boundaries the fluid step sets after sweeping `direction`."""
+    g = _grid(q)
+    m = _modes(modes)
+    rc = lib().orc_fluid_boundary_swept(_dptr(q), ctypes.byref(g), direction,
+                                        m.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), depth)
+    if rc:
+        raise RuntimeError(f"orc_fluid_boundary_swept returned {rc}")
 
 
 def fluidstep(q, h, dt, step_direction, step_number, gamma, method=HLLC, limiter=LIM_OSPRE,

@@ -160,8 +160,10 @@ def test_limiters_parity(limiter):
 
 
 @pytest.mark.parametrize("bc", ["const", "mirror", "circular"])
-@pytest.mark.parametrize("sh", [(20, 24, 28), (1, 36, 44), (1, 1, 64)])
+@pytest.mark.parametrize("sh", [(20, 24, 28), (1, 36, 44), (1, 1, 64), (12, 11, 13)])
 def test_fluidstep_parity_with_boundaries(bc, sh):
+    """(12, 11, 13): lines of 11-13 cells, where a mirror face reads cells the
+    opposite face writes, so the faces must be set one after the other."""
     a = abi()
     mode = {"const": a.extrapConstant, "mirror": a.mirror, "circular": a.circular}[bc]
     omode = {"const": po.BC_CONST, "mirror": po.BC_MIRROR, "circular": po.BC_CIRCULAR}[bc]
@@ -179,6 +181,26 @@ def test_fluidstep_parity_with_boundaries(bc, sh):
             gc().fluidstep([f], None, None, None, [dt, 1, direction, it, 2], gm)
             po.fluidstep(ref, gm.h, dt, direction, it, G, po.HLLC, po.LIM_OSPRE, modes=(omode,) * 6)
     compare(f.state, ref, f"fluidstep {bc} {sh}")
+    f.free()
+
+
+@pytest.mark.parametrize("bc", ["const", "mirror"])
+def test_set_fluid_boundary_on_short_lines(bc):
+    """setFluidBoundary on unrotated arrays (each memory axis's own kernels,
+    with the +y/+z const and z-minus mirror quirks) on 11-13-cell lines, whose
+    two faces overlap for mirror: exactly the oracle's face-by-face result."""
+    a = abi()
+    sh = (12, 11, 13)
+    q = random_state(sh, seed=11, smooth=True)
+    mode = a.extrapConstant if bc == "const" else a.mirror
+    omode = po.BC_CONST if bc == "const" else po.BC_MIRROR
+    f = gc().FluidManager(q, G, bc_modes=(mode,) * 6)
+    gm = geom(sh, (0.05, 0.05, 0.05))
+    ref = q.copy()
+    for d in (1, 2, 3):
+        gc().setBoundaries(f, gm, d)
+        po.set_fluid_boundary(ref, d, (omode,) * 6)
+        assert np.array_equal(f.state, ref), d
     f.free()
 
 

@@ -298,3 +298,42 @@ def test_bc_const_and_mirror_index_maps():
         assert np.all(r[[0, 1, 2, 3], g] == q[[0, 1, 2, 3], 2 * d + 1 - g])
         assert np.all(r[4, g] == -q[4, 2 * d + 1 - g])
         assert np.all(r[[0, 1, 2, 3], 16 - 1 - g] == q[[0, 1, 2, 3], 16 - 1 - 2 * d + g])
+
+
+@pytest.mark.parametrize("direction", [2, 3])
+@pytest.mark.parametrize("mode", ["const", "mirror", "linear"])
+def test_swept_boundaries_are_x_boundaries_of_the_rotated_arrays(direction, mode):
+    """flux.cu rotates the arrays so the swept direction is memory x and then
+    calls setFluidBoundary (flux.cu:99-107), which picks the memory-x kernels:
+    the fluid step's boundaries along y or z are the x boundaries of the
+    transposed arrays, with that direction's momentum as the normal one."""
+    d = 4
+    sh = (13, 14, 15) if direction == 3 else (3, 14, 5)
+    q = _labelled(sh)
+    code = {"const": po.BC_CONST, "mirror": po.BC_MIRROR, "linear": po.BC_LINEAR}[mode]
+    modes = [po.BC_CIRCULAR] * 6
+    modes[2 * (direction - 1)] = code
+    if mode != "linear":  # linear extrapolation exists on the -x face only
+        modes[2 * (direction - 1) + 1] = code
+    r = q.copy()
+    po.fluid_boundary_swept(r, direction, modes, d)
+    # (slab, z, y, x): move the swept axis last (memory x), and the swept
+    # direction's momentum into the px slot
+    ax = 1 if direction == 3 else 2
+    order = [0, 1, 2, 3]
+    order[ax], order[3] = order[3], order[ax]
+    t = np.ascontiguousarray(np.transpose(q, order))
+    slabs = [0, 1, 1 + direction] + [k for k in (2, 3, 4) if k != 1 + direction]
+    t = np.ascontiguousarray(t[slabs])
+    xm = [code, po.BC_CIRCULAR if mode == "linear" else code] + [po.BC_CIRCULAR] * 4
+    po.set_fluid_boundary(t, 1, xm, d)
+    back = np.empty_like(t)
+    back[slabs] = t
+    expect = np.transpose(back, order)
+    assert np.array_equal(r, expect)
+    # and the per-axis kernels differ where the quirks act (const: +face
+    # source; mirror along z: the -face offset), so the choice matters
+    if mode == "const" or (mode == "mirror" and direction == 3):
+        r2 = q.copy()
+        po.set_fluid_boundary(r2, direction, modes, d)
+        assert not np.array_equal(r2, r)
