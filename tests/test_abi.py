@@ -125,3 +125,33 @@ def test_source_step_dispatch_errors_without_device_work():
     assert call(8, r=None) == abi.ERROR_NULL_POINTER
     rad.prefactor = 1.0
     assert call(0) == abi.ERROR_NOIMPLEMENT and call(8) == abi.ERROR_NOIMPLEMENT
+
+
+def test_index_exchange_metadata_follows_flux_m_tables():
+    """alterArrayMetadata (cudaArrayRotateB.cu:583-640) walked over flux.m's own
+    preperm / permcall tables (flux/flux.m:29-40): before every sweep the
+    swept direction fluxcall(n) sits in memory x, and every sequence ends in
+    XYZ order (permtag 1) with the dims restored; unknown codes are rejected."""
+    from gpuimogen_amd import gpuclass as gc
+    L = abi.lib()
+    for s in range(6):
+        m = abi.MGArray()
+        m.dim[0], m.dim[1], m.dim[2] = 16, 12, 14
+        m.nGPUs = 1
+        m.currentPermutation[0], m.currentPermutation[1], m.currentPermutation[2] = 1, 2, 3
+        m.permtag = 1
+        m.partitionDir = abi.PARTITION_X
+        m.partNumel[0] = 16 * 12 * 14
+        if gc._PREPERM[s]:
+            assert L.alterArrayMetadata(ctypes.byref(m), ctypes.byref(m), gc._PREPERM[s]) == 0
+        for n in range(3):
+            d = gc._FLUXCALL[n][s]
+            assert m.currentPermutation[0] == d, (s, n)
+            assert m.dim[0] == (16, 12, 14)[d - 1]
+            assert m.partNumel[0] == 16 * 12 * 14
+            if gc._PERMCALL[n][s]:
+                assert L.alterArrayMetadata(ctypes.byref(m), ctypes.byref(m), gc._PERMCALL[n][s]) == 0
+        assert tuple(m.currentPermutation) == (1, 2, 3) and m.permtag == 1
+        assert tuple(m.dim) == (16, 12, 14) and m.partitionDir == abi.PARTITION_X
+    m = abi.MGArray()
+    assert L.alterArrayMetadata(ctypes.byref(m), ctypes.byref(m), 7) == abi.ERROR_INVALID_ARGS
