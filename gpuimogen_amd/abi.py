@@ -143,6 +143,8 @@ SIGNATURES = {
     "GPU_clone": (c_int, [_I64P, _I64P]),
     "GPU_copy": (c_int, [_I64P, _I64P]),
     "GPU_ctrl_memory": (c_int, [c_int, _P(ctypes.c_size_t), _P(ctypes.c_size_t)]),
+    "GPU_ctrl_reset": (c_int, []),
+    "GPU_ctrl_info": (c_int, [c_int, ctypes.c_char_p, c_int]),
     "performFluidUpdate_3D": (c_int, [_MGP, _P(ParallelTopology), FluidStepParams, _MGP]),
     "performFluidUpdate_1D": (c_int, [_MGP, FluidStepParams, _P(ParallelTopology), _MGP]),
     "setFluidBoundary": (c_int, [_MGP, _P(GeometryParams), c_int]),

@@ -587,6 +587,34 @@ int GPU_copy(const int64_t *dstTag, const int64_t *srcTag)
     return IM_HIP(hipStreamSynchronize(st->stream));
 }
 
+int GPU_ctrl_reset(void)
+{
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    int rc = IM_HIP(hipStreamSynchronize(st->stream));
+    if (rc) return rc;
+    if (st->scratch) {
+        rc = IM_HIP(hipFree(st->scratch));
+        st->scratch = nullptr;
+        st->scratchBytes = 0;
+    }
+    return rc;
+}
+
+int GPU_ctrl_info(int device, char *text, int len)
+{
+    if (!text || len < 1) return IM_FAIL(ERROR_NULL_POINTER, "no text buffer");
+    hipDeviceProp_t p;
+    int rc = IM_HIP(hipGetDeviceProperties(&p, device));
+    if (rc) return rc;
+    snprintf(text, (size_t)len,
+             "device %d: %s (%s), %d compute units, %zu MiB global memory, %zu KiB LDS per block, "
+             "wavefront %d, %d MHz",
+             device, p.name, p.gcnArchName, p.multiProcessorCount, (size_t)(p.totalGlobalMem >> 20),
+             (size_t)(p.sharedMemPerBlock >> 10), p.warpSize, p.clockRate / 1000);
+    return SUCCESSFUL;
+}
+
 int GPU_ctrl_memory(int device, size_t *freeBytes, size_t *totalBytes)
 {
     int cur = 0;

@@ -230,6 +230,14 @@ int GPU_clone(const int64_t *tag, int64_t *newTag);
 int GPU_copy(const int64_t *dstTag, const int64_t *srcTag);
 /* GPU_ctrl.cu 'memory' query */
 int GPU_ctrl_memory(int device, size_t *freeBytes, size_t *totalBytes);
+/* GPU_ctrl.cu:154-165 'reset'. The reference resets the device; here the
+ * library releases its cached temporary storage (nothing cached survives a
+ * reset, as the reference requires of temporaries) and caller-owned arrays
+ * stay valid. Device loops re-capture their graphs when the temporary block
+ * moves. */
+int GPU_ctrl_reset(void);
+/* GPU_ctrl.cu:19-35, 69-79 'info': a one-line description of the device. */
+int GPU_ctrl_info(int device, char *text, int len);
 
 /* ======================================================================== */
 /* Fluid step core                                                           */

@@ -97,6 +97,32 @@ def test_upload_download_and_slabs():
     c.free()
 
 
+def test_gpu_ctrl_reset_and_info():
+    """GPU_ctrl('reset') releases the cached temporaries between calls without
+    touching the caller's arrays: a fluid step after a reset gives bitwise the
+    step without one. GPU_ctrl('info') names the device."""
+    import ctypes
+    a = abi()
+    sh = (10, 12, 14)
+    q = random_state(sh, seed=21, smooth=True)
+    gm = geom(sh, (0.05, 0.05, 0.05))
+    fa = gc().FluidManager(q, G, bc_modes=(a.extrapConstant,) * 6)
+    fb = gc().FluidManager(q, G, bc_modes=(a.extrapConstant,) * 6)
+    for it in range(2):
+        for direction in (1, -1):
+            gc().fluidstep([fa], None, None, None, [0.002, 1, direction, it, 2], gm)
+            a.call("GPU_ctrl_reset")
+            gc().fluidstep([fb], None, None, None, [0.002, 1, direction, it, 2], gm)
+    assert np.array_equal(fa.state, fb.state)
+    fa.free()
+    fb.free()
+    buf = ctypes.create_string_buffer(512)
+    a.call("GPU_ctrl_info", 0, buf, 512)
+    text = buf.value.decode()
+    assert text.startswith("device 0:") and "compute units" in text
+    print(text, flush=True)
+
+
 SHAPES = [(23, 29, 37), (4, 6, 256), (6, 5, 192), (5, 200, 8), (150, 3, 20), (1, 40, 48), (1, 1, 100),
           (3, 7, 97)]
 
