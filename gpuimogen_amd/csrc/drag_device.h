@@ -332,5 +332,9 @@ int launch_ens_drag(const EnsDragTable *tab, const EnsDragItem *items, int nitem
                     unsigned long long *qcount, hipStream_t s);
 // changes whenever imogen_drag_set_model loads a model (captured graphs hold it by value)
 uint64_t learned_model_generation();
+// Allocates the learned drag's device counters (cells, fallbacks, queue
+// length) if needed. Call at setup, outside any graph capture: the launchers
+// of captured loops never allocate.
+int learned_drag_prepare();
 
 }  // namespace imogen

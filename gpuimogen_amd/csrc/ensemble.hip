@@ -554,6 +554,7 @@ int imogen_ensemble_set_dust(ImogenEnsemble *E, const int *hasDust, double dustG
     DeviceState *st = state();
     if (!st) return ERROR_CUDA_BLEW_UP;
     if (IM_HIP(hipStreamSynchronize(st->stream))) return ERROR_CUDA_BLEW_UP;
+    if (learned_drag_prepare()) return ERROR_CUDA_BLEW_UP;  // no allocation once graphs capture
     std::vector<int> dusty;
     int64_t total = 0, cells = 0;
     for (int p = 0; p < e->P; p++)

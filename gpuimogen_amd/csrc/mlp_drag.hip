@@ -710,10 +710,15 @@ int launch_ens_drag(const EnsDragTable *tab, const EnsDragItem *items, int nitem
                        frac, c, method, g_model.dev, queue, qcount);
     int rc = IM_LAUNCHED("ens_drag_kernel");
     if (rc || method != DRAG_LEARNED) return rc;
-    if (ensure_counts()) return ERROR_CUDA_BLEW_UP;
+    if (!g_model.counts) return IM_FAIL(ERROR_INVALID_ARGS, "learned drag counters not prepared (learned_drag_prepare)");
     hipLaunchKernelGGL(ens_fallback_kernel, dim3(1024), dim3(256), 0, s, tab, dtp, frac, c, queue, qcount, ncells,
                        g_model.counts);
     return IM_LAUNCHED("ens_fallback_kernel");
+}
+
+int learned_drag_prepare()
+{
+    return ensure_counts();
 }
 
 uint64_t learned_model_generation()

@@ -367,6 +367,7 @@ int imogen_timeloop_set_drag(ImogenTimeloop *T, const ThermoDetails *gas, const 
         if (L->fluid[0][0].dim[i] != L->fluid[1][0].dim[i])
             return IM_FAIL(ERROR_INVALID_ARGS, "gas and dust grids differ in dimension %d", i + 1);
     if (pg != pd) return IM_FAIL(ERROR_INVALID_ARGS, "gas and dust slab pitches differ");
+    if (learned_drag_prepare()) return ERROR_CUDA_BLEW_UP;  // no allocation once graphs capture
     L->dragMethod = method;
     L->dc = drag_consts(gas, dust);
     L->modelGen = learned_model_generation();
