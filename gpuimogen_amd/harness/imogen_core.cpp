@@ -4,17 +4,24 @@
 // it builds the fluid arrays through the same tag primitives the MATLAB side
 // uses (GPU_upload, GPU_makeslab, GPU_setslab, GPU_getslab), then runs the
 // iteration of performCompleteTimestep (gpuimogen_core.cu:954-1054):
-//     tau = CFL time step; dt = cfl * tau
-//     source(dt/2); performFluidUpdate_3D(+1); source(dt); performFluidUpdate_3D(-1); source(dt/2)
-// where the source is the two-fluid drag when a dust fluid is present.
+//     tau = min over fluids of the CFL time step; dt = cfl * tau
+//     source(dt/2); performFluidUpdate_3D(+1) of every fluid; source(dt);
+//     performFluidUpdate_3D(-1) of every fluid; source(dt/2)
+// where source is performSourceFunctions (sourceStep.cu:200-226): nothing for
+// one fluid, the two-fluid drag (type 8, one sourcefunction_2FluidDrag call
+// over the interval) for gas + dust.
 //
-//   imogen_core --problem sedov|sod [--n N] [--steps K] [--method hllc|hll|xinjin]
-//               [--limiter 0..6] [--cfl C] [--ic FILE] [--out FILE]
+//   imogen_core --problem sedov|sod|gasdust [--n N] [--steps K]
+//               [--method hllc|hll|xinjin] [--limiter 0..6] [--cfl C]
+//               [--drag M] [--thermo FILE] [--ic FILE] [--out FILE]
 //
 // --ic reads the initial state as raw little-endian doubles shaped
 // (5, nz, ny, nx) with nx = ny = nz = N (sedov) or nx = N, ny = nz = 1 (sod);
-// --out writes the final state in the same layout. The last line printed is
-// one JSON object with the timing and conserved totals.
+// gasdust (required --ic) reads (2, 5, N, N, N): gas then dust, periodic
+// boundaries, drag method M (default 4, LogTrap2), and --thermo the two
+// ThermoDetails records (gas, dust) as 16 doubles in field order. --out writes
+// the final state in the input layout. The last line printed is one JSON
+// object with the timing and conserved totals.
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -112,8 +119,8 @@ std::vector<double> sod(int n, double gamma)
 
 int main(int argc, char **argv)
 {
-    std::string problem = "sedov", method = "hllc", icfile, outfile;
-    int n = 64, steps = 10, limiter = LIMITER_OSPRE;
+    std::string problem = "sedov", method = "hllc", icfile, outfile, thermofile;
+    int n = 64, steps = 10, limiter = LIMITER_OSPRE, drag = DRAG_LOGTRAP2;
     double cfl = 0.4;
     for (int i = 1; i < argc; i++) {
         std::string a = argv[i];
@@ -132,6 +139,8 @@ int main(int argc, char **argv)
         else if (a == "--cfl") cfl = atof(next());
         else if (a == "--ic") icfile = next();
         else if (a == "--out") outfile = next();
+        else if (a == "--drag") drag = atoi(next());
+        else if (a == "--thermo") thermofile = next();
         else {
             fprintf(stderr, "unknown option %s\n", a.c_str());
             return 2;
@@ -150,7 +159,26 @@ int main(int argc, char **argv)
     std::vector<double> q;
     int bc[6];
     int ndep = 0;
-    if (problem == "sedov") {
+    int nfluids = 1;
+    ThermoDetails thermo[2];
+    memset(thermo, 0, sizeof(thermo));
+    if (problem == "gasdust") {
+        dims[0] = dims[1] = dims[2] = n;
+        nfluids = 2;
+        for (int s = 0; s < 6; s++) bc[s] = circular;
+        FILE *fh = thermofile.empty() ? nullptr : fopen(thermofile.c_str(), "rb");
+        if (!fh || fread(thermo, sizeof(ThermoDetails), 2, fh) != 2) {
+            fprintf(stderr, "gasdust needs --thermo with 2 ThermoDetails records\n");
+            return 1;
+        }
+        fclose(fh);
+        gamma = thermo[0].gamma;
+        q.assign(2 * 5 * (size_t)n * n * n, 0.0);
+        if (icfile.empty()) {
+            fprintf(stderr, "gasdust needs --ic\n");
+            return 1;
+        }
+    } else if (problem == "sedov") {
         dims[0] = dims[1] = dims[2] = n;
         gamma = 5.0 / 3.0;
         q = sedov(n, gamma, &ndep);
@@ -174,8 +202,13 @@ int main(int argc, char **argv)
         fclose(fh);
     }
 
-    Fluid f;
-    upload_fluid(f, q, dims, bc);
+    const size_t N = (size_t)dims[0] * dims[1] * dims[2];
+    Fluid fl[2];
+    for (int k = 0; k < nfluids; k++) {
+        std::vector<double> part(q.begin() + k * 5 * N, q.begin() + (k + 1) * 5 * N);
+        upload_fluid(fl[k], part, dims, bc);
+    }
+    const double gam[2] = {gamma, nfluids > 1 ? thermo[1].gamma : gamma};
 
     GeometryParams geo;
     memset(&geo, 0, sizeof(geo));
@@ -200,25 +233,62 @@ int main(int argc, char **argv)
     p.geometry = geo;
     p.slopeLimiter = limiter;
 
+    // the fluids as performSourceFunctions sees them (GridFluid, cudaCommon.h:224-234)
+    GridFluid gf[2];
+    memset(gf, 0, sizeof(gf));
+    for (int k = 0; k < nfluids; k++) {
+        gf[k].DataHolder = fl[k].arr[0];
+        for (int s = 0; s < 5; s++) {
+            gf[k].data[s] = fl[k].arr[s];
+            gf[k].bcond[s] = fl[k].arr[s].boundaryConditions;
+        }
+        gf[k].thermo = thermo[k];
+        gf[k].rhoMin = 1e-8;
+    }
+    ParametricRadiation rad;
+    memset(&rad, 0, sizeof(rad));
+    const int srcType = (nfluids > 1) ? 8 : 0;
+    p.multifluidDragMethod = drag;
+    auto source = [&](double dt) {
+        FluidStepParams sp = p;
+        sp.dt = dt;
+        check(performSourceFunctions(srcType, gf, nfluids, sp, &topo, nullptr, &rad, nullptr), "performSourceFunctions");
+    };
+    auto fluid_step = [&](int direction) {
+        for (int k = 0; k < nfluids; k++) {
+            p.thermoGamma = gam[k];
+            p.stepDirection = direction;
+            check(performFluidUpdate_3D(fl[k].arr, &topo, p, nullptr), "performFluidUpdate_3D");
+        }
+    };
+
     double t = 0.0;
     check(imogen_synchronize(), "imogen_synchronize");
     const auto t0 = std::chrono::steady_clock::now();
     for (int it = 0; it < steps; it++) {
         double tau = 0.0;
-        check(imogen_cflTimestepFused(f.arr, &geo, fm, gamma, rez, &tau), "imogen_cflTimestepFused");
+        for (int k = 0; k < nfluids; k++) {
+            double tk = 0.0;
+            check(imogen_cflTimestepFused(fl[k].arr, &geo, fm, gam[k], rez, &tk), "imogen_cflTimestepFused");
+            tau = (k == 0 || tk < tau) ? tk : tau;
+        }
         p.dt = cfl * tau;
         p.stepNumber = it;
-        p.stepDirection = 1;
-        check(performFluidUpdate_3D(f.arr, &topo, p, nullptr), "performFluidUpdate_3D(+1)");
-        p.stepDirection = -1;
-        check(performFluidUpdate_3D(f.arr, &topo, p, nullptr), "performFluidUpdate_3D(-1)");
+        source(0.5 * p.dt);
+        fluid_step(1);
+        source(p.dt);
+        fluid_step(-1);
+        source(0.5 * p.dt);
         t += 2 * p.dt;
     }
     check(imogen_synchronize(), "imogen_synchronize");
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 
-    std::vector<double> out = download_fluid(f, dims);
-    const size_t N = (size_t)dims[0] * dims[1] * dims[2];
+    std::vector<double> out;
+    for (int k = 0; k < nfluids; k++) {
+        std::vector<double> o = download_fluid(fl[k], dims);
+        out.insert(out.end(), o.begin(), o.end());
+    }
     double mass = 0, energy = 0;
     for (size_t i = 0; i < N; i++) {
         mass += out[i];
@@ -232,12 +302,12 @@ int main(int argc, char **argv)
         }
         fclose(fh);
     }
-    check(GPU_free(f.owner), "GPU_free");
+    for (int k = 0; k < nfluids; k++) check(GPU_free(fl[k].owner), "GPU_free");
     const double cu = (double)N * 2 * steps;
     printf("{\"problem\": \"%s\", \"method\": \"%s\", \"dims\": [%d, %d, %d], \"steps\": %d, \"seconds\": %.6f, "
            "\"cell_updates_per_s\": %.6e, \"sim_time\": %.9e, \"mass\": %.17g, \"energy\": %.17g, "
-           "\"deposit_cells\": %d}\n",
+           "\"deposit_cells\": %d, \"fluids\": %d}\n",
            problem.c_str(), method.c_str(), dims[0], dims[1], dims[2], steps, secs, cu / secs, t, mass, energy,
-           ndep);
+           ndep, nfluids);
     return 0;
 }

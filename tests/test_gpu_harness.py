@@ -68,3 +68,39 @@ def test_core_sod_matches_oracle(tmp_path, method):
     scale = np.maximum(np.abs(ref).reshape(5, -1).sum(1), np.abs(ref).sum() * 1e-3)
     print(f"sod {method}: rel L1", d / scale, flush=True)
     assert np.all(d / scale <= 1e-12)
+
+
+def test_core_gasdust_with_drag_source_matches_oracle(tmp_path):
+    """Two fluids through the compiled source step: per iteration the drag
+    (LogTrap2) over dt/2, both fluids' fluid step (+1), the drag over dt, (-1),
+    the drag over dt/2 (performCompleteTimestep, gpuimogen_core.cu:954-1054,
+    with performSourceFunctions type 8), against the oracle's drag and fluid
+    step in the same order."""
+    from gpuimogen_amd import learned_drag as L, problems
+    n, steps = 12, 3
+    gas, dust, h, _ = problems.gasdust3d(n, seed=2)
+    tg, td = L.cold_h2(), L.iron_dust()
+    td.m *= 20.0  # a gentler drag rate keeps 12^3 well inside LogTrap2's accuracy
+    thermo = tmp_path / "thermo.bin"
+    np.array([getattr(t, f) for t in (tg, td) for f, _ in t._fields_], dtype=np.float64).tofile(thermo)
+    q0 = np.stack([gas, dust])
+    got, info = run_core(tmp_path, ["--problem", "gasdust", "--n", str(n), "--steps", str(steps), "--drag", "4",
+                                    "--thermo", str(thermo)], q0)
+    assert info["fluids"] == 2
+    ot = [po.Thermo(*[getattr(t, f) for f, _ in t._fields_]) for t in (tg, td)]
+    g, d = gas.copy(), dust.copy()
+    modes = (po.BC_CIRCULAR,) * 6
+    for it in range(steps):
+        tau = min(po.cfl_timestep(q, po.soundspeed(q, t.gamma), h, po.HLLC, (n, n, n))
+                  for q, t in ((g, tg), (d, td)))
+        dt = 0.4 * tau
+        po.drag(g, d, ot[0], ot[1], 0.5 * dt, po.DRAG_LOGTRAP2)
+        for direction in (1, -1):
+            po.fluidstep(g, h, dt, direction, it, tg.gamma, po.HLLC, po.LIM_OSPRE, modes=modes)
+            po.fluidstep(d, h, dt, direction, it, td.gamma, po.HLLC, po.LIM_OSPRE, modes=modes)
+            po.drag(g, d, ot[0], ot[1], dt if direction > 0 else 0.5 * dt, po.DRAG_LOGTRAP2)
+    assert np.all(np.isfinite(got))
+    for k, ref in enumerate((g, d)):
+        err = rel_l1(got[k], ref)
+        print(f"gas+dust fluid {k}: rel L1 {err}", flush=True)
+        assert max(err) <= 1e-12
