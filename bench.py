@@ -173,6 +173,7 @@ def run_gasdust(args):
     pe, pl = out["exact"].pop("dust_mom"), out["learned"].pop("dust_mom")
     rel_l1 = float(np.abs(pl - pe).sum() / np.abs(pe).sum())
     return {"workload": f"gasdust3d_{n}cubed", "steps": args.gasdust_steps,
+            "loop": "device time loop (dTime on the device, drag source around both fluid steps), eager launches",
             "step_speedup": out["exact"]["seconds"] / out["learned"]["seconds"],
             "drag_speedup": out["exact"]["drag_ms_per_call"] / out["learned"]["drag_ms_per_call"],
             "dust_momentum_rel_l1": rel_l1, "exact": out["exact"], "learned": out["learned"],

@@ -85,6 +85,9 @@ static int on_xyz_copy(MGArray *fluid, Fn &&fn)
     int rc = fluid_view(fluid, &base, &pitch);
     if (rc) return rc;
     const int *perm = fluid->currentPermutation;
+    if (perm[0] + perm[1] + perm[2] != 6 || perm[0] * perm[1] * perm[2] != 6)
+        return IM_FAIL(ERROR_INVALID_ARGS, "currentPermutation (%d %d %d) is not a permutation of 1 2 3", perm[0],
+                       perm[1], perm[2]);
     int pdim[3], toXyz[3], back[3];
     for (int a = 0; a < 3; a++) {
         if (perm[a] < 1 || perm[a] > 3) return IM_FAIL(ERROR_INVALID_ARGS, "currentPermutation[%d] = %d", a, perm[a]);
