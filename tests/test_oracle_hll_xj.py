@@ -12,10 +12,16 @@ What pins what:
   XJ floors            closed form on a uniform state (:2620-2628)
   both                 Sod vs the exact solution, conservation, fixed points,
                        direction maps
-Parity unpinned: the isothermal HLL branch (its speed bounds, passed C_iso^2
-for the sound speed, and its energy flux follow the reference's text,
-cudaFluidStep.cu:1779-1799 and :2131-2191; only conservation and the
-fixed point are checked).
+  HLL isothermal       the interface flux read off the sweep at a jump
+                       (Richardson in lam): upwind physical flux when both
+                       bounds share a sign, one HLL speed pair consistent over
+                       mass and all momenta when subsonic, and that pair equal
+                       to the reference's bounds (C_iso^2 as the sound speed,
+                       cs overwritten before the one-sided bounds), with states
+                       that separate them from textbook Einfeldt
+Parity unpinned: the isothermal energy flux (E is a passive scalar there; the
+HLL form lacks the 1/(SR-SL) factor in the reference, cudaFluidStep.cu:2178-2181,
+and the oracle follows that text; only conservation and the fixed point check it).
 """
 import numpy as np
 import pytest
@@ -133,6 +139,87 @@ def test_hll_isothermal_conserves_and_keeps_uniform_state():
     r = u.copy()
     po.fluid_update_1d(r, 1, 0.05, 2.0, method=po.HLL, ciso=0.8)
     assert np.allclose(r, u, rtol=1e-15, atol=0)
+
+
+def _iso_physical_flux(U, pcoef):
+    """Mass and momentum fluxes along x with pressure pcoef * rho (pcoef = C_iso^2
+    for an isothermal gas)."""
+    rho, _, px, py, pz = U
+    u = px / rho
+    return np.array([px, np.nan, px * u + pcoef * rho, py * u, pz * u])
+
+
+def _iso_interface_flux(UL, UR, c, pcoefL=None, n=24):
+    """The flux the oracle's isothermal HLL sweep puts through the jump of a
+    piecewise-constant line (left half UL, right half UR). Next to a jump one
+    of the two one-sided differences is zero, so every limiter gives a zero
+    slope and the cell left of the jump changes by -lam (F* - F(UL)) + O(lam^2);
+    Richardson over lam and lam/2 leaves O(lam^2) ~ 1e-9. pcoefL: the pressure
+    coefficient of the flux between two UL cells (C_iso^2 unless that interface
+    takes the reference's right-upwind branch)."""
+    def rate(lam):
+        q = np.zeros((5, 1, 1, n))
+        q[:, 0, 0, : n // 2] = UL[:, None]
+        q[:, 0, 0, n // 2:] = UR[:, None]
+        out = q.copy()
+        po.fluid_update_1d(out, 1, lam, 2.0, method=po.HLL, ciso=c)
+        return (out - q)[:, 0, 0, n // 2 - 1] / lam
+    return _iso_physical_flux(UL, c * c if pcoefL is None else pcoefL) - (2 * rate(5e-5) - rate(1e-4))
+
+
+@pytest.mark.parametrize("sign", [1, -1])
+def test_hll_isothermal_supersonic_flux_is_the_upwind_physical_flux(sign):
+    """Whatever the speed bounds, a jump whose bounds are both of one sign
+    passes the physical flux of the upwind state (mass and momenta; the
+    energy flux of the isothermal branch is the reference's own). Flow to +x
+    takes P = C_iso^2 rho; flow to -x takes the reference's HLL_RIGHT branch,
+    whose momentum flux adds C_iso rho, not C_iso^2 rho
+    (cudaFluidStep.cu:2191: FLUID_ISOTEMP where the other branches have
+    FLUID_ISOTEMPSQR) -- the interface between the two left cells takes that
+    branch as well."""
+    c = 0.8
+    pcoef = c * c if sign > 0 else c
+    UL = cons(1.0, 3.0 * sign, 0.2, -0.1, 0.64, g=2.0)
+    UR = cons(0.5, 2.8 * sign, -0.3, 0.4, 0.32, g=2.0)
+    F = _iso_interface_flux(UL, UR, c, pcoefL=pcoef)
+    up = _iso_physical_flux(UL if sign > 0 else UR, pcoef)
+    for k in (0, 2, 3, 4):
+        assert abs(F[k] - up[k]) <= 1e-7 * max(1.0, abs(up[k])), k
+
+
+@pytest.mark.parametrize("case", [
+    # c, (rho, vx) left, (rho, vx) right
+    (0.8, (1.0, 0.2), (0.5, -0.1)),
+    (0.8, (1.3, 0.1), (0.4, 0.6)),
+    (0.5, (1.0, 0.0), (1.0, 0.6)),   # the -x bound comes from vL - (Vroe - C^2)
+    (0.5, (1.2, -0.1), (0.9, 0.55)),
+])
+def test_hll_isothermal_subsonic_flux_has_the_reference_speed_bounds(case):
+    """A subsonic jump: the flux is (SR F_L - SL F_R + SL SR (U_R - U_L)) / (SR - SL)
+    for one pair (SL, SR) -- solved from the mass and normal-momentum fluxes, it
+    must reproduce both transverse momentum fluxes -- and that pair is the
+    reference's computeEinfeldtBounds_isothermal (cudaFluidStep.cu:1779-1799)
+    called with C_iso^2 as the sound speed: cs is overwritten by Vroe - cs
+    before the one-sided bounds, so SL = min(Vroe - C^2, vL - (Vroe - C^2)) and
+    SR = max(Vroe + C^2, vR + (Vroe - C^2)), Vroe the Roe-averaged velocity. The
+    last two cases separate this from the textbook Einfeldt bounds
+    (vL - C^2, vR + C^2), which differ there by 0.2-0.3."""
+    from scipy.optimize import fsolve
+    c, (rl, vl), (rr, vr) = case
+    UL = cons(rl, vl, 0.2, -0.1, c * c * rl, g=2.0)
+    UR = cons(rr, vr, -0.3, 0.4, c * c * rr, g=2.0)
+    F = _iso_interface_flux(UL, UR, c)
+    FL, FR = _iso_physical_flux(UL, c * c), _iso_physical_flux(UR, c * c)
+
+    def hll(sl, sr):
+        return (sr * FL - sl * FR + sl * sr * (UR - UL)) / (sr - sl)
+
+    sl, sr = fsolve(lambda x: [hll(*x)[0] - F[0], hll(*x)[2] - F[2]], [-1.0, 1.0], xtol=1e-14)
+    assert np.allclose(hll(sl, sr)[3:], F[3:], rtol=0, atol=1e-8)
+    vroe = (np.sqrt(rl) * vl + np.sqrt(rr) * vr) / (np.sqrt(rl) + np.sqrt(rr))
+    s = vroe - c * c
+    assert abs(sl - min(s, vl - s)) <= 1e-7 and abs(sr - max(vroe + c * c, vr + s)) <= 1e-7
+    assert sl < 0 < sr
 
 
 # ---------------------------------------------------------------- Xin-Jin
