@@ -18,10 +18,10 @@ What pins what:
                        mass and all momenta when subsonic, and that pair equal
                        to the reference's bounds (C_iso^2 as the sound speed,
                        cs overwritten before the one-sided bounds), with states
-                       that separate them from textbook Einfeldt
-Parity unpinned: the isothermal energy flux (E is a passive scalar there; the
-HLL form lacks the 1/(SR-SL) factor in the reference, cudaFluidStep.cu:2178-2181,
-and the oracle follows that text; only conservation and the fixed point check it).
+                       that separate them from textbook Einfeldt; the
+                       energy flux (a passive scalar, the reference's HLL form
+                       without 1/(SR-SL), cudaFluidStep.cu:2178-2181) against
+                       that form with the fitted speed pair
 """
 import numpy as np
 import pytest
@@ -164,7 +164,8 @@ def _iso_interface_flux(UL, UR, c, pcoefL=None, n=24):
         out = q.copy()
         po.fluid_update_1d(out, 1, lam, 2.0, method=po.HLL, ciso=c)
         return (out - q)[:, 0, 0, n // 2 - 1] / lam
-    return _iso_physical_flux(UL, c * c if pcoefL is None else pcoefL) - (2 * rate(5e-5) - rate(1e-4))
+    d = 2 * rate(5e-5) - rate(1e-4)
+    return _iso_physical_flux(UL, c * c if pcoefL is None else pcoefL) - d, d
 
 
 @pytest.mark.parametrize("sign", [1, -1])
@@ -181,7 +182,7 @@ def test_hll_isothermal_supersonic_flux_is_the_upwind_physical_flux(sign):
     pcoef = c * c if sign > 0 else c
     UL = cons(1.0, 3.0 * sign, 0.2, -0.1, 0.64, g=2.0)
     UR = cons(0.5, 2.8 * sign, -0.3, 0.4, 0.32, g=2.0)
-    F = _iso_interface_flux(UL, UR, c, pcoefL=pcoef)
+    F, _ = _iso_interface_flux(UL, UR, c, pcoefL=pcoef)
     up = _iso_physical_flux(UL if sign > 0 else UR, pcoef)
     for k in (0, 2, 3, 4):
         assert abs(F[k] - up[k]) <= 1e-7 * max(1.0, abs(up[k])), k
@@ -203,12 +204,13 @@ def test_hll_isothermal_subsonic_flux_has_the_reference_speed_bounds(case):
     before the one-sided bounds, so SL = min(Vroe - C^2, vL - (Vroe - C^2)) and
     SR = max(Vroe + C^2, vR + (Vroe - C^2)), Vroe the Roe-averaged velocity. The
     last two cases separate this from the textbook Einfeldt bounds
-    (vL - C^2, vR + C^2), which differ there by 0.2-0.3."""
+    (vL - C^2, vR + C^2), which differ there by 0.2-0.3. The energy flux is
+    then checked against the reference's form with the fitted pair."""
     from scipy.optimize import fsolve
     c, (rl, vl), (rr, vr) = case
     UL = cons(rl, vl, 0.2, -0.1, c * c * rl, g=2.0)
     UR = cons(rr, vr, -0.3, 0.4, c * c * rr, g=2.0)
-    F = _iso_interface_flux(UL, UR, c)
+    F, d = _iso_interface_flux(UL, UR, c)
     FL, FR = _iso_physical_flux(UL, c * c), _iso_physical_flux(UR, c * c)
 
     def hll(sl, sr):
@@ -220,6 +222,17 @@ def test_hll_isothermal_subsonic_flux_has_the_reference_speed_bounds(case):
     s = vroe - c * c
     assert abs(sl - min(s, vl - s)) <= 1e-7 and abs(sr - max(vroe + c * c, vr + s)) <= 1e-7
     assert sl < 0 < sr
+
+    # energy, a passive scalar here: the reference's HLL energy flux
+    # SR (vL - SL) wL - SL (vR - SR) wR with w = KE + C_iso^2 rho has no
+    # 1/(SR - SL) (cudaFluidStep.cu:2178-2181), so between two UL cells it is
+    # vL wL (SR - SL) of their own bounds, not vL wL
+    def w(U):
+        return 0.5 * (U[2] ** 2 + U[3] ** 2 + U[4] ** 2) / U[0] + c * c * U[0]
+    sll, srl = min(vl - c * c, c * c), max(vl + c * c, 2 * vl - c * c)
+    Fe = vl * w(UL) * (srl - sll) - d[1]
+    want = sr * (vl - sl) * w(UL) - sl * (vr - sr) * w(UR)
+    assert abs(Fe - want) <= 1e-8 * max(1.0, abs(want))
 
 
 # ---------------------------------------------------------------- Xin-Jin
