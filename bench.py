@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark: HLLC fluid step on the 3D Sedov-Taylor blast, FP64, one MI355X.
 
-Workload (the north-star configuration that fits one GPU): 256^3 cells on the
+Workload (the largest single-GPU configuration): 256^3 cells on the
 unit cube, rho = 1, v = 0, P = 1e-5, E = 1 deposited in the 123 cells within
 r <= 3 dx of the centre, gamma = 5/3, constant-extrapolation boundaries,
 HLLC with 2nd-order MUSCL (van Leer limiter), SSP-RK2, CFL 0.4 with the
@@ -330,8 +330,8 @@ def main():
         return
 
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build has no "
-              "multi-GPU path: see DESIGN.md for how the fluid step would shard.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the library steps "
+              "single-GPU canisters (nGPUs == 1) and has no multi-GPU path.", file=sys.stderr)
         sys.exit(2)
 
     from gpuimogen_amd import problems
