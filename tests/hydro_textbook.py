@@ -114,3 +114,127 @@ def rusanov_step(U, lam, g, c):
     Up, Fp = np.roll(U, -1, axis=1), np.roll(F, -1, axis=1)
     G = 0.5 * (F + Fp) - 0.5 * c * (Up - U)
     return U - lam * (G - np.roll(G, 1, axis=1))
+
+
+# ---------------------------------------------------------------- limiters, r-form
+# Half-slopes s(A, B) = B psi(A/B) / 2 from the textbook limiter functions
+# psi(r) of the slope ratio r = A/B (Sweby 1984; Waterson & Deconinck 2007 for
+# Ospre; van Albada et al. 1982), with psi = 0 for r <= 0.
+def psi_ospre(r):
+    return np.where(r > 0, 1.5 * (r * r + r) / (r * r + r + 1), 0.0)
+
+
+def psi_vanleer(r):
+    return np.where(r > 0, 2 * r / (1 + r), 0.0)
+
+
+def psi_vanalbada(r):
+    return np.where(r >= 0, (r * r + r) / (r * r + 1), 0.0)
+
+
+def psi_mc(r):
+    return np.where(r > 0, np.minimum(np.minimum(0.5 * (1 + r), 2 * r), 2.0), 0.0)
+
+
+def half_slope(psi, A, B):
+    """s = B psi(A/B) / 2 elementwise; 0 where B == 0 (both forms vanish there)."""
+    A = np.asarray(A, dtype=float)
+    B = np.asarray(B, dtype=float)
+    safe = np.where(B == 0, 1.0, B)
+    return np.where(B == 0, 0.0, 0.5 * B * psi(A / safe))
+
+
+# ---------------------------------------------------------------- MUSCL + SSP-RK2
+def to_cons(rho, e, v):
+    """Conserved (rho, E, m) from density, internal energy density e and velocity (3, n)."""
+    U = np.empty((5,) + np.shape(rho))
+    U[0] = rho
+    U[1] = e + 0.5 * rho * (v ** 2).sum(axis=0)
+    U[2:5] = rho * v
+    return U
+
+
+def muscl_faces(U, g, slope, recon="prim"):
+    """Left and right face states (conserved) of every cell of a periodic line.
+
+    recon "prim": limited slopes of density, internal energy density and
+    velocity (the variables a MUSCL scheme reconstructs when velocity, not
+    momentum, is the reconstruction variable); "cons": of the conserved
+    variables themselves; "prim_p_rho": density, velocity and the specific
+    internal energy e/rho. slope(A, B) is the half-slope of backward/forward
+    differences A, B."""
+    if recon == "cons":
+        W = U
+    else:
+        rho, v, p = prim(U, g)
+        e = p / (g - 1)
+        W = np.vstack([rho[None], (e / rho if recon == "prim_p_rho" else e)[None], v])
+    A = W - np.roll(W, 1, axis=1)
+    B = np.roll(W, -1, axis=1) - W
+    s = slope(A, B)
+    WL, WR = W - s, W + s
+
+    def back(X):
+        if recon == "cons":
+            return X
+        eint = X[1] * X[0] if recon == "prim_p_rho" else X[1]
+        return to_cons(X[0], eint, X[2:5])
+
+    return back(WL), back(WR)
+
+
+def muscl_heun_line(flux, U, lam, g, slope, recon="prim"):
+    """Second-order MUSCL + SSP-RK2 (Heun) on a periodic line with the
+    two-state flux `flux(UL, UR, g)`: interface i+1/2 takes the right face of
+    cell i and the left face of cell i+1."""
+    def dF(V):
+        L, R = muscl_faces(V, g, slope, recon)
+        F = flux(R, np.roll(L, -1, axis=1), g)
+        return F - np.roll(F, 1, axis=1)
+
+    U1 = U - lam * dF(U)
+    return 0.5 * (U1 + U - lam * dF(U1))
+
+
+# ---------------------------------------------------------------- Jin-Xin relaxation
+def jin_xin_line(U, lam, g, psi=psi_ospre, second_order=True, rhomin=0.0, emin=0.0):
+    """Relaxed Jin-Xin scheme on a periodic line (Jin & Xin, CPAM 48 (1995)
+    235, section 4), written in its interface-flux form with the frozen speed
+    c = max_i(|u_i| + c_s,i) over the line:
+        G_{i+1/2} = (w+_i + d+_i - w-_{i+1} + d-_{i+1}) / 2,  w+- = c U +- F(U),
+    where d+- are MUSCL edge corrections of the characteristic variables
+    (d+_i = +slope of w+ towards i+1/2, d-_{i+1} = +slope of w- towards
+    i+1/2; zero for the first-order predictor). Time stepping: half-step
+    first-order predictor, then the second-order corrector from the
+    predicted state (with its own frozen speed), each followed by the density
+    and internal-energy floors when rhomin/emin are given."""
+    def freeze(V):
+        rho, v, p = prim(V, g)
+        return np.max(np.abs(v[0]) + np.sqrt(g * p / rho))
+
+    def G(V, c, tvd):
+        F = phys_flux(V, g)
+        wp, wm = c * V + F, c * V - F
+        if tvd:
+            def slope(w):
+                A = w - np.roll(w, 1, axis=1)
+                B = np.roll(w, -1, axis=1) - w
+                return half_slope(psi, A, B)
+            wp = wp + slope(wp)           # right edge of cell i
+            wm = wm - slope(wm)           # left edge of cell i
+        return 0.5 * (wp - np.roll(wm, -1, axis=1))
+
+    def floor(V):
+        V = V.copy()
+        V[0] = np.maximum(V[0], rhomin)
+        ke = 0.5 * (V[2] ** 2 + V[3] ** 2 + V[4] ** 2) / V[0]
+        low = (V[1] - ke) < V[0] * emin
+        V[1] = np.where(low, ke + V[0] * emin, V[1])
+        return V
+
+    Gp = G(U, freeze(U), False)
+    Uh = floor(U - 0.5 * lam * (Gp - np.roll(Gp, 1, axis=1)))
+    if not second_order:
+        return Uh
+    Gc = G(Uh, freeze(Uh), True)
+    return floor(U - lam * (Gc - np.roll(Gc, 1, axis=1)))
