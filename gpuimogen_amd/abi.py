@@ -195,6 +195,7 @@ SIGNATURES = {
     "imogen_profile_read": (c_int, [c_int, _I64P, _DP]),
     "imogen_stream": (c_void_p, []),
     "imogen_synchronize": (c_int, []),
+    "imogen_check_errors": (c_int, []),
     "imogen_last_error": (ctypes.c_char_p, []),
     "imogen_version": (ctypes.c_char_p, []),
 }

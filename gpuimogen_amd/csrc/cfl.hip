@@ -7,7 +7,8 @@
 // through LDS, and a one-block second pass reduces the per-block minima, so
 // the result is deterministic and no host round trip per block is needed.
 // Comparisons use `u < m ? u : m` exactly as the reference does, so NaN cells
-// are skipped as they are there.
+// are skipped as they are there; the standalone pass also counts them into the
+// device error word, so the entry points can report the invalid state.
 #include "cfl_device.h"
 #include "imogen_internal.h"
 
@@ -25,9 +26,10 @@ __global__ void soundspeed_kernel(const double *__restrict__ q, int64_t pitch, i
 // 256-byte slab pitch guarantees both).
 __global__ __launch_bounds__(256) void cfl_pass1(const double *__restrict__ q, int64_t pitch, int64_t n,
                                                  const double *__restrict__ cs, CflArgs a,
-                                                 double *__restrict__ blockMin)
+                                                 double *__restrict__ blockMin, unsigned long long *__restrict__ err)
 {
     double m = 1e37;
+    unsigned nan = 0;
     const int64_t nth = (int64_t)gridDim.x * blockDim.x;
     const int64_t n2 = n / 2;
     const double2 *q2 = (const double2 *)q;
@@ -54,10 +56,13 @@ __global__ __launch_bounds__(256) void cfl_pass1(const double *__restrict__ q, i
         }
 #pragma unroll
         for (int h = 0; h < 2; h++) {
+            const bool fresh = (h == 0) || (i2 != i);  // i2 == i revisits the first pair
             double u = cfl_cell(a, r[h].x, e[h].x, x[h].x, y[h].x, z[h].x, c[h].x);
             m = (u < m) ? u : m;
+            nan += (u != u) && fresh;
             u = cfl_cell(a, r[h].y, e[h].y, x[h].y, y[h].y, z[h].y, c[h].y);
             m = (u < m) ? u : m;
+            nan += (u != u) && fresh;
         }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) {
@@ -65,7 +70,9 @@ __global__ __launch_bounds__(256) void cfl_pass1(const double *__restrict__ q, i
         const double u = cfl_cell(a, q[t], q[pitch + t], q[2 * pitch + t], q[3 * pitch + t], q[4 * pitch + t],
                                   a.fused ? 0.0 : cs[t]);
         m = (u < m) ? u : m;
+        nan += (u != u);
     }
+    wave_count(err + ERRW_CFL_NAN, nan);
     m = wave_min(m);
     __shared__ double part[4];
     const int w = threadIdx.x >> 6;
@@ -110,6 +117,9 @@ int launch_cfl(const double *base, int64_t pitch, int64_t n, const double *cs, d
                int dim, int xinjin, int halve, double hx, double hy, double hz, double hmin, double *devOut,
                hipStream_t s)
 {
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    unsigned long long *err = st->err;
     int64_t blocks = (n / 4 + 255) / 256;
     if (blocks > 2048) blocks = 2048;
     if (blocks < 1) blocks = 1;
@@ -117,7 +127,7 @@ int launch_cfl(const double *base, int64_t pitch, int64_t n, const double *cs, d
         return IM_FAIL(ERROR_INVALID_ARGS, "CFL needs 16-byte aligned slabs with an even pitch");
     double *blockMin = devOut + 8;
     CflArgs a{gamma * (gamma - 1), hx, hy, hz, hmin, fused, dim, xinjin};
-    hipLaunchKernelGGL(cfl_pass1, dim3((unsigned)blocks), dim3(256), 0, s, base, pitch, n, cs, a, blockMin);
+    hipLaunchKernelGGL(cfl_pass1, dim3((unsigned)blocks), dim3(256), 0, s, base, pitch, n, cs, a, blockMin, err);
     int rc = IM_LAUNCHED("cfl_pass1");
     if (rc) return rc;
     hipLaunchKernelGGL(cfl_pass2, dim3(1), dim3(256), 0, s, blockMin, (int)blocks, halve ? 1.0 : 0.0, devOut);

@@ -28,6 +28,16 @@ __device__ __forceinline__ double wave_min(double v)
     return v;
 }
 
+// Adds the wave's total of the lanes' counts to *slot with one atomic (none
+// when the total is 0). Must be reached by all 64 lanes; slot null: no-op.
+__device__ __forceinline__ void wave_count(unsigned long long *slot, unsigned v)
+{
+    if (!slot) return;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(slot, (unsigned long long)v);
+}
+
 // dim = simulation dimension (1..3), xinjin selects the per-direction form.
 // fused != 0: sound speed computed in-line from gamma instead of read from cs.
 struct CflArgs {

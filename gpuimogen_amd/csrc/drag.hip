@@ -18,21 +18,27 @@
 // in registers: 72 bytes read and 56 bytes written per cell.
 #include <cstring>
 
+#include "cfl_device.h"
 #include "drag_device.h"
 #include "imogen_internal.h"
 
 namespace imogen {
 
+// Cells the method cannot be applied to (drag_cell_defined) are updated as
+// the arithmetic falls (NaN, as the reference's kernels do) and counted into
+// err[ERRW_DRAG_INVALID]; the adaptive integrator's budget overruns into
+// err[ERRW_DRAG_BUDGET].
 template <int METHOD>
 __global__ __launch_bounds__(256) void drag_kernel(double *__restrict__ g, double *__restrict__ d, int64_t pitch,
                                                    int64_t n, DragConsts c, DragDt dtv,
-                                                   unsigned long long *__restrict__ work)
+                                                   unsigned long long *__restrict__ err)
 {
     const double dt = drag_dt(dtv);
-    unsigned long long steps = 0;
+    unsigned invalid = 0, overrun = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         DragCell s = drag_load(g + i, d + i, pitch);
         if (METHOD == DRAG_ETDRK1) {
+            invalid += !drag_cell_defined(s.rg, s.rd, s.s0, s.u0);
             drag_etd1_apply(g + i, d + i, pitch, dt, drag_k(c, s.rg, s.rd, s.s0, s.u0));
             continue;
         }
@@ -40,29 +46,32 @@ __global__ __launch_bounds__(256) void drag_kernel(double *__restrict__ g, doubl
         if (METHOD == DRAG_EXACT) {
             int k = 0;
             f = drag_exact_ratio(c, s.rg, s.rd, s.s0, s.u0, dt, &k);
-            steps += (unsigned)(k < 0 ? -k : k);
+            invalid += (k == kDragInvalid);
+            overrun += (k < 0 && k != kDragInvalid);
         } else {
+            invalid += !drag_cell_defined(s.rg, s.rd, s.s0, s.u0);
             f = exp(-dt * drag_logtrap_rate(c, s.rg, s.rd, s.s0, s.u0, dt, METHOD == DRAG_LOGTRAP3 ? 3 : 2));
         }
         drag_apply(g + i, d + i, pitch, s, f);
     }
-    if (work) {
-        // one atomic per wave
-        for (int off = 32; off > 0; off >>= 1) steps += __shfl_xor(steps, off, 64);
-        if ((threadIdx.x & 63) == 0 && steps) atomicAdd(work, steps);
-    }
+    wave_count(err + ERRW_DRAG_INVALID, invalid);
+    if (METHOD == DRAG_EXACT) wave_count(err + ERRW_DRAG_BUDGET, overrun);
 }
 
 // explicit midpoint (method 0) / RK4 (method 1): 72 bytes read, 64 written per cell
 template <int METHOD>
 __global__ __launch_bounds__(256) void drag_rk_kernel(double *__restrict__ g, double *__restrict__ d, int64_t pitch,
-                                                      int64_t n, DragConsts c, DragDt dtv)
+                                                      int64_t n, DragConsts c, DragDt dtv,
+                                                      unsigned long long *__restrict__ err)
 {
     const double dt = drag_dt(dtv);
+    unsigned invalid = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const DragCell s = drag_rk_load(g + i, d + i, pitch);
+        invalid += !drag_cell_defined(s.rg, s.rd, s.s0, s.u0);
         drag_rk_apply(g + i, d + i, pitch, drag_rk_magdv<METHOD>(c, s.rg, s.rd, s.s0, s.u0, dt));
     }
+    wave_count(err + ERRW_DRAG_INVALID, invalid);
 }
 
 DragConsts drag_consts(const ThermoDetails *tg, const ThermoDetails *td)
@@ -84,33 +93,36 @@ DragConsts drag_consts(const ThermoDetails *tg, const ThermoDetails *td)
 }
 
 int launch_drag(double *g, double *d, int64_t pitch, int64_t n, const DragConsts &c, DragDt dt, int method,
-                unsigned long long *work, hipStream_t s)
+                hipStream_t s)
 {
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    unsigned long long *err = st->err;
     int64_t blocks = (n + 255) / 256;
     if (blocks > 8192) blocks = 8192;
     if (blocks < 1) blocks = 1;
     switch (method) {
     case DRAG_LOGTRAP2:
         hipLaunchKernelGGL((drag_kernel<DRAG_LOGTRAP2>), dim3((unsigned)blocks), dim3(256), 0, s, g, d, pitch, n, c,
-                           dt, work);
+                           dt, err);
         break;
     case DRAG_LOGTRAP3:
         hipLaunchKernelGGL((drag_kernel<DRAG_LOGTRAP3>), dim3((unsigned)blocks), dim3(256), 0, s, g, d, pitch, n, c,
-                           dt, work);
+                           dt, err);
         break;
     case DRAG_EXACT:
         hipLaunchKernelGGL((drag_kernel<DRAG_EXACT>), dim3((unsigned)blocks), dim3(256), 0, s, g, d, pitch, n, c,
-                           dt, work);
+                           dt, err);
         break;
     case DRAG_ETDRK1:
         hipLaunchKernelGGL((drag_kernel<DRAG_ETDRK1>), dim3((unsigned)blocks), dim3(256), 0, s, g, d, pitch, n, c,
-                           dt, work);
+                           dt, err);
         break;
     case DRAG_EMP:
-        hipLaunchKernelGGL((drag_rk_kernel<DRAG_EMP>), dim3((unsigned)blocks), dim3(256), 0, s, g, d, pitch, n, c, dt);
+        hipLaunchKernelGGL((drag_rk_kernel<DRAG_EMP>), dim3((unsigned)blocks), dim3(256), 0, s, g, d, pitch, n, c, dt, err);
         break;
     case DRAG_RK4:
-        hipLaunchKernelGGL((drag_rk_kernel<DRAG_RK4>), dim3((unsigned)blocks), dim3(256), 0, s, g, d, pitch, n, c, dt);
+        hipLaunchKernelGGL((drag_rk_kernel<DRAG_RK4>), dim3((unsigned)blocks), dim3(256), 0, s, g, d, pitch, n, c, dt, err);
         break;
     default: return IM_FAIL(ERROR_INVALID_ARGS, "drag method %d", method);
     }
@@ -152,7 +164,7 @@ int sourcefunction_2FluidDrag(MGArray *fluidA, MGArray *fluidB, GeometryParams *
     case DRAG_EXACT:
     case DRAG_EMP:
     case DRAG_RK4:
-    case DRAG_ETDRK1: rc = launch_drag(g, d, pg, n, c, host_dt(dt), method, nullptr, st->stream); break;
+    case DRAG_ETDRK1: rc = launch_drag(g, d, pg, n, c, host_dt(dt), method, st->stream); break;
     case DRAG_LEARNED: rc = launch_learned_drag(g, d, pg, n, c, host_dt(dt), nullptr, st->stream); break;
     case DRAG_ETDRK2:  // listed as not implemented by the reference itself (classes/ImogenManager.m:181)
         rc = IM_FAIL(ERROR_NOIMPLEMENT, "drag method %d is not built", method);
@@ -160,7 +172,8 @@ int sourcefunction_2FluidDrag(MGArray *fluidA, MGArray *fluidB, GeometryParams *
     default: rc = IM_FAIL(ERROR_INVALID_ARGS, "unknown drag method %d", method);
     }
     prof_end(slot, st->stream);
-    return rc;
+    if (rc) return rc;
+    return take_device_errors(st);  // waits: reports cells the method could not be applied to
 }
 
 int performSourceFunctions(int srcType, GridFluid *fluids, int nFluids, FluidStepParams fsp, ParallelTopology *topo,

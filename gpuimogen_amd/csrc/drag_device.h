@@ -141,19 +141,45 @@ __device__ __forceinline__ double drag_exact_rhs(const DragConsts &c, double rho
     return -drag_k(c, rhog, rhod, s, u0 + h * (s0sq - s * s));
 }
 
+// Whether the relaxation problem is defined on a cell: positive finite
+// densities, a finite |dv| and, when the cell moves (|dv| >= 1e-9, the rate's
+// cut-off), a positive finite gas internal energy.
+__device__ __forceinline__ bool drag_cell_defined(double rhog, double rhod, double s0, double u0)
+{
+    const double big = 1.7976931348623157e308;
+    return (rhog > 0 && rhog <= big) && (rhod > 0 && rhod <= big) && (s0 <= big) &&
+           (s0 < 1e-9 || (u0 > 0 && u0 <= big));
+}
+
+// drag_exact_ratio's step count for a cell the problem is not defined on
+constexpr int kDragInvalid = -(1 << 30);
+
 // |dv(t)|/|dv(0)|: adaptive Dormand-Prince 5(4) (J. Comput. Appl. Math. 6
 // (1980) 19) on y = ln|dv|, absolute tolerance c.tol on y, first trial step t.
 // *steps receives the number of steps tried (negative if the budget ran out).
+// A cell the problem is not defined on (drag_cell_defined), or whose rate is
+// not finite, returns NaN at once with *steps = kDragInvalid; a trial step
+// whose error estimate is not finite (an overflowing stage of a too-long
+// step) is rejected and shrunk by the minimum factor.
 __device__ __forceinline__ double drag_exact_ratio(const DragConsts &c, double rhog, double rhod, double s0,
                                                    double u0, double t, int *steps)
 {
+    const double big = 1.7976931348623157e308;
     *steps = 0;
+    if (!drag_cell_defined(rhog, rhod, s0, u0)) {
+        *steps = kDragInvalid;
+        return __builtin_nan("");
+    }
     if (s0 < 1e-9 || t <= 0) return 1.0;
     const double hc = .5 * rhod / (rhog + rhod);
     const double s0sq = s0 * s0;
     const double y0 = log(s0);
     double y = y0, tt = 0.0, h = t;
     double k1 = drag_exact_rhs(c, rhog, rhod, s0sq, u0, hc, y);
+    if (!(fabs(k1) <= big)) {
+        *steps = kDragInvalid;
+        return __builtin_nan("");
+    }
     int n = 0;
     while (tt < t) {
         if (n >= 4096) {
@@ -178,7 +204,12 @@ __device__ __forceinline__ double drag_exact_ratio(const DragConsts &c, double r
                           (17253.0 / 339200) * k5 + (22.0 / 525) * k6 - (1.0 / 40) * k7);
         err = fabs(err) / c.tol;
         n++;
+        if (!(err <= big)) err = 1e300;  // reject, shrink by the minimum factor
         if (err <= 1.0) {
+            if (!(fabs(k7) <= big)) {
+                *steps = kDragInvalid;
+                return __builtin_nan("");
+            }
             tt += h;
             y = yn;
             k1 = k7;
@@ -307,8 +338,10 @@ inline DragDt host_dt(double dt)
 }
 
 DragConsts drag_consts(const ThermoDetails *tg, const ThermoDetails *td);
+// Every drag kernel counts the cells the method cannot be applied to (and the
+// adaptive integrator's budget overruns) into the device error word.
 int launch_drag(double *g, double *d, int64_t pitch, int64_t n, const DragConsts &c, DragDt dt, int method,
-                unsigned long long *work, hipStream_t s);
+                hipStream_t s);
 // queue: one int64 per cell, or null for the library's scratch block
 int launch_learned_drag(double *g, double *d, int64_t pitch, int64_t n, const DragConsts &c, DragDt dt,
                         int64_t *queue, hipStream_t s);

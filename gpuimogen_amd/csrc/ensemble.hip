@@ -66,17 +66,20 @@ static int x_class_of(int N)
 // kernels
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void ens_cfl_pass1(const EnsProblem *__restrict__ probs,
-                                                     double *__restrict__ part)
+                                                     double *__restrict__ part, unsigned long long *__restrict__ err)
 {
     const EnsProblem P = probs[blockIdx.y];
     const int64_t n = (int64_t)P.nx * P.ny;
     CflArgs a{P.gamma * (P.gamma - 1), P.hx, P.hy, 1.0, P.hx < P.hy ? P.hx : P.hy, 1, 2, 0};
     double m = 1e37;
+    unsigned nan = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const double *c = P.q + i;
         const double u = cfl_cell(a, c[0], c[P.pitch], c[2 * P.pitch], c[3 * P.pitch], c[4 * P.pitch], 0.0);
         m = (u < m) ? u : m;
+        nan += (u != u);
     }
+    wave_count(err + ERRW_CFL_NAN, nan);
     m = wave_min(m);
     __shared__ double w[4];
     if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = m;
@@ -89,10 +92,13 @@ __global__ __launch_bounds__(256) void ens_cfl_pass1(const EnsProblem *__restric
 }
 
 // one wave per problem: tau (halved, HLLC rule cflTimestep.cu:232-237), dt,
-// lambda per direction, and the problem's clock t += 2 dt
+// lambda per direction, and the problem's clock t += 2 dt. A limit that is
+// not positive and below 1e36 (every cell NaN, or no cell limiting: the 1e37
+// sentinel) takes no time step (dt = 0), counted in the device error word.
 __global__ __launch_bounds__(64) void ens_cfl_pass2(const EnsProblem *__restrict__ probs, const double *part,
                                                     int nb, double cfl, double *__restrict__ lam,
-                                                    double *__restrict__ clock, double *__restrict__ dtp)
+                                                    double *__restrict__ clock, double *__restrict__ dtp,
+                                                    unsigned long long *__restrict__ err)
 {
     const int p = blockIdx.x;
     const int de = probs[p].dust;
@@ -109,7 +115,11 @@ __global__ __launch_bounds__(64) void ens_cfl_pass2(const EnsProblem *__restrict
     if (threadIdx.x == 0) {
         const double tg = m / 2, td = md / 2;
         const double tau = (de >= 0 && td < tg) ? td : tg;
-        const double dt = cfl * tau;
+        double dt = cfl * tau;
+        if (!(tau > 0 && tau < 1e36)) {
+            dt = 0.0;
+            atomicAdd(err + ERRW_TIMESTEP, 1ull);
+        }
         lam[2 * p] = dt / probs[p].hx;
         lam[2 * p + 1] = dt / probs[p].hy;
         if (de >= 0) {
@@ -239,35 +249,43 @@ __global__ __launch_bounds__(256) void ens_bc_const(const EnsProblem *__restrict
 {
     const EnsProblem P = probs[blockIdx.x];
     const int nx = P.nx, ny = P.ny;
-    if (dir == 1) {
-        if (nx < d + 2) return;
-        const int64_t total = (int64_t)ny * 5 * 2 * d;
-        for (int64_t t = threadIdx.x; t < total; t += blockDim.x) {
-            const int g = (int)(t % d);
-            int64_t r = t / d;
-            const int side = (int)(r % 2);
-            r /= 2;
-            const int slab = (int)(r % 5);
-            const int64_t row = r / 5;
-            double *a = P.q + slab * P.pitch + row * nx;
-            if (side == 0) a[g] = a[d];
-            else a[nx - d + g] = a[nx - d - 1];
-        }
-    } else {
-        if (ny < d + 2) return;
-        const int64_t total = (int64_t)nx * 5 * 2;
-        for (int64_t t = threadIdx.x; t < total; t += blockDim.x) {
-            const int x = (int)(t % nx);
-            int64_t r = t / nx;
-            const int side = (int)(r % 2);
-            const int slab = (int)(r / 2);
-            double *a = P.q + slab * P.pitch + x;
-            if (side == 0) {
-                const double f = a[(int64_t)d * nx];
-                for (int g = 0; g < d; g++) a[(int64_t)g * nx] = f;
-            } else {
-                const double f = a[(int64_t)(xrules ? ny - d - 1 : ny - d) * nx];
-                for (int g = 0; g < d; g++) a[(int64_t)(ny - d + g) * nx] = f;
+    const int N = (dir == 1) ? nx : ny;
+    if (N < d + 2) return;
+    // Both faces in one pass only when the cells one face reads are disjoint
+    // from those the other writes (N >= 3 d + 2); otherwise the minus face
+    // first, then the plus face, as the reference's two calls run.
+    const bool together = N >= 3 * d + 2;
+    for (int pass = 0; pass < (together ? 1 : 2); pass++) {
+        if (pass) __syncthreads();
+        const int nsides = together ? 2 : 1;
+        if (dir == 1) {
+            const int64_t total = (int64_t)ny * 5 * nsides * d;
+            for (int64_t t = threadIdx.x; t < total; t += blockDim.x) {
+                const int g = (int)(t % d);
+                int64_t r = t / d;
+                const int side = together ? (int)(r % 2) : pass;
+                r /= nsides;
+                const int slab = (int)(r % 5);
+                const int64_t row = r / 5;
+                double *a = P.q + slab * P.pitch + row * nx;
+                if (side == 0) a[g] = a[d];
+                else a[nx - d + g] = a[nx - d - 1];
+            }
+        } else {
+            const int64_t total = (int64_t)nx * 5 * nsides;
+            for (int64_t t = threadIdx.x; t < total; t += blockDim.x) {
+                const int x = (int)(t % nx);
+                int64_t r = t / nx;
+                const int side = together ? (int)(r % 2) : pass;
+                const int slab = (int)(r / nsides);
+                double *a = P.q + slab * P.pitch + x;
+                if (side == 0) {
+                    const double f = a[(int64_t)d * nx];
+                    for (int g = 0; g < d; g++) a[(int64_t)g * nx] = f;
+                } else {
+                    const double f = a[(int64_t)(xrules ? ny - d - 1 : ny - d) * nx];
+                    for (int g = 0; g < d; g++) a[(int64_t)(ny - d + g) * nx] = f;
+                }
             }
         }
     }
@@ -427,11 +445,13 @@ static int ens_sweep(EnsembleImpl *e, int dir, int rt, hipStream_t s)
 static int ens_iteration(EnsembleImpl *e, int it, hipStream_t s)
 {
     static const int kFluxcall[3][6] = {{1, 2, 1, 2, 3, 3}, {3, 1, 2, 3, 1, 2}, {2, 3, 3, 1, 2, 1}};
-    hipLaunchKernelGGL(ens_cfl_pass1, dim3(kEnsCflBlocks, (unsigned)e->NE), dim3(256), 0, s, e->dprobs, e->cflPart);
+    unsigned long long *err = state()->err;
+    hipLaunchKernelGGL(ens_cfl_pass1, dim3(kEnsCflBlocks, (unsigned)e->NE), dim3(256), 0, s, e->dprobs, e->cflPart,
+                       err);
     int rc = IM_LAUNCHED("ens_cfl_pass1");
     if (rc) return rc;
     hipLaunchKernelGGL(ens_cfl_pass2, dim3((unsigned)e->P), dim3(64), 0, s, e->dprobs, e->cflPart, kEnsCflBlocks,
-                       e->cfl, e->lam, e->clock, e->dtp);
+                       e->cfl, e->lam, e->clock, e->dtp, err);
     rc = IM_LAUNCHED("ens_cfl_pass2");
     if (rc) return rc;
     if ((rc = ens_source(e, 0.5, s))) return rc;
@@ -700,7 +720,7 @@ int imogen_ensemble_run(ImogenEnsemble *E, int firstIteration, int iterations, i
         }
         if (IM_HIP(hipGraphLaunch(g, st->stream))) return ERROR_CUDA_BLEW_UP;
     }
-    return SUCCESSFUL;
+    return take_device_errors(st);
 }
 
 int imogen_ensemble_times(ImogenEnsemble *E, double *t)

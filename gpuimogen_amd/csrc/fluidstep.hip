@@ -368,7 +368,7 @@ static int cfl_common(MGArray *fluid, MGArray *csound, GeometryParams *geom, int
     rc = IM_HIP(hipStreamSynchronize(st->stream));
     if (rc) return rc;
     tstep[0] = st->pinned[0];
-    return SUCCESSFUL;
+    return take_device_errors(st);
 }
 
 int computeLocalCFLTimestep(MGArray *fluid, MGArray *csound, GeometryParams *geom, int method,

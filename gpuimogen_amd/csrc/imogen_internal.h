@@ -22,6 +22,16 @@ int fail(int status, const char *file, int line, const char *fmt, ...);
 #define IM_LAUNCHED(name) ::imogen::check_hip(hipGetLastError(), name, __FILE__, __LINE__)
 #define IM_FAIL(status, ...) ::imogen::fail((status), __FILE__, __LINE__, __VA_ARGS__)
 
+// Counters of invalid states that kernels meet (the device error word):
+// kernels add to them, take_device_errors turns them into ERROR_CRASH.
+enum DeviceErrorSlot {
+    ERRW_DRAG_INVALID = 0,  // drag cells with a density, |dv| or (moving) internal energy not positive/finite
+    ERRW_DRAG_BUDGET = 1,   // cells whose adaptive drag integration ran out of steps
+    ERRW_CFL_NAN = 2,       // cells whose CFL limit is NaN (a standalone CFL pass)
+    ERRW_TIMESTEP = 3,      // time steps not taken: tau not positive and finite (device clocks)
+    ERRW_SLOTS = 8
+};
+
 // Per-device state: one stream all kernels go to, a growable scratch block
 // and a small pinned host buffer for scalar read-back.
 struct DeviceState {
@@ -31,10 +41,16 @@ struct DeviceState {
     size_t scratchBytes = 0;
     double *pinned = nullptr;   // 64 doubles of pinned host memory
     double *devScalars = nullptr;  // 4096 doubles of device memory for reductions
+    unsigned long long *err = nullptr;      // ERRW_SLOTS device counters, zero when no error is pending
+    unsigned long long *errHost = nullptr;  // their pinned host copy
 };
 
 // State of the current device (initialised on first use).
 DeviceState *state();
+// Waits for the stream, reads the device error counters and, when any is
+// set, clears them and returns ERROR_CRASH naming what was counted. Never
+// call it while the stream is being captured.
+int take_device_errors(DeviceState *st);
 
 // Optional per-kernel timing with HIP events on the launch stream
 // (imogen_profile_enable / imogen_profile_read). prof_begin returns a slot

@@ -23,6 +23,7 @@
 #include <cstring>
 #include <vector>
 
+#include "cfl_device.h"
 #include "drag_device.h"
 #include "imogen_internal.h"
 
@@ -238,7 +239,10 @@ __device__ __forceinline__ void learned_cells(double *__restrict__ g, double *__
     const bool live = i < n;
     const int64_t ic = live ? i : n - 1;  // lanes past the end work on a valid cell and store nothing
     const DragCell s = drag_load(g + ic, d + ic, pitch);
-    const bool moving = s.s0 >= 1e-9;
+    // cells the problem is not defined on go to the exact integrator, which
+    // flags them (drag_exact_ratio)
+    const bool defined = drag_cell_defined(s.rg, s.rd, s.s0, s.u0);
+    const bool moving = defined && s.s0 >= 1e-9;
     DragRate r0 = {0.0, 1.0, 1.0, 0.0};
     if (moving) r0 = drag_k_parts(c, s.rg, s.rd, s.s0, s.u0);
     float f[5];
@@ -253,7 +257,7 @@ __device__ __forceinline__ void learned_cells(double *__restrict__ g, double *__
     float o[3];
     mlp_wave(m, z, o);
     double fac = 1.0;
-    bool reject = false;
+    bool reject = live && !defined;
     if (moving) {
         const double tau = dt * r0.k;
         const double q0 = inside ? (double)(o[0] * m.norm.tstd[0] + m.norm.tmean[0]) : 0.0;
@@ -300,7 +304,8 @@ __global__ __launch_bounds__(256) void fallback_drag_kernel(double *__restrict__
                                                             int64_t pitch, int64_t n, DragConsts c, DragDt dtv,
                                                             const int64_t *__restrict__ queue,
                                                             const unsigned long long *__restrict__ qcount,
-                                                            unsigned long long *__restrict__ counts)
+                                                            unsigned long long *__restrict__ counts,
+                                                            unsigned long long *__restrict__ err)
 {
     const double dt = drag_dt(dtv);
     const unsigned long long nq = *qcount;
@@ -308,14 +313,19 @@ __global__ __launch_bounds__(256) void fallback_drag_kernel(double *__restrict__
         atomicAdd(counts, (unsigned long long)n);
         if (nq) atomicAdd(counts + 1, nq);
     }
+    unsigned invalid = 0, overrun = 0;
     for (unsigned long long t = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; t < nq;
          t += (unsigned long long)gridDim.x * blockDim.x) {
         const int64_t i = queue[t];
         const DragCell s = drag_load(g + i, d + i, pitch);
         int steps;
         const double fac = drag_exact_ratio(c, s.rg, s.rd, s.s0, s.u0, dt, &steps);
+        invalid += (steps == kDragInvalid);
+        overrun += (steps < 0 && steps != kDragInvalid);
         drag_apply(g + i, d + i, pitch, s, fac);
     }
+    wave_count(err + ERRW_DRAG_INVALID, invalid);
+    wave_count(err + ERRW_DRAG_BUDGET, overrun);
 }
 
 __global__ void zero_counter(unsigned long long *p)
@@ -331,7 +341,8 @@ __global__ __launch_bounds__(256) void ens_drag_kernel(const EnsDragTable *__res
                                                        const EnsDragItem *__restrict__ items, int nitems,
                                                        const double *__restrict__ dtp, double frac, DragConsts c,
                                                        int method, MlpDev m, int64_t *__restrict__ queue,
-                                                       unsigned long long *__restrict__ qcount)
+                                                       unsigned long long *__restrict__ qcount,
+                                                       unsigned long long *__restrict__ err)
 {
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     if (wave >= nitems) return;
@@ -343,29 +354,38 @@ __global__ __launch_bounds__(256) void ens_drag_kernel(const EnsDragTable *__res
         return;
     }
     const int64_t i = it.base + (threadIdx.x & 63);
-    if (i >= T.n) return;
-    const DragCell s = drag_load(T.g + i, T.d + i, T.pitch);
-    double f;
-    if (method == DRAG_EXACT) {
-        int k;
-        f = drag_exact_ratio(c, s.rg, s.rd, s.s0, s.u0, dt, &k);
-    } else {
-        f = exp(-dt * drag_logtrap_rate(c, s.rg, s.rd, s.s0, s.u0, dt, method == DRAG_LOGTRAP3 ? 3 : 2));
+    unsigned invalid = 0, overrun = 0;
+    if (i < T.n) {
+        const DragCell s = drag_load(T.g + i, T.d + i, T.pitch);
+        double f;
+        if (method == DRAG_EXACT) {
+            int k;
+            f = drag_exact_ratio(c, s.rg, s.rd, s.s0, s.u0, dt, &k);
+            invalid = (k == kDragInvalid);
+            overrun = (k < 0 && k != kDragInvalid);
+        } else {
+            invalid = !drag_cell_defined(s.rg, s.rd, s.s0, s.u0);
+            f = exp(-dt * drag_logtrap_rate(c, s.rg, s.rd, s.s0, s.u0, dt, method == DRAG_LOGTRAP3 ? 3 : 2));
+        }
+        drag_apply(T.g + i, T.d + i, T.pitch, s, f);
     }
-    drag_apply(T.g + i, T.d + i, T.pitch, s, f);
+    wave_count(err + ERRW_DRAG_INVALID, invalid);
+    wave_count(err + ERRW_DRAG_BUDGET, overrun);
 }
 
 __global__ __launch_bounds__(256) void ens_fallback_kernel(const EnsDragTable *__restrict__ tab,
                                                            const double *__restrict__ dtp, double frac, DragConsts c,
                                                            const int64_t *__restrict__ queue,
                                                            const unsigned long long *__restrict__ qcount,
-                                                           int64_t ncells, unsigned long long *__restrict__ counts)
+                                                           int64_t ncells, unsigned long long *__restrict__ counts,
+                                                           unsigned long long *__restrict__ err)
 {
     const unsigned long long nq = *qcount;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         atomicAdd(counts, (unsigned long long)ncells);
         if (nq) atomicAdd(counts + 1, nq);
     }
+    unsigned invalid = 0, overrun = 0;
     for (unsigned long long t = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; t < nq;
          t += (unsigned long long)gridDim.x * blockDim.x) {
         const int64_t e = queue[t];
@@ -374,8 +394,12 @@ __global__ __launch_bounds__(256) void ens_fallback_kernel(const EnsDragTable *_
         const DragCell s = drag_load(T.g + i, T.d + i, T.pitch);
         int steps;
         const double fac = drag_exact_ratio(c, s.rg, s.rd, s.s0, s.u0, frac * dtp[T.p], &steps);
+        invalid += (steps == kDragInvalid);
+        overrun += (steps < 0 && steps != kDragInvalid);
         drag_apply(T.g + i, T.d + i, T.pitch, s, fac);
     }
+    wave_count(err + ERRW_DRAG_INVALID, invalid);
+    wave_count(err + ERRW_DRAG_BUDGET, overrun);
 }
 
 // standalone BF16 inference (tests)
@@ -687,7 +711,7 @@ int launch_learned_drag(double *g, double *d, int64_t pitch, int64_t n, const Dr
     rc = IM_LAUNCHED("learned_drag_kernel");
     if (rc) return rc;
     hipLaunchKernelGGL(fallback_drag_kernel, dim3(1024), dim3(256), 0, s, g, d, pitch, n, c, dt, queue, qcount,
-                       g_model.counts);
+                       g_model.counts, state()->err);
     prof_end(slot, s);
     return IM_LAUNCHED("fallback_drag_kernel");
 }
@@ -706,13 +730,14 @@ int launch_ens_drag(const EnsDragTable *tab, const EnsDragItem *items, int nitem
         int rc = IM_LAUNCHED("zero_counter");
         if (rc) return rc;
     }
+    unsigned long long *err = state()->err;
     hipLaunchKernelGGL(ens_drag_kernel, dim3((unsigned)((nitems + 3) / 4)), dim3(256), 0, s, tab, items, nitems, dtp,
-                       frac, c, method, g_model.dev, queue, qcount);
+                       frac, c, method, g_model.dev, queue, qcount, err);
     int rc = IM_LAUNCHED("ens_drag_kernel");
     if (rc || method != DRAG_LEARNED) return rc;
     if (!g_model.counts) return IM_FAIL(ERROR_INVALID_ARGS, "learned drag counters not prepared (learned_drag_prepare)");
     hipLaunchKernelGGL(ens_fallback_kernel, dim3(1024), dim3(256), 0, s, tab, dtp, frac, c, queue, qcount, ncells,
-                       g_model.counts);
+                       g_model.counts, err);
     return IM_LAUNCHED("ens_fallback_kernel");
 }
 

@@ -48,9 +48,36 @@ DeviceState *state()
         if (IM_HIP(hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking))) return nullptr;
         if (IM_HIP(hipHostMalloc((void **)&st->pinned, 64 * sizeof(double), hipHostMallocDefault))) return nullptr;
         if (IM_HIP(hipMalloc((void **)&st->devScalars, 4096 * sizeof(double)))) return nullptr;
+        if (IM_HIP(hipMalloc((void **)&st->err, ERRW_SLOTS * sizeof(unsigned long long)))) return nullptr;
+        if (IM_HIP(hipHostMalloc((void **)&st->errHost, ERRW_SLOTS * sizeof(unsigned long long),
+                                 hipHostMallocDefault)))
+            return nullptr;
+        if (IM_HIP(hipMemsetAsync(st->err, 0, ERRW_SLOTS * sizeof(unsigned long long), st->stream))) return nullptr;
+        if (IM_HIP(hipStreamSynchronize(st->stream))) return nullptr;
         st->device = dev;
     }
     return st;
+}
+
+int take_device_errors(DeviceState *st)
+{
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    const size_t bytes = ERRW_SLOTS * sizeof(unsigned long long);
+    if (IM_HIP(hipMemcpyAsync(st->errHost, st->err, bytes, hipMemcpyDeviceToHost, st->stream))) return ERROR_CUDA_BLEW_UP;
+    if (IM_HIP(hipStreamSynchronize(st->stream))) return ERROR_CUDA_BLEW_UP;
+    const unsigned long long *e = st->errHost;
+    bool any = false;
+    for (int k = 0; k < ERRW_SLOTS; k++) any = any || e[k];
+    if (!any) return SUCCESSFUL;
+    const unsigned long long c0 = e[ERRW_DRAG_INVALID], c1 = e[ERRW_DRAG_BUDGET], c2 = e[ERRW_CFL_NAN],
+                             c3 = e[ERRW_TIMESTEP];
+    if (IM_HIP(hipMemsetAsync(st->err, 0, bytes, st->stream))) return ERROR_CUDA_BLEW_UP;
+    if (IM_HIP(hipStreamSynchronize(st->stream))) return ERROR_CUDA_BLEW_UP;
+    return IM_FAIL(ERROR_CRASH,
+                   "invalid fluid state: %llu drag cells with non-positive or non-finite density, |dv| or internal "
+                   "energy; %llu cells exhausted the drag integrator's step budget; %llu cells with a NaN CFL limit; "
+                   "%llu time steps not taken (CFL limit not positive and finite)",
+                   c0, c1, c2, c3);
 }
 
 void *scratch(size_t bytes)
@@ -157,6 +184,11 @@ int imogen_synchronize(void)
     DeviceState *st = state();
     if (!st) return ERROR_CUDA_BLEW_UP;
     return IM_HIP(hipStreamSynchronize(st->stream));
+}
+
+int imogen_check_errors(void)
+{
+    return take_device_errors(state());
 }
 
 int imogen_device_alloc(size_t bytes, void **ptr)

@@ -32,8 +32,12 @@ constexpr int kClockDoubles = 8;
 constexpr int kCflSlot = 2056;  // tau + 7 spare + up to 2048 block minima (launch_cfl)
 constexpr int kMaxFluids = 4;
 
+// A limit tau that is not positive and below 1e36 (every cell NaN, or no cell
+// limiting: the reductions' 1e37 / 1e38 sentinels) takes no time step: dTime
+// = 0 and the refusal is counted in the device error word.
 __global__ void clock_advance(const double *__restrict__ cflOut, int nfluids, double cfl, double tmax,
-                              double hx, double hy, double hz, double *__restrict__ clk)
+                              double hx, double hy, double hz, double *__restrict__ clk,
+                              unsigned long long *__restrict__ err)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     double tau = cflOut[0];
@@ -44,6 +48,10 @@ __global__ void clock_advance(const double *__restrict__ cflOut, int nfluids, do
     double dt = cfl * tau;
     const double t = clk[0];
     if (t + 2 * dt > tmax) dt = 0.5 * (tmax - t);
+    if (!(tau > 0 && tau < 1e36)) {
+        dt = 0.0;
+        atomicAdd(err + ERRW_TIMESTEP, 1ull);
+    }
     clk[1] = dt;
     clk[2] = dt / hx;
     clk[3] = dt / hy;
@@ -145,7 +153,7 @@ static int source(TimeloopImpl *L, double fraction, hipStream_t s)
     for (int call = 0; call < 2; call++) {
         const int slot = prof_begin(PROF_DRAG, s);
         rc = (L->dragMethod == DRAG_LEARNED) ? launch_learned_drag(g, d, pg, n, L->dc, dt, nullptr, s)
-                                             : launch_drag(g, d, pg, n, L->dc, dt, L->dragMethod, nullptr, s);
+                                             : launch_drag(g, d, pg, n, L->dc, dt, L->dragMethod, s);
         prof_end(slot, s);
         if (rc) return rc;
     }
@@ -172,7 +180,7 @@ static int iteration(TimeloopImpl *L, int it, bool fusedCfl, hipStream_t s)
         if (rc) return rc;
     }
     hipLaunchKernelGGL(clock_advance, dim3(1), dim3(64), 0, s, L->dbuf + kClockDoubles, L->nf, L->cfl, L->tmax,
-                       L->geo.h[0], L->geo.h[1], L->geo.h[2], L->dbuf);
+                       L->geo.h[0], L->geo.h[1], L->geo.h[2], L->dbuf, state()->err);
     if ((rc = IM_LAUNCHED("clock_advance"))) return rc;
     if ((rc = source(L, 0.5, s))) return rc;
     for (int sgn = 1; sgn >= -1; sgn -= 2) {
@@ -333,7 +341,7 @@ int imogen_timeloop_run(ImogenTimeloop *T, int firstIteration, int iterations, i
         }
         if (IM_HIP(hipGraphLaunch(g, st->stream))) return ERROR_CUDA_BLEW_UP;
     }
-    return SUCCESSFUL;
+    return take_device_errors(st);
 }
 
 int imogen_timeloop_set_drag(ImogenTimeloop *T, const ThermoDetails *gas, const ThermoDetails *dust, int method)

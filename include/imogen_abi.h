@@ -394,8 +394,10 @@ int imogen_ensemble_set_state(ImogenEnsemble *e, int p, const double *host);
 int imogen_ensemble_get_state(ImogenEnsemble *e, int p, double *host);
 /* Runs iterations firstIteration .. firstIteration+iterations-1 (the
  * iteration number picks the sweep order as stepNumber does); useGraph != 0
- * captures one HIP graph per sweep-order residue and replays it. Returns
- * without waiting for the device. */
+ * captures one HIP graph per sweep-order residue and replays it. Waits for
+ * the last iteration and returns ERROR_CRASH when the kernels met invalid
+ * states (imogen_check_errors); a problem whose CFL limit is not positive and
+ * finite takes no time step in that iteration. */
 int imogen_ensemble_run(ImogenEnsemble *e, int firstIteration, int iterations, int useGraph);
 /* Simulation time reached by every problem (nprob doubles; waits). */
 int imogen_ensemble_times(ImogenEnsemble *e, double *t);
@@ -440,7 +442,10 @@ typedef struct ImogenTimeloop ImogenTimeloop;
 int imogen_timeloop_create(MGArray *fluids, int nfluids, const GeometryParams *geo, const FluidStepParams *params,
                            const double *cflGamma, double cfl, double tmax, ImogenTimeloop **out);
 /* Iterations firstIteration .. firstIteration + iterations - 1 (the number
- * picks the sweep order as stepNumber does); returns without waiting. */
+ * picks the sweep order as stepNumber does). Waits for the last iteration and
+ * returns ERROR_CRASH when the kernels met invalid states (see
+ * imogen_check_errors); an iteration whose CFL limit is not positive and
+ * finite takes no time step (dTime = 0). */
 int imogen_timeloop_run(ImogenTimeloop *loop, int firstIteration, int iterations, int useGraph);
 /* Simulation time and the last dTime (waits). */
 /* Adds the gas-dust drag source of sources/src0001.m to every iteration:
@@ -465,6 +470,17 @@ int imogen_timeloop_destroy(ImogenTimeloop *loop);
 void *imogen_stream(void);
 /* Blocks until all work of this library on the current device is done. */
 int imogen_synchronize(void);
+/* Waits for the library stream and reports the invalid states its kernels
+ * have met since the last report: drag cells with a density, |dv| or (moving
+ * cells) gas internal energy that is not positive and finite, cells whose
+ * adaptive drag integration ran out of steps, NaN CFL limits, and time steps
+ * a device clock refused (limit not positive and finite). Returns
+ * ERROR_CRASH (and clears the record) when any was met. The entry points that
+ * wait anyway -- sourcefunction_2FluidDrag, performSourceFunctions,
+ * computeLocalCFLTimestep, imogen_timeloop_run, imogen_ensemble_run --
+ * return the same status themselves. No counterpart in the reference, whose
+ * kernels let such states run on silently. */
+int imogen_check_errors(void);
 /* Plain device buffers for callers without their own GPU runtime (the learned
  * drag's training data). Copies are ordered on the library stream and return
  * when complete; free waits for the stream first. */

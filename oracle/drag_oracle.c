@@ -25,6 +25,7 @@
  */
 #include "imogen_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 
@@ -120,18 +121,35 @@ static double exact_rhs(const double *P, double rhog, double rhod, double s0sq, 
 
 /* |dv(t)| / |dv(0)| by adaptive Dormand-Prince 5(4) on y = ln|dv|. Returns
  * the ratio; *steps receives the accepted + rejected step count (negative
- * when the step budget ran out). */
+ * when the step budget ran out). A cell the relaxation problem is not defined
+ * on -- a density that is not positive and finite, a non-finite |dv|, or a
+ * moving cell whose gas internal energy is not positive and finite, or whose
+ * drag rate is not finite -- returns NaN at once with *steps =
+ * ORC_DRAG_INVALID. A trial step whose error estimate is not finite (an
+ * overflowing stage of a too-long step) is rejected and shrunk. */
 double orc_drag_exact_ratio(const double *P, double rhog, double rhod, double s0, double u0, double t, double tol,
                             int *steps)
 {
     int n = 0;
     if (steps) *steps = 0;
+    if (!(rhog > 0 && rhog <= DBL_MAX) || !(rhod > 0 && rhod <= DBL_MAX) || !(s0 <= DBL_MAX)) {
+        if (steps) *steps = ORC_DRAG_INVALID;
+        return NAN;
+    }
     if (s0 < 1e-9 || t <= 0) return 1.0;
+    if (!(u0 > 0 && u0 <= DBL_MAX)) {
+        if (steps) *steps = ORC_DRAG_INVALID;
+        return NAN;
+    }
     const double c = .5 * rhod / (rhog + rhod);
     const double s0sq = s0 * s0;
     const double y0 = log(s0);
     double y = y0, tt = 0.0, h = t;
     double k1 = exact_rhs(P, rhog, rhod, s0sq, u0, c, y);
+    if (!(fabs(k1) <= DBL_MAX)) {
+        if (steps) *steps = ORC_DRAG_INVALID;
+        return NAN;
+    }
     while (tt < t) {
         if (n >= 4096) {
             if (steps) *steps = -n;
@@ -155,7 +173,12 @@ double orc_drag_exact_ratio(const double *P, double rhog, double rhod, double s0
                           (17253.0 / 339200) * k5 + (22.0 / 525) * k6 - (1.0 / 40) * k7);
         err = fabs(err) / tol;
         n++;
+        if (!(err <= DBL_MAX)) err = 1e300; /* reject, shrink by the minimum factor */
         if (err <= 1.0) {
+            if (!(fabs(k7) <= DBL_MAX)) {
+                if (steps) *steps = ORC_DRAG_INVALID;
+                return NAN;
+            }
             tt += h;
             y = yn;
             k1 = k7;
@@ -316,9 +339,9 @@ int orc_drag(double *gas, double *dust, const orc_grid *g, const orc_thermo *tg,
     orc_drag_params(tg, td, P);
     const long n = (long)g->nx * g->ny * g->nz;
     const long pitch = g->pitch;
-    long total = 0;
+    long total = 0, bad = 0;
     long i;
-#pragma omp parallel for schedule(static) reduction(+ : total)
+#pragma omp parallel for schedule(static) reduction(+ : total, bad)
     for (i = 0; i < n; i++) {
         double *gc = gas + i, *dc = dust + i;
         double s0, u0;
@@ -336,7 +359,9 @@ int orc_drag(double *gas, double *dust, const orc_grid *g, const orc_thermo *tg,
         if (method == ORC_DRAG_EXACT) {
             int steps;
             f = orc_drag_exact_ratio(P, gc[0], dc[0], s0, u0, dt, tol, &steps);
-            total += steps < 0 ? -steps : steps;
+            if (steps == ORC_DRAG_INVALID) bad++;
+            else if (steps < 0) bad++, total += -steps;
+            else total += steps;
         } else {
             double k = logtrap_rate(P, gc[0], dc[0], s0, u0, dt, method == ORC_DRAG_LOGTRAP3 ? 3 : 2);
             f = exp(-dt * k);
@@ -344,5 +369,5 @@ int orc_drag(double *gas, double *dust, const orc_grid *g, const orc_thermo *tg,
         apply_factor(gc, dc, pitch, f);
     }
     if (work) *work = total;
-    return 0;
+    return bad ? -101 : 0; /* ERROR_CRASH: invalid cells or an exhausted step budget */
 }

@@ -115,14 +115,17 @@ double orc_drag_k(const double *P9, double rhog, double rhod, double magdv, doub
 
 /* |dv(t)|/|dv(0)| of the relaxation problem, adaptive Dormand-Prince 5(4) on
  * ln|dv| to absolute tolerance tol. steps: step count (negative if the budget
- * of 4096 steps ran out). */
+ * of 4096 steps ran out; ORC_DRAG_INVALID, with a NaN result returned at
+ * once, for a cell the problem is not defined on). */
+#define ORC_DRAG_INVALID (-(1 << 30))
 double orc_drag_rk_magdv(const double *P, double rhog, double rhod, double s0, double u0, double dt, int method);
 double orc_drag_exact_ratio(const double *P9, double rhog, double rhod, double s0, double u0, double t, double tol,
                             int *steps);
 
 /* One drag source call over the grid: updates gas E, p and dust p in place.
- * Returns 0 or a negative status; *work (optional) receives the exact
- * integrator's total step count. */
+ * Returns 0 or a negative status (-101 when the exact integrator met invalid
+ * cells or ran out of steps; every cell is still updated); *work (optional)
+ * receives the exact integrator's total step count. */
 int orc_drag(double *gas, double *dust, const orc_grid *g, const orc_thermo *tgas, const orc_thermo *tdust,
              double dt, int method, double tol, long *work);
 

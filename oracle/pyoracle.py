@@ -220,6 +220,13 @@ def drag_exact_ratio(P, rhog, rhod, s0, u0, t, tol=1e-12):
     return r, steps.value
 
 
+DRAG_INVALID = -(1 << 30)  # drag_exact_ratio's step count for a cell it cannot integrate
+
+
+class DragStateError(RuntimeError):
+    """The exact integrator met cells it cannot integrate (every cell is still updated)."""
+
+
 def drag(gas, dust, tgas, tdust, dt, method=DRAG_LOGTRAP2, tol=1e-12):
     """One drag source call (in place on both 5-slab states)."""
     g = _grid(gas)
@@ -228,6 +235,8 @@ def drag(gas, dust, tgas, tdust, dt, method=DRAG_LOGTRAP2, tol=1e-12):
     work = ctypes.c_long(0)
     rc = lib().orc_drag(_dptr(gas), _dptr(dust), ctypes.byref(g), ctypes.byref(tgas), ctypes.byref(tdust), dt,
                         method, tol, ctypes.byref(work))
+    if rc == -101:
+        raise DragStateError("orc_drag: invalid cells or an exhausted step budget")
     if rc:
         raise RuntimeError(f"orc_drag returned {rc}")
     return work.value

@@ -365,3 +365,90 @@ def test_trained_model_cuts_fallbacks_and_stays_accurate():
         print(f"{name} network: {fb} of {ncell} cells fell back; max |ln r error| {err.max():.2e}", flush=True)
         assert err.max() <= 1e-6
     assert counts["trained"] < 0.5 * counts["zero"], counts
+
+
+# ---------------------------------------------------------------- invalid states
+def _poisoned_pair(seed=40):
+    """4 x 6 x 50 gas/dust pair with three cells the drag is not defined on:
+    a NaN gas cell, a moving cell with zero gas internal energy, and a cell
+    with negative dust density."""
+    sh = (4, 6, 50)
+    g, d = random_pair(sh, seed=seed)
+    bad_g, bad_d = g.copy(), d.copy()
+    bad_g[:, 1, 2, 3] = np.nan
+    bad_g[1, 2, 4, 10] = 0.5 * (bad_g[2:, 2, 4, 10] ** 2).sum() / bad_g[0, 2, 4, 10]
+    bad_d[0, 3, 5, 49] = -0.5
+    mask = np.ones(sh, bool)
+    mask[1, 2, 3] = mask[2, 4, 10] = mask[3, 5, 49] = False
+    return sh, g, d, bad_g, bad_d, mask
+
+
+@pytest.mark.parametrize("method", [4, 6, 7])
+def test_invalid_cells_fail_promptly_and_leave_the_rest_exact(method):
+    """Regression: the adaptive integrator used to spin to its 4096-step
+    budget on NaN cells (a NaN error estimate grew the step forever) and the
+    overrun was dropped. Now every drag path counts such cells into the device
+    error word, sourcefunction_2FluidDrag returns ERROR_CRASH at once, and the
+    valid cells get exactly what they get without the invalid ones."""
+    import time
+    a = abi()
+    sh, g, d, bad_g, bad_d, mask = _poisoned_pair()
+    if method == 7:
+        L = ld()
+        params, norm, _ = L.load_model()
+        L.set_model(params, norm)
+    print(f"invalid drag cells, method {method}, shape {sh}", flush=True)
+    ok_g, ok_d = run_gpu_drag(g, d, 0.02, method, sh)
+    tg, td = thermo_pair()
+    fg = gc().FluidManager(bad_g, tg.gamma, thermoDetails=tg)
+    fd = gc().FluidManager(bad_d, td.gamma, thermoDetails=td)
+    geo = gc().GeometryManager((sh[2], sh[1], sh[0]), (0.1, 0.1, 0.1))
+    t0 = time.perf_counter()
+    with pytest.raises(a.ImogenError) as e:
+        gc().cudaSource2FluidDrag([fg, fd], geo, [0.02, method])
+    took = time.perf_counter() - t0
+    assert e.value.code == a.ERROR_CRASH
+    assert took < 2.0, took
+    got_g, got_d = fg.state, fd.state
+    fg.free()
+    fd.free()
+    assert np.array_equal(got_g[:, mask], ok_g[:, mask]) and np.array_equal(got_d[:, mask], ok_d[:, mask])
+    assert a.lib().imogen_check_errors() == 0  # the report cleared the record
+    # the oracle flags the same state (exact integrator)
+    if method == 6:
+        og, od = gas_dust_thermo()
+        rg, rd = bad_g.copy(), bad_d.copy()
+        with pytest.raises(po.DragStateError):
+            po.drag(rg, rd, og, od, 0.02, 6)
+
+
+def test_device_loop_reports_invalid_state_and_stops_the_clock():
+    """A gas-dust device loop on a state with invalid cells returns
+    ERROR_CRASH from imogen_timeloop_run; an all-NaN fluid has no finite CFL
+    limit, so its clock takes no time step."""
+    from gpuimogen_amd import imogen as im
+    a = abi()
+    sh, g, d, bad_g, bad_d, _ = _poisoned_pair(41)
+    tg, td = thermo_pair()
+    fg = gc().FluidManager(bad_g, tg.gamma, thermoDetails=tg, bc_modes=(a.circular,) * 6)
+    fd = gc().FluidManager(bad_d, td.gamma, thermoDetails=td, bc_modes=(a.circular,) * 6)
+    geo = gc().GeometryManager((sh[2], sh[1], sh[0]), (0.1, 0.1, 0.1))
+    loop = im.DeviceLoop([fg, fd], [a.METHOD_HLLC, a.METHOD_HLLC], geo, drag_method=a.DRAG_LOGTRAP2)
+    print("device loop on an invalid gas-dust state", flush=True)
+    with pytest.raises(a.ImogenError) as e:
+        loop.run(3, graph=False)
+    assert e.value.code == a.ERROR_CRASH
+    loop.free()
+    fg.free()
+    fd.free()
+    nan = np.full((5,) + sh, np.nan)
+    f = gc().FluidManager(nan, 5 / 3, bc_modes=(a.circular,) * 6)
+    loop = im.DeviceLoop([f], [a.METHOD_HLLC], geo)
+    print("device loop on an all-NaN state", flush=True)
+    with pytest.raises(a.ImogenError) as e:
+        loop.run(2, graph=True)
+    assert e.value.code == a.ERROR_CRASH
+    t, dt = loop.clock()
+    assert t == 0.0 and dt == 0.0
+    loop.free()
+    f.free()

@@ -285,3 +285,64 @@ def test_etdrk1_decay_and_local_order():
         errs.append(abs(np.linalg.norm(rel_dv(g, d)) / s0 - exact))
     orders = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
     assert np.all(orders > 1.7) and np.all(orders < 2.4), (errs, orders)
+
+
+# ---------------------------------------------------------------- invalid states
+@pytest.mark.parametrize("cell", [
+    dict(rhog=float("nan"), rhod=0.1, s0=0.2, u0=1.0),
+    dict(rhog=1.0, rhod=0.1, s0=float("nan"), u0=1.0),
+    dict(rhog=1.0, rhod=0.1, s0=float("inf"), u0=1.0),
+    dict(rhog=1.0, rhod=0.1, s0=0.2, u0=float("nan")),
+    dict(rhog=1.0, rhod=0.1, s0=0.2, u0=-0.5),
+    dict(rhog=1.0, rhod=0.1, s0=0.2, u0=0.0),
+    dict(rhog=-1.0, rhod=0.1, s0=0.2, u0=1.0),
+    dict(rhog=1.0, rhod=0.0, s0=0.2, u0=1.0),
+])
+def test_exact_integrator_returns_at_once_on_invalid_cells(cell):
+    """A cell the relaxation problem is not defined on returns NaN before any
+    step, flagged invalid -- not a spin to the step budget (a NaN error
+    estimate used to fail `err <= 1` and grow the step forever)."""
+    tg, td = gas_dust_thermo()
+    P9 = po.drag_params(tg, td)
+    r, steps = po.drag_exact_ratio(P9, cell["rhog"], cell["rhod"], cell["s0"], cell["u0"], 1.0)
+    assert np.isnan(r)
+    assert steps == po.DRAG_INVALID
+
+
+def test_exact_integrator_budget_overrun_is_reported():
+    """A tolerance no step size can meet exhausts the 4096-step budget and
+    says so with a negative count."""
+    tg, td = gas_dust_thermo()
+    P9 = po.drag_params(tg, td)
+    u0 = 1.0 / ((tg.gamma - 1) * 1.0)
+    r, steps = po.drag_exact_ratio(P9, 1.0, 0.5, 0.3, u0, 1.0, tol=1e-40)
+    assert steps == -4096
+    assert np.isfinite(r)
+
+
+def test_drag_call_reports_invalid_cells_and_still_updates_the_rest():
+    tg, td = gas_dust_thermo()
+    rng = np.random.default_rng(41)
+    sh = (4, 6, 50)
+    g = np.zeros((5,) + sh)
+    d = np.zeros((5,) + sh)
+    g[0] = rng.uniform(0.5, 2, sh)
+    d[0] = rng.uniform(0.05, 0.5, sh)
+    for c in range(2, 5):
+        g[c] = g[0] * rng.normal(0, 0.3, sh)
+        d[c] = d[0] * rng.normal(0, 0.3, sh)
+    g[1] = 1.0 / (tg.gamma - 1) + 0.5 * (g[2] ** 2 + g[3] ** 2 + g[4] ** 2) / g[0]
+    d[1] = 0.5 * (d[2] ** 2 + d[3] ** 2 + d[4] ** 2) / d[0]
+    bad_g, bad_d = g.copy(), d.copy()
+    bad_g[:, 1, 2, 3] = np.nan                 # a NaN cell
+    bad_g[1, 2, 4, 10] = 0.0                   # negative internal energy
+    bad_d[0, 3, 5, 49] = -1.0                  # negative dust density
+    ok_g, ok_d = g.copy(), d.copy()
+    po.drag(ok_g, ok_d, tg, td, 0.05, method=po.DRAG_EXACT)
+    with pytest.raises(po.DragStateError):
+        po.drag(bad_g, bad_d, tg, td, 0.05, method=po.DRAG_EXACT)
+    mask = np.ones(sh, bool)
+    mask[1, 2, 3] = mask[2, 4, 10] = mask[3, 5, 49] = False
+    assert np.array_equal(bad_g[:, mask], ok_g[:, mask])
+    assert np.array_equal(bad_d[:, mask], ok_d[:, mask])
+    assert np.isnan(bad_g[2, 2, 4, 10]) and np.isnan(bad_d[2, 3, 5, 49])
