@@ -142,6 +142,16 @@ def run_gasdust(args):
         gc.synchronize()
         abi.call("imogen_profile_enable", 1)
         L.learned_stats()
+        prof = {"launches": 0, "total_ms": 0.0}
+
+        def take_drag_events():
+            # the event pool is read and reset every chunk, so it never runs out
+            cnt, tot = ctypes.c_int64(0), ctypes.c_double(0.0)
+            abi.call("imogen_profile_read", 5, ctypes.byref(cnt), ctypes.byref(tot))
+            prof["launches"] += int(cnt.value)
+            prof["total_ms"] += tot.value
+            abi.call("imogen_profile_enable", 1)
+
         t0 = time.perf_counter()
         last = t0
         done = 0
@@ -150,21 +160,20 @@ def run_gasdust(args):
             k = min(chunk, args.gasdust_steps - done)
             loop.run(k, graph=False)
             done += k
-            gc.synchronize()
+            take_drag_events()  # synchronises
             now = time.perf_counter()
             if done % max(1, args.gasdust_steps // 4) == 0 or now - last > 20 or done == args.gasdust_steps:
                 log(f"[bench] gas-dust {name}: step {done}/{args.gasdust_steps} ({now - t0:.1f} s)")
                 last = now
         el = time.perf_counter() - t0
-        prof = {}
-        for kid, kname in ((5, "drag"), (6, "learned_drag")):
-            cnt, tot = ctypes.c_int64(0), ctypes.c_double(0.0)
-            abi.call("imogen_profile_read", kid, ctypes.byref(cnt), ctypes.byref(tot))
-            prof[kname] = {"launches": int(cnt.value), "total_ms": tot.value}
         abi.call("imogen_profile_enable", 0)
         cells, fb = L.learned_stats()
+        # six drag calls per iteration: src0001's two half-interval calls in
+        # each of the three sources, each source's pair applied by one launch
+        drag_it = prof["total_ms"] / args.gasdust_steps
         out[name] = {"seconds": el, "ms_per_step": el / args.gasdust_steps * 1e3,
-                     "drag_ms_per_call": prof["drag"]["total_ms"] / max(1, prof["drag"]["launches"]),
+                     "drag_ms_per_step": drag_it, "drag_ms_per_call": drag_it / 6,
+                     "drag_launches_per_step": prof["launches"] / args.gasdust_steps,
                      "sim_time": loop.time, "fallback_share": (fb / cells) if cells else None,
                      "dust_mom": fd.state[2:5].copy()}
         loop.free()

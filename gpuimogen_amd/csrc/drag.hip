@@ -27,49 +27,83 @@ namespace imogen {
 // Cells the method cannot be applied to (drag_cell_defined) are updated as
 // the arithmetic falls (NaN, as the reference's kernels do) and counted into
 // err[ERRW_DRAG_INVALID]; the adaptive integrator's budget overruns into
-// err[ERRW_DRAG_BUDGET].
+// err[ERRW_DRAG_BUDGET]. `calls` (1 or 2) drag calls of dt each are applied to
+// a cell in registers before it is stored: sources/src0001.m's two
+// back-to-back half-interval calls in one pass over memory, with the same
+// arithmetic as two launches (the second call reads what the first stored).
 template <int METHOD>
 __global__ __launch_bounds__(256) void drag_kernel(double *__restrict__ g, double *__restrict__ d, int64_t pitch,
-                                                   int64_t n, DragConsts c, DragDt dtv,
+                                                   int64_t n, DragConsts c, DragDt dtv, int calls,
                                                    unsigned long long *__restrict__ err)
 {
     const double dt = drag_dt(dtv);
     unsigned invalid = 0, overrun = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        DragCell s = drag_load(g + i, d + i, pitch);
-        if (METHOD == DRAG_ETDRK1) {
-            invalid += !drag_cell_defined(s.rg, s.rd, s.s0, s.u0);
-            drag_etd1_apply(g + i, d + i, pitch, dt, drag_k(c, s.rg, s.rd, s.s0, s.u0));
-            continue;
+        // the nine values the method reads and writes, as a pitch-1 cell
+        double G[5], D[5];
+#pragma unroll
+        for (int k = 0; k < 5; k++) G[k] = g[i + k * pitch];
+        D[0] = d[i];
+#pragma unroll
+        for (int k = 2; k < 5; k++) D[k] = d[i + k * pitch];
+        D[1] = 0.0;  // the dust energy is neither read nor written
+        for (int r = 0; r < calls; r++) {
+            DragCell s = drag_load(G, D, 1);
+            if (METHOD == DRAG_ETDRK1) {
+                invalid += !drag_cell_defined(s.rg, s.rd, s.s0, s.u0);
+                drag_etd1_apply(G, D, 1, dt, drag_k(c, s.rg, s.rd, s.s0, s.u0));
+                continue;
+            }
+            double f;
+            if (METHOD == DRAG_EXACT) {
+                int k = 0;
+                f = drag_exact_ratio(c, s.rg, s.rd, s.s0, s.u0, dt, &k);
+                invalid += (k == kDragInvalid);
+                overrun += (k < 0 && k != kDragInvalid);
+            } else {
+                invalid += !drag_cell_defined(s.rg, s.rd, s.s0, s.u0);
+                f = exp(-dt * drag_logtrap_rate(c, s.rg, s.rd, s.s0, s.u0, dt, METHOD == DRAG_LOGTRAP3 ? 3 : 2));
+            }
+            drag_apply(G, D, 1, s, f);
         }
-        double f;
-        if (METHOD == DRAG_EXACT) {
-            int k = 0;
-            f = drag_exact_ratio(c, s.rg, s.rd, s.s0, s.u0, dt, &k);
-            invalid += (k == kDragInvalid);
-            overrun += (k < 0 && k != kDragInvalid);
-        } else {
-            invalid += !drag_cell_defined(s.rg, s.rd, s.s0, s.u0);
-            f = exp(-dt * drag_logtrap_rate(c, s.rg, s.rd, s.s0, s.u0, dt, METHOD == DRAG_LOGTRAP3 ? 3 : 2));
+        g[i + pitch] = G[1];
+#pragma unroll
+        for (int k = 2; k < 5; k++) {
+            g[i + k * pitch] = G[k];
+            d[i + k * pitch] = D[k];
         }
-        drag_apply(g + i, d + i, pitch, s, f);
     }
     wave_count(err + ERRW_DRAG_INVALID, invalid);
     if (METHOD == DRAG_EXACT) wave_count(err + ERRW_DRAG_BUDGET, overrun);
 }
 
-// explicit midpoint (method 0) / RK4 (method 1): 72 bytes read, 64 written per cell
+// explicit midpoint (method 0) / RK4 (method 1): 72 bytes read, 64 written per
+// cell (the dust energy is reset); `calls` as drag_kernel
 template <int METHOD>
 __global__ __launch_bounds__(256) void drag_rk_kernel(double *__restrict__ g, double *__restrict__ d, int64_t pitch,
-                                                      int64_t n, DragConsts c, DragDt dtv,
+                                                      int64_t n, DragConsts c, DragDt dtv, int calls,
                                                       unsigned long long *__restrict__ err)
 {
     const double dt = drag_dt(dtv);
     unsigned invalid = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const DragCell s = drag_rk_load(g + i, d + i, pitch);
-        invalid += !drag_cell_defined(s.rg, s.rd, s.s0, s.u0);
-        drag_rk_apply(g + i, d + i, pitch, drag_rk_magdv<METHOD>(c, s.rg, s.rd, s.s0, s.u0, dt));
+        double G[5], D[5];
+#pragma unroll
+        for (int k = 0; k < 5; k++) G[k] = g[i + k * pitch];
+        D[0] = d[i];
+        D[1] = 0.0;
+#pragma unroll
+        for (int k = 2; k < 5; k++) D[k] = d[i + k * pitch];
+        for (int r = 0; r < calls; r++) {
+            const DragCell s = drag_rk_load(G, D, 1);
+            invalid += !drag_cell_defined(s.rg, s.rd, s.s0, s.u0);
+            drag_rk_apply(G, D, 1, drag_rk_magdv<METHOD>(c, s.rg, s.rd, s.s0, s.u0, dt));
+        }
+#pragma unroll
+        for (int k = 1; k < 5; k++) {
+            g[i + k * pitch] = G[k];
+            d[i + k * pitch] = D[k];
+        }
     }
     wave_count(err + ERRW_DRAG_INVALID, invalid);
 }
@@ -93,8 +127,9 @@ DragConsts drag_consts(const ThermoDetails *tg, const ThermoDetails *td)
 }
 
 int launch_drag(double *g, double *d, int64_t pitch, int64_t n, const DragConsts &c, DragDt dt, int method,
-                hipStream_t s)
+                hipStream_t s, int calls)
 {
+    if (calls < 1 || calls > 2) return IM_FAIL(ERROR_INVALID_ARGS, "drag calls per pass: %d", calls);
     DeviceState *st = state();
     if (!st) return ERROR_CUDA_BLEW_UP;
     unsigned long long *err = st->err;
@@ -104,25 +139,25 @@ int launch_drag(double *g, double *d, int64_t pitch, int64_t n, const DragConsts
     switch (method) {
     case DRAG_LOGTRAP2:
         hipLaunchKernelGGL((drag_kernel<DRAG_LOGTRAP2>), dim3((unsigned)blocks), dim3(256), 0, s, g, d, pitch, n, c,
-                           dt, err);
+                           dt, calls, err);
         break;
     case DRAG_LOGTRAP3:
         hipLaunchKernelGGL((drag_kernel<DRAG_LOGTRAP3>), dim3((unsigned)blocks), dim3(256), 0, s, g, d, pitch, n, c,
-                           dt, err);
+                           dt, calls, err);
         break;
     case DRAG_EXACT:
         hipLaunchKernelGGL((drag_kernel<DRAG_EXACT>), dim3((unsigned)blocks), dim3(256), 0, s, g, d, pitch, n, c,
-                           dt, err);
+                           dt, calls, err);
         break;
     case DRAG_ETDRK1:
         hipLaunchKernelGGL((drag_kernel<DRAG_ETDRK1>), dim3((unsigned)blocks), dim3(256), 0, s, g, d, pitch, n, c,
-                           dt, err);
+                           dt, calls, err);
         break;
     case DRAG_EMP:
-        hipLaunchKernelGGL((drag_rk_kernel<DRAG_EMP>), dim3((unsigned)blocks), dim3(256), 0, s, g, d, pitch, n, c, dt, err);
+        hipLaunchKernelGGL((drag_rk_kernel<DRAG_EMP>), dim3((unsigned)blocks), dim3(256), 0, s, g, d, pitch, n, c, dt, calls, err);
         break;
     case DRAG_RK4:
-        hipLaunchKernelGGL((drag_rk_kernel<DRAG_RK4>), dim3((unsigned)blocks), dim3(256), 0, s, g, d, pitch, n, c, dt, err);
+        hipLaunchKernelGGL((drag_rk_kernel<DRAG_RK4>), dim3((unsigned)blocks), dim3(256), 0, s, g, d, pitch, n, c, dt, calls, err);
         break;
     default: return IM_FAIL(ERROR_INVALID_ARGS, "drag method %d", method);
     }

@@ -339,12 +339,14 @@ inline DragDt host_dt(double dt)
 
 DragConsts drag_consts(const ThermoDetails *tg, const ThermoDetails *td);
 // Every drag kernel counts the cells the method cannot be applied to (and the
-// adaptive integrator's budget overruns) into the device error word.
+// adaptive integrator's budget overruns) into the device error word. calls:
+// 1, or 2 for two back-to-back drag calls of dt each applied in one pass.
 int launch_drag(double *g, double *d, int64_t pitch, int64_t n, const DragConsts &c, DragDt dt, int method,
-                hipStream_t s);
-// queue: one int64 per cell, or null for the library's scratch block
+                hipStream_t s, int calls = 1);
+// queue: one int64 per cell, or null for the library's scratch block; calls
+// as launch_drag (two calls: the same result as two launches)
 int launch_learned_drag(double *g, double *d, int64_t pitch, int64_t n, const DragConsts &c, DragDt dt,
-                        int64_t *queue, hipStream_t s);
+                        int64_t *queue, hipStream_t s, int calls = 1);
 
 // Drag over many gas/dust array pairs (the dusty problems of an ensemble):
 // table entry t = (gas, dust, slab pitch, cells, problem index for dtp), items
@@ -362,7 +364,7 @@ struct EnsDragItem {
 };
 int launch_ens_drag(const EnsDragTable *tab, const EnsDragItem *items, int nitems, int64_t ncells,
                     const double *dtp, double frac, const DragConsts &c, int method, int64_t *queue,
-                    unsigned long long *qcount, hipStream_t s);
+                    unsigned long long *qcount, hipStream_t s, int calls = 1);
 // changes whenever imogen_drag_set_model loads a model (captured graphs hold it by value)
 uint64_t learned_model_generation();
 // Allocates the learned drag's device counters (cells, fallbacks, queue

@@ -395,9 +395,12 @@ static int ens_bc(EnsembleImpl *e, int dir, hipStream_t s, int xrules)
 static int ens_source(EnsembleImpl *e, double fraction, hipStream_t s)
 {
     if (!e->Pd) return SUCCESSFUL;
-    for (int call = 0; call < 2; call++) {
+    // both half-interval calls in one pass over memory (two launches for the
+    // learned method, as the single-grid device loop)
+    const bool learned = e->dragMethod == DRAG_LEARNED;
+    for (int call = 0; call < (learned ? 2 : 1); call++) {
         int rc = launch_ens_drag(e->dragTab, e->dragItems, e->nDragItems, e->dragCells, e->dtp, fraction / 2, e->dc,
-                                 e->dragMethod, e->queue, e->qcount, s);
+                                 e->dragMethod, e->queue, e->qcount, s, learned ? 1 : 2);
         if (rc) return rc;
     }
     for (int d = 1; d <= 2; d++) {

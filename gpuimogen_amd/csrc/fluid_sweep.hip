@@ -58,31 +58,68 @@ static int march_segments(int64_t columns, int N)
     return best;
 }
 
-template <class M, int LIM>
-static int launch_x(double *q, int64_t pitch, int N, int64_t nlines, double lambda, const double *lamp,
-                    const HydroConsts &hc, int rt, const CflFuse &cf, hipStream_t s)
+// Launch shape of an X sweep over nlines lines of N cells: K cells per lane,
+// nl lanes per line, 256-thread blocks. K = 0 when N is above 1024.
+struct XGeom {
+    int K = 0, nl = 0;
+    int64_t blocks = 0;
+};
+
+static XGeom x_geom(int N, int64_t nlines)
 {
-    int K;
-    if (N <= 64) K = 1;
-    else if (N <= 128) K = 2;
-    else if (N <= 192) K = 3;
-    else if (N <= 256) K = 4;
-    else if (N <= 384) K = 6;
-    else if (N <= 512) K = 8;
-    else if (N <= 768) K = 12;
-    else if (N <= 1024) K = 16;
-    else return IM_FAIL(ERROR_NOIMPLEMENT, "X sweep supports lines of at most 1024 cells (got %d)", N);
-    const int nl = (N + K - 1) / K;
-    const int lpw = 64 / nl;
+    XGeom g;
+    if (N <= 64) g.K = 1;
+    else if (N <= 128) g.K = 2;
+    else if (N <= 192) g.K = 3;
+    else if (N <= 256) g.K = 4;
+    else if (N <= 384) g.K = 6;
+    else if (N <= 512) g.K = 8;
+    else if (N <= 768) g.K = 12;
+    else if (N <= 1024) g.K = 16;
+    else return g;
+    g.nl = (N + g.K - 1) / g.K;
+    const int lpw = 64 / g.nl;
     int64_t waves = (nlines + lpw - 1) / lpw;
     // Resident-wave budget: 2 waves per SIMD on 1024 SIMDs; beyond it a wave
     // takes several line groups and prefetches the next one.
     if (waves > 2048) waves = 2048;
-    const int64_t blocks = (waves + 3) / 4;
+    g.blocks = (waves + 3) / 4;
+    return g;
+}
+
+// Launch shape of a Y (dir 2) or Z (dir 3) march: S segments per column, a
+// grid of blocksPerRow x nother blocks of 256 threads (nthr in all).
+struct MarchGeom {
+    int N = 0, nother = 0, S = 1, seglen = 0, wavesPerRow = 0, blocksPerRow = 0;
+    int64_t nthr = 0;
+};
+
+static MarchGeom march_geom(int nx, int ny, int nz, int dir)
+{
+    MarchGeom g;
+    g.N = (dir == 2) ? ny : nz;
+    g.nother = (dir == 2) ? nz : ny;
+    g.S = march_segments((int64_t)nx * g.nother, g.N);
+    g.seglen = (g.N + g.S - 1) / g.S;
+    const int W = 64 / g.S;
+    g.wavesPerRow = (nx + W - 1) / W;
+    g.blocksPerRow = (g.wavesPerRow + 3) / 4;
+    g.nthr = (int64_t)g.blocksPerRow * g.nother * 256;
+    return g;
+}
+
+template <class M, int LIM>
+static int launch_x(double *q, int64_t pitch, int N, int64_t nlines, double lambda, const double *lamp,
+                    const HydroConsts &hc, int rt, const CflFuse &cf, hipStream_t s)
+{
+    const XGeom g = x_geom(N, nlines);
+    if (g.K == 0) return IM_FAIL(ERROR_NOIMPLEMENT, "X sweep supports lines of at most 1024 cells (got %d)", N);
+    const int K = g.K, nl = g.nl;
+    const int64_t blocks = g.blocks;
     if (cf.part && blocks * 4 > cf.cap) return IM_FAIL(ERROR_INVALID_ARGS, "CFL partial buffer too small");
     dim3 grid((unsigned)blocks), block(256);
     const bool full = (nl * K == N) && (64 % nl == 0);
-    double *dummy = (double *)scratch((size_t)blocks * 256 * 5 * sizeof(double));
+    double *dummy = (double *)scratch_block((size_t)blocks * 256 * 5 * sizeof(double));
     if (!dummy) return IM_FAIL(ERROR_NOMEM, "no scratch for the X sweep");
 #define IM_XK(KK)                                                                                       \
     case KK:                                                                                            \
@@ -119,21 +156,15 @@ template <class M, int LIM>
 static int launch_march(double *q, int64_t pitch, int nx, int ny, int nz, int dir, double lambda,
                         const double *lamp, const HydroConsts &hc, int rt, const CflFuse &cf, hipStream_t s)
 {
-    const int N = (dir == 2) ? ny : nz;
     const int64_t sstride = (dir == 2) ? (int64_t)nx : (int64_t)nx * ny;
     const int64_t ostride = (dir == 2) ? (int64_t)nx * ny : (int64_t)nx;
-    const int nother = (dir == 2) ? nz : ny;
-    const int64_t columns = (int64_t)nx * nother;
-    const int S = march_segments(columns, N);
-    const int seglen = (N + S - 1) / S;
-    const int W = 64 / S;
-    const int wavesPerRow = (nx + W - 1) / W;
-    const int blocksPerRow = (wavesPerRow + 3) / 4;
-    dim3 grid((unsigned)blocksPerRow, (unsigned)nother), block(256);
-    const int64_t nthr = (int64_t)blocksPerRow * nother * 256;
+    const MarchGeom g = march_geom(nx, ny, nz, dir);
+    const int N = g.N, S = g.S, seglen = g.seglen, wavesPerRow = g.wavesPerRow;
+    dim3 grid((unsigned)g.blocksPerRow, (unsigned)g.nother), block(256);
+    const int64_t nthr = g.nthr;
     if (cf.part && nthr / 64 > cf.cap) return IM_FAIL(ERROR_INVALID_ARGS, "CFL partial buffer too small");
     // 4 deferred output slots + 1 dummy slot of 5 doubles per lane
-    double *defer = (double *)scratch((size_t)nthr * 25 * sizeof(double));
+    double *defer = (double *)scratch_block((size_t)nthr * 25 * sizeof(double));
     if (!defer) return IM_FAIL(ERROR_NOMEM, "no scratch for deferred outputs");
 #define IM_MARCH(D, C)                                                                                  \
     hipLaunchKernelGGL((sweep_march<M, D, LIM, C>), grid, block, 0, s, q, pitch, nx, sstride, ostride, N, S, \
@@ -180,29 +211,9 @@ static int launch_method(double *base, int64_t pitch, int nx, int ny, int nz, in
 // The grid arithmetic of launch_x / launch_march, for sizing fused-CFL buffers.
 int64_t sweep_waves(int nx, int ny, int nz, int dir)
 {
-    if (dir == 1) {
-        int K = 16;
-        if (nx <= 64) K = 1;
-        else if (nx <= 128) K = 2;
-        else if (nx <= 192) K = 3;
-        else if (nx <= 256) K = 4;
-        else if (nx <= 384) K = 6;
-        else if (nx <= 512) K = 8;
-        else if (nx <= 768) K = 12;
-        const int nl = (nx + K - 1) / K;
-        const int lpw = 64 / nl;
-        int64_t waves = ((int64_t)ny * nz + lpw - 1) / lpw;
-        if (waves > 2048) waves = 2048;
-        return (waves + 3) / 4 * 4;
-    }
-    const int N = (dir == 2) ? ny : nz;
-    const int nother = (dir == 2) ? nz : ny;
-    const int64_t columns = (int64_t)nx * nother;
-    const int S = march_segments(columns, N);
-    const int W = 64 / S;
-    const int wavesPerRow = (nx + W - 1) / W;
-    const int blocksPerRow = (wavesPerRow + 3) / 4;
-    return (int64_t)blocksPerRow * nother * 4;
+    if (dir == 1) return x_geom(nx, (int64_t)ny * nz).blocks * 4;
+    const MarchGeom g = march_geom(nx, ny, nz, dir);
+    return (int64_t)g.blocksPerRow * g.nother * 4;
 }
 
 int launch_fluid_sweep(double *base, int64_t pitch, int nx, int ny, int nz, int dir, int method, double lambda,

@@ -132,11 +132,18 @@ int fluid_boundary(MGArray *fluid, int direction, int xrules)
     if (direction < 1 || direction > 3)
         return IM_FAIL(ERROR_INVALID_ARGS, "direction %d is not one of 1 (X) 2 (Y) 3 (Z)", direction);
     int modes10[10];
+    bool out[2] = {false, false};
     for (int side = 0; side < 2; side++) {
+        // the density array's mode decides (cudaStatics.cu:140-152)
         const BCModeTypes bt = fluid[0].boundaryConditions.mode[2 * (direction - 1) + side];
-        if (bt == outflow || bt == freebalance)
-            return IM_FAIL(ERROR_NOIMPLEMENT, "outflow/freebalance boundaries are not built");
+        if (bt == freebalance)
+            return IM_FAIL(ERROR_NOIMPLEMENT, "freebalance boundaries are not built (they need the gravity potential)");
+        out[side] = (bt == outflow);
         for (int i = 0; i < 5; i++) {
+            if (out[side]) {
+                modes10[2 * i + side] = -1;
+                continue;
+            }
             rc = sas_code(&fluid[i], direction, side, &modes10[2 * i + side]);
             if (rc) return rc;
         }
@@ -145,11 +152,30 @@ int fluid_boundary(MGArray *fluid, int direction, int xrules)
     // memory axis that holds it (cudaStatics.cu:230)
     const int memdir = is_xyz(fluid) ? direction : MGA_dir2memdir(fluid->currentPermutation, direction);
     if (memdir < 1) return IM_FAIL(ERROR_INVALID_ARGS, "no memory axis holds direction %d", direction);
+    if ((out[0] || out[1]) && !is_xyz(fluid))
+        return IM_FAIL(ERROR_NOIMPLEMENT, "outflow boundaries of rotated arrays are not built");
     DeviceState *st = state();
     if (!st) return ERROR_CUDA_BLEW_UP;
+    const int depth = fluid->haloSize > 0 ? fluid->haloSize : 4;
+    const int *d3 = fluid->dim;
     const int slot = prof_begin(PROF_BC, st->stream);
-    rc = launch_fluid_bc(base, pitch, fluid->dim[0], fluid->dim[1], fluid->dim[2], memdir, modes10,
-                         fluid->haloSize > 0 ? fluid->haloSize : 4, st->stream, xrules);
+    if (!out[0] && !out[1]) {
+        rc = launch_fluid_bc(base, pitch, d3[0], d3[1], d3[2], memdir, modes10, depth, st->stream, xrules);
+    } else if (out[0] && out[1] && d3[direction - 1] >= 2 * depth + 2) {
+        rc = launch_outflow_bc(base, pitch, d3[0], d3[1], d3[2], direction, 0, 2, depth, st->stream);
+    } else {
+        // the minus face, then the plus face (setFluidBoundary's side loop)
+        rc = SUCCESSFUL;
+        for (int side = 0; side < 2 && !rc; side++) {
+            if (out[side]) {
+                rc = launch_outflow_bc(base, pitch, d3[0], d3[1], d3[2], direction, side, 1, depth, st->stream);
+            } else {
+                int one[10];
+                for (int k = 0; k < 10; k++) one[k] = ((k & 1) == side) ? modes10[k] : -1;
+                rc = launch_fluid_bc(base, pitch, d3[0], d3[1], d3[2], memdir, one, depth, st->stream, xrules);
+            }
+        }
+    }
     prof_end(slot, st->stream);
     return rc;
 }
@@ -362,6 +388,8 @@ static int cfl_common(MGArray *fluid, MGArray *csound, GeometryParams *geom, int
                     csound ? 0 : 1, dim, xj, halve, geom->h[0], geom->h[1], geom->h[2], hmin, st->devScalars,
                     st->stream);
     prof_end(slot, st->stream);
+    if (rc) return rc;
+    rc = pinned_buffers(st);
     if (rc) return rc;
     rc = IM_HIP(hipMemcpyAsync(st->pinned, st->devScalars, sizeof(double), hipMemcpyDeviceToHost, st->stream));
     if (rc) return rc;

@@ -32,21 +32,25 @@ enum DeviceErrorSlot {
     ERRW_SLOTS = 8
 };
 
-// Per-device state: one stream all kernels go to, a growable scratch block
-// and a small pinned host buffer for scalar read-back.
+// Per-device state: one stream all kernels go to, a growable scratch block,
+// the device addresses of the library's static device arrays, and small
+// pinned host buffers for scalar read-back.
 struct DeviceState {
     int device = -1;
     hipStream_t stream = nullptr;
     void *scratch = nullptr;
     size_t scratchBytes = 0;
-    double *pinned = nullptr;   // 64 doubles of pinned host memory
+    double *pinned = nullptr;   // 64 doubles of pinned host memory (pinned_buffers)
     double *devScalars = nullptr;  // 4096 doubles of device memory for reductions
     unsigned long long *err = nullptr;      // ERRW_SLOTS device counters, zero when no error is pending
-    unsigned long long *errHost = nullptr;  // their pinned host copy
+    unsigned long long *errHost = nullptr;  // their pinned host copy (pinned_buffers)
 };
 
-// State of the current device (initialised on first use).
+// State of the current device, initialised on first use: the stream and the
+// device addresses of the static device arrays. Allocates no memory.
 DeviceState *state();
+// Allocates the pinned host buffers (pinned, errHost) on first use.
+int pinned_buffers(DeviceState *st);
 // Waits for the stream, reads the device error counters and, when any is
 // set, clears them and returns ERROR_CRASH naming what was counted. Never
 // call it while the stream is being captured.
@@ -62,7 +66,7 @@ void prof_end(int slot, hipStream_t s);
 // Nested suspension of the event timing (kept off while a graph is captured).
 void prof_suspend(int on);
 // Device scratch of at least `bytes`; contents are undefined on return.
-void *scratch(size_t bytes);
+void *scratch_block(size_t bytes);
 
 // Validates a 5-array fluid canister (slab views of one 5-slab owner, XYZ
 // order, one GPU) and returns its base pointer and slab pitch in doubles.
@@ -113,6 +117,10 @@ int launch_xinjin_sweep(double *base, int64_t pitch, int nx, int ny, int nz, int
 // kernels (setFluidBoundary / setArrayBoundaryConditions).
 int launch_fluid_bc(double *base, int64_t pitch, int nx, int ny, int nz, int dir, const int *modes10,
                     int depth, hipStream_t s, int xrules = 0);
+// Outflow ghosts of faces side0 .. side0 + nsides - 1 of direction dir (all 5
+// slabs; both faces together only when they are disjoint, N >= 2 depth + 2).
+int launch_outflow_bc(double *base, int64_t pitch, int nx, int ny, int nz, int dir, int side0, int nsides,
+                      int depth, hipStream_t s);
 int launch_soundspeed(const double *base, int64_t pitch, int64_t n, double gamma, double *cs,
                       hipStream_t s);
 int launch_cfl(const double *base, int64_t pitch, int64_t n, const double *cs, double gamma, int fused,

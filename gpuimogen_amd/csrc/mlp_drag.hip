@@ -26,6 +26,7 @@
 #include "cfl_device.h"
 #include "drag_device.h"
 #include "imogen_internal.h"
+#include "sweep_device.h"
 
 namespace imogen {
 
@@ -89,9 +90,17 @@ __device__ __forceinline__ bf16x4 load_frag(const unsigned short *w, int ld, int
     return r;
 }
 
+// four FP32 values to BF16 (round to nearest even: v_cvt_pk_bf16_f32)
+__device__ __forceinline__ bf16x4 to_bf16x4(const f32x4 &v)
+{
+    return __builtin_bit_cast(bf16x4, __builtin_convertvector(v, bfv4));
+}
+
 // Inference GELU: the same tanh form as x * sigmoid(2 sqrt(2/pi) (x + 0.044715 x^3)),
 // from the hardware 2^x and reciprocal (about 1 ulp of FP32 each); the result
-// is rounded to BF16 right after, 2^-9 relative, far coarser.
+// is rounded to BF16 right after, 2^-9 relative, far coarser. (A linearly
+// interpolated table in LDS took as many VALU issue cycles: more operations,
+// fewer transcendentals.)
 __device__ __forceinline__ float gelu_fast(float x)
 {
     const float c1 = -2.0f * 0.7978845608028654f * 1.4426950408889634f;
@@ -100,23 +109,40 @@ __device__ __forceinline__ float gelu_fast(float x)
     return x * __builtin_amdgcn_rcpf(1.0f + e);
 }
 
-// four FP32 values to BF16 (round to nearest even: v_cvt_pk_bf16_f32)
-__device__ __forceinline__ bf16x4 to_bf16x4(const f32x4 &v)
-{
-    return __builtin_bit_cast(bf16x4, __builtin_convertvector(v, bfv4));
-}
-
 __device__ __forceinline__ bf16x4 act_to_b(const f32x4 &d)
 {
     const f32x4 a = {gelu_fast(d[0]), gelu_fast(d[1]), gelu_fast(d[2]), gelu_fast(d[3])};
     return to_bf16x4(a);
 }
 
+// One block's LDS copy of the model: the BF16 weights (w1t, w2t, w3t back to
+// back, as in the device block) and the biases (b1, b2, b3). Read from LDS,
+// the weight fragments of the 96 MFMAs per 64 cells wait tens of cycles
+// instead of a global-memory round trip.
+struct alignas(16) MlpLds {
+    unsigned short w[64 * 16 + 64 * 64 + 16 * 64];
+    float b[64 + 64 + 16];
+};
+
+// Copies the model into LDS. Every thread of the block must call it (it
+// ends with a barrier).
+__device__ __forceinline__ void stage_model(MlpLds &L, const MlpDev &m)
+{
+    const uint4 *src = reinterpret_cast<const uint4 *>(m.w1t);
+    uint4 *dst = reinterpret_cast<uint4 *>(L.w);
+    constexpr int nw = (int)sizeof(MlpLds::w) / 16;
+    for (int k = threadIdx.x; k < nw; k += blockDim.x) dst[k] = src[k];
+    for (int k = threadIdx.x; k < 144; k += blockDim.x) L.b[k] = m.b1[k];
+    __syncthreads();
+}
+
 // Network outputs for the 64 cells of a wave: every lane passes its own
 // standardised features f[5] and receives its own outputs. Must be called by
-// all 64 lanes together.
-__device__ __forceinline__ void mlp_wave(const MlpDev &m, const float (&f)[5], float (&o)[3])
+// all 64 lanes together. The weights and biases come from LDS.
+__device__ __forceinline__ void mlp_wave(const MlpLds &L, const float (&f)[5], float (&o)[3])
 {
+    const unsigned short *w1t = L.w, *w2t = L.w + 64 * 16, *w3t = L.w + 64 * 16 + 64 * 64;
+    const float *b1 = L.b, *b2 = L.b + 64, *b3 = L.b + 128;
     const int lane = threadIdx.x & 63;
     const int q = lane >> 4;   // which group of 4 K (or M) rows this lane holds
     const int col = lane & 15; // which of the 16 cells of a tile
@@ -136,28 +162,33 @@ __device__ __forceinline__ void mlp_wave(const MlpDev &m, const float (&f)[5], f
 #pragma unroll
         for (int mt = 0; mt < 4; mt++) {
             const int u = 16 * mt + 4 * q;
-            f32x4 acc = {m.b1[u], m.b1[u + 1], m.b1[u + 2], m.b1[u + 3]};
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(load_frag(m.w1t, 16, 16 * mt + col, 4 * q), xb, acc, 0, 0,
+            f32x4 acc = {b1[u], b1[u + 1], b1[u + 2], b1[u + 3]};
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(load_frag(w1t, 16, 16 * mt + col, 4 * q), xb, acc, 0, 0,
                                                            0);
             h1[mt] = act_to_b(acc);
         }
-        // layer 2
-        bf16x4 h2[4];
+        // layer 2: the four output tiles' accumulators advance together, so
+        // consecutive MFMAs never wait on each other's results
+        f32x4 acc2[4];
 #pragma unroll
         for (int mt = 0; mt < 4; mt++) {
             const int u = 16 * mt + 4 * q;
-            f32x4 acc = {m.b2[u], m.b2[u + 1], m.b2[u + 2], m.b2[u + 3]};
-#pragma unroll
-            for (int kt = 0; kt < 4; kt++)
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(load_frag(m.w2t, 64, 16 * mt + col, 16 * kt + 4 * q),
-                                                               h1[kt], acc, 0, 0, 0);
-            h2[mt] = act_to_b(acc);
+            acc2[mt] = f32x4{b2[u], b2[u + 1], b2[u + 2], b2[u + 3]};
         }
-        // layer 3: one tile, outputs 0..2 in rows 0..2 (lanes 0..15)
-        f32x4 acc = {m.b3[4 * q], m.b3[4 * q + 1], m.b3[4 * q + 2], m.b3[4 * q + 3]};
 #pragma unroll
         for (int kt = 0; kt < 4; kt++)
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(load_frag(m.w3t, 64, col, 16 * kt + 4 * q), h2[kt], acc,
+#pragma unroll
+            for (int mt = 0; mt < 4; mt++)
+                acc2[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
+                    load_frag(w2t, 64, 16 * mt + col, 16 * kt + 4 * q), h1[kt], acc2[mt], 0, 0, 0);
+        bf16x4 h2[4];
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) h2[mt] = act_to_b(acc2[mt]);
+        // layer 3: one tile, outputs 0..2 in rows 0..2 (lanes 0..15)
+        f32x4 acc = {b3[4 * q], b3[4 * q + 1], b3[4 * q + 2], b3[4 * q + 3]};
+#pragma unroll
+        for (int kt = 0; kt < 4; kt++)
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(load_frag(w3t, 64, col, 16 * kt + 4 * q), h2[kt], acc,
                                                            0, 0, 0);
         res[g][0] = acc[0];
         res[g][1] = acc[1];
@@ -180,8 +211,58 @@ __device__ __forceinline__ void mlp_wave(const MlpDev &m, const float (&f)[5], f
 // ---------------------------------------------------------------------------
 struct DragRate {
     double k, Re, Kn;
-    double g;  // d ln k / d tau at tau = 0 along the relaxation path
+    double g;    // d ln k / d tau at tau = 0 along the relaxation path
+    double lRe;  // ln Re
+    double lT;   // ln(alpha u), the rate's temperature logarithm
 };
+
+// The surrogate path's FP64 arithmetic restates the drag formulas of
+// drag_device.h (drag_k, drag_load, drag_apply) with refined reciprocals
+// (rcp_nr, within 1-2 ulp) in place of IEEE divisions: its accuracy bar (the
+// residual tolerance against the exact integrator) is many orders above these
+// ulps, and a reciprocal shared by several quotients costs a fraction of an
+// IEEE division each.
+
+// |dv| and u_0 of a register cell, with 1/rho_g and 1/rho_d
+__device__ __forceinline__ DragCell learned_load(const double (&G)[5], const double (&D)[5], double &irg,
+                                                 double &ird)
+{
+    DragCell s;
+    s.rg = G[0];
+    s.rd = D[0];
+    irg = rcp_nr(s.rg);
+    ird = rcp_nr(s.rd);
+    double dv = G[2] * irg - D[2] * ird;
+    double dvsq = dv * dv;
+    dv = G[3] * irg - D[3] * ird;
+    dvsq += dv * dv;
+    dv = G[4] * irg - D[4] * ird;
+    dvsq += dv * dv;
+    s.s0 = sqrt_nr(dvsq);
+    s.u0 = (G[1] - .5 * (G[2] * G[2] + G[3] * G[3] + G[4] * G[4]) * irg) * irg;
+    return s;
+}
+
+// drag_apply's update about the barycentric velocity (dv scaled by f)
+__device__ __forceinline__ void learned_apply(double (&G)[5], double (&D)[5], const DragCell &s, double irg,
+                                              double ird, double f)
+{
+    const double irs = rcp_nr(s.rg + s.rd);
+    const double mu = s.rd * irs * s.rg;
+    double pdustsq = 0.0;
+#pragma unroll
+    for (int c = 2; c <= 4; c++) {
+        const double pg = G[c], pd = D[c];
+        pdustsq -= pd * pd;
+        const double vstick = (pg + pd) * irs;
+        const double dv_t = (pg * irg - pd * ird) * f;
+        G[c] = s.rg * vstick + dv_t * mu;
+        const double q = s.rd * vstick - dv_t * mu;
+        D[c] = q;
+        pdustsq += q * q;
+    }
+    G[1] -= .5 * pdustsq * ird;
+}
 
 // k0, Re0, Kn0 and the initial slope. With k = C_d(Re, Kn) |dv| (rho_g +
 // rho_d) eps, Re ~ |dv| T^viscpow, Kn ~ T^lampow, T ~ u, and along the path
@@ -192,31 +273,48 @@ __device__ __forceinline__ DragRate drag_k_parts(const DragConsts &c, double rho
                                                  double u)
 {
     DragRate r;
-    const double lT = log(c.alpha * u);
-    r.Re = c.delta * rhog * magdv * pow_from_log(lT, c.viscpow);
-    r.Kn = c.beta * pow_from_log(lT, c.lampow) / rhog;
-    const double E = exp(-0.999 / r.Kn);
+    r.lT = log(c.alpha * u);
+    r.Re = c.delta * rhog * magdv * pow_from_log(r.lT, c.viscpow);
+    r.Kn = c.beta * pow_from_log(r.lT, c.lampow) * rcp_nr(rhog);
+    const double E = exp(-0.999 * rcp_nr(r.Kn));
     const double cunningham = 1 + r.Kn * (1.142 + 0.558 * E);
-    const double p319 = exp(-.319 * log(r.Re));
-    const double N = 3 / r.Re + .45 * p319 + .0509 * r.Re / (8710 + r.Re);
-    const double Cd = N / cunningham;
-    r.k = Cd * magdv * (rhog + rhod) * c.epsilon;
+    r.lRe = log(r.Re);
+    const double p319 = exp(-.319 * r.lRe);
+    const double iRe = rcp_nr(r.Re);
     const double q = 8710 + r.Re;
-    const double a = (-3 / r.Re - .319 * .45 * p319 + .0509 * 8710 * r.Re / (q * q)) / N;
-    const double b = -(1.142 * r.Kn + 0.558 * E * (r.Kn + 0.999)) / cunningham;
-    r.g = -(1.0 + a) + (a * c.viscpow + b * c.lampow) * rhod * magdv * magdv / ((rhog + rhod) * u);
+    const double iq = rcp_nr(q);
+    const double N = 3 * iRe + .45 * p319 + .0509 * r.Re * iq;
+    const double icun = rcp_nr(cunningham);
+    r.k = N * icun * magdv * (rhog + rhod) * c.epsilon;
+    const double a = (-3 * iRe - .319 * .45 * p319 + .0509 * 8710 * r.Re * iq * iq) * rcp_nr(N);
+    const double b = -(1.142 * r.Kn + 0.558 * E * (r.Kn + 0.999)) * icun;
+    r.g = -(1.0 + a) + (a * c.viscpow + b * c.lampow) * rhod * magdv * magdv * rcp_nr((rhog + rhod) * u);
     return r;
+}
+
+// The rate at |dv| = s1 = s0 e^{-x}, u = u1 from the initial one r0: ln Re
+// follows as ln Re0 - x + viscpow (ln T1 - ln T0), one logarithm fewer than
+// drag_k.
+__device__ __forceinline__ double drag_k_next(const DragConsts &c, const DragRate &r0, double rhog, double rhod,
+                                              double s1, double x, double u1)
+{
+    const double lT = log(c.alpha * u1);
+    const double Re = c.delta * rhog * s1 * pow_from_log(lT, c.viscpow);
+    const double Kn = c.beta * pow_from_log(lT, c.lampow) * rcp_nr(rhog);
+    const double cunningham = 1 + Kn * (1.142 + 0.558 * exp(-0.999 * rcp_nr(Kn)));
+    const double lRe = r0.lRe - x + c.viscpow * (lT - r0.lT);
+    const double N = 3 * rcp_nr(Re) + .45 * exp(-.319 * lRe) + .0509 * Re * rcp_nr(8710 + Re);
+    return N * rcp_nr(cunningham) * s1 * (rhog + rhod) * c.epsilon;
 }
 
 // raw features: ln(dt k0), ln(rho_d/rho_g), ln(|dv|/c_s), ln Re0, ln Kn0
 __device__ __forceinline__ void drag_features(const DragConsts &c, const DragCell &s, const DragRate &r, double dt,
                                               float (&f)[5])
 {
-    const double cs = sqrt((c.gm1 + 1.0) * c.gm1 * s.u0);
     f[0] = __logf((float)(dt * r.k));
-    f[1] = __logf((float)(s.rd / s.rg));
-    f[2] = __logf((float)(s.s0 / cs));
-    f[3] = __logf((float)r.Re);
+    f[1] = __logf((float)s.rd) - __logf((float)s.rg);
+    f[2] = __logf((float)s.s0) - 0.5f * __logf((float)((c.gm1 + 1.0) * c.gm1 * s.u0));
+    f[3] = (float)r.lRe;
     f[4] = __logf((float)r.Kn);
 }
 
@@ -224,23 +322,22 @@ __device__ __forceinline__ void drag_features(const DragConsts &c, const DragCel
 // learned drag source
 // ---------------------------------------------------------------------------
 // Cells the surrogate does not accept are appended to a queue (one atomic
-// per wave) and integrated by fallback_drag_kernel in dense waves, so a few
+// per wave) and integrated by the fallback kernels in dense waves, so a few
 // rejected lanes do not hold their whole wave in the adaptive integrator.
-// One wave of 64 cells [base, base+64) of an n-cell range: features, network,
-// residual check; accepted cells are updated, rejected ones appended to the
-// fallback queue as qtag | index (one atomic per wave).
-__device__ __forceinline__ void learned_cells(double *__restrict__ g, double *__restrict__ d, int64_t pitch,
-                                              int64_t base, int64_t n, const DragConsts &c, double dt,
-                                              const MlpDev &m, int64_t *__restrict__ queue,
-                                              unsigned long long *__restrict__ qcount, int64_t qtag)
+// A queue entry is qtag | cell index, with kQueueSecond set when the cell
+// still owes a second learned call after its exact one (a source's two
+// half-interval calls applied in one pass, below).
+constexpr int64_t kQueueSecond = (int64_t)1 << 62;
+constexpr int64_t kQueueIndex = ((int64_t)1 << 32) - 1;
+
+// One learned drag call on cell s (register copy): features, network,
+// residual check. Must be called by all 64 lanes of the wave together (the
+// network runs on the wave). ok: the surrogate's factor `fac` is accepted;
+// a cell the problem is not defined on is never accepted (the exact
+// integrator flags it, drag_exact_ratio).
+__device__ __forceinline__ void learned_step(const DragConsts &c, const DragCell &s, double dt, const MlpDev &m,
+                                             const MlpLds &Lm, bool &ok, double &fac)
 {
-    const int lane = threadIdx.x & 63;
-    const int64_t i = base + lane;
-    const bool live = i < n;
-    const int64_t ic = live ? i : n - 1;  // lanes past the end work on a valid cell and store nothing
-    const DragCell s = drag_load(g + ic, d + ic, pitch);
-    // cells the problem is not defined on go to the exact integrator, which
-    // flags them (drag_exact_ratio)
     const bool defined = drag_cell_defined(s.rg, s.rd, s.s0, s.u0);
     const bool moving = defined && s.s0 >= 1e-9;
     DragRate r0 = {0.0, 1.0, 1.0, 0.0};
@@ -255,9 +352,9 @@ __device__ __forceinline__ void learned_cells(double *__restrict__ g, double *__
         z[j] = inside ? (f[j] - m.norm.fmean[j]) / m.norm.fstd[j] : 0.0f;
     }
     float o[3];
-    mlp_wave(m, z, o);
-    double fac = 1.0;
-    bool reject = live && !defined;
+    mlp_wave(Lm, z, o);
+    fac = 1.0;
+    ok = defined;  // a cell below the rate's |dv| cut-off stays as it is
     if (moving) {
         const double tau = dt * r0.k;
         const double q0 = inside ? (double)(o[0] * m.norm.tstd[0] + m.norm.tmean[0]) : 0.0;
@@ -267,65 +364,217 @@ __device__ __forceinline__ void learned_cells(double *__restrict__ g, double *__
         const double kmid = r0.k * (1.0 + tau * (.5 * r0.g + tau * q1));
         const double kend = r0.k * (1.0 + tau * (r0.g + tau * q2));
         // end state implied by kbar; its true rate checks the prediction
-        const double s1 = s.s0 * exp(-kbar * dt);
-        const double u1 = s.u0 + .5 * s.rd / (s.rg + s.rd) * (s.s0 * s.s0 - s1 * s1);
-        const double k1 = drag_k(c, s.rg, s.rd, s1, u1);
+        const double x = kbar * dt;
+        fac = exp(-x);
+        const double s1 = s.s0 * fac;
+        const double u1 = s.u0 + .5 * s.rd * rcp_nr(s.rg + s.rd) * (s.s0 * s.s0 - s1 * s1);
+        const double k1 = drag_k_next(c, r0, s.rg, s.rd, s1, x, u1);
         const double ksimp = (r0.k + 4.0 * kmid + k1) * (1.0 / 6.0);
         const double e1 = fabs(dt * (k1 - kend));
         const double e2 = fabs(dt * (kbar - ksimp));
-        const bool ok = (e1 <= m.norm.residTol) && (e2 <= m.norm.residTol) && kbar > 0.0;
-        if (ok) fac = exp(-kbar * dt);
-        else reject = live;
+        ok = (e1 <= m.norm.residTol) && (e2 <= m.norm.residTol) && kbar > 0.0;
+        if (!ok) fac = 1.0;
     }
+}
+
+// The nine values a drag call reads and writes, as a pitch-1 cell (the dust
+// energy is neither read nor written: D[1] stays 0).
+__device__ __forceinline__ void cell_in(const double *g, const double *d, int64_t pitch, double (&G)[5],
+                                        double (&D)[5])
+{
+#pragma unroll
+    for (int k = 0; k < 5; k++) G[k] = g[k * pitch];
+    D[0] = d[0];
+    D[1] = 0.0;
+#pragma unroll
+    for (int k = 2; k < 5; k++) D[k] = d[k * pitch];
+}
+
+__device__ __forceinline__ void cell_out(double *g, double *d, int64_t pitch, const double (&G)[5],
+                                         const double (&D)[5])
+{
+    g[pitch] = G[1];
+#pragma unroll
+    for (int k = 2; k < 5; k++) {
+        g[k * pitch] = G[k];
+        d[k * pitch] = D[k];
+    }
+}
+
+// One learned drag call on a register cell, applied when accepted. Not
+// inlined: the learned and the fallback kernels run these very instructions,
+// so a call gives bitwise the same result whichever kernel takes it (copies
+// inlined into different code can contract their FMAs differently). Must be
+// called by all 64 lanes of the wave together.
+__device__ __forceinline__ bool learned_call(double (&G)[5], double (&D)[5], const DragConsts &c,
+                                                       double dt, const MlpDev &m, const MlpLds &Lm)
+{
+    double irg, ird;
+    const DragCell s = learned_load(G, D, irg, ird);
+    bool ok;
+    double fac;
+    learned_step(c, s, dt, m, Lm, ok, fac);
+    if (ok) learned_apply(G, D, s, irg, ird, fac);
+    return ok;
+}
+
+// One exact (adaptive integrator) drag call on a register cell; counts
+// invalid cells and budget overruns when `count`. Not inlined, as learned_call.
+__device__ __forceinline__ void exact_call(double (&G)[5], double (&D)[5], const DragConsts &c,
+                                                     double dt, bool count, unsigned &invalid, unsigned &overrun)
+{
+    const DragCell s = drag_load(G, D, 1);
+    int steps;
+    const double fac = drag_exact_ratio(c, s.rg, s.rd, s.s0, s.u0, dt, &steps);
+    if (count) {
+        invalid += (steps == kDragInvalid);
+        overrun += (steps < 0 && steps != kDragInvalid);
+    }
+    drag_apply(G, D, 1, s, fac);
+}
+
+// One wave of 64 cells [base, base+64) of an n-cell range: `calls` (1 or 2)
+// learned drag calls of dt each, in registers. A cell that one call does not
+// accept is stored as the accepted calls left it and queued (one atomic per
+// wave): its remaining calls are taken by the fallback kernel exactly as two
+// separate launches would take them (exact integrator for the rejected call,
+// then, if the first was rejected, the learned second call with its own
+// check), so both forms give the same result.
+__device__ __forceinline__ void learned_cells(double *__restrict__ g, double *__restrict__ d, int64_t pitch,
+                                              int64_t base, int64_t n, const DragConsts &c, double dt,
+                                              const MlpDev &m, const MlpLds &Lm, int64_t *__restrict__ queue,
+                                              unsigned long long *__restrict__ qcount, int64_t qtag, int calls)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t i = base + lane;
+    const bool live = i < n;
+    const int64_t ic = live ? i : n - 1;  // lanes past the end work on a valid cell and store nothing
+    double G[5], D[5];
+    cell_in(g + ic, d + ic, pitch, G, D);
+    int done = 0;  // calls accepted before the first rejection
+    bool rejected = false;
+#pragma unroll 1
+    for (int r = 0; r < calls; r++) {
+        // a rejected lane's copy is left as it was: the call only keeps the
+        // wave together for the network
+        double G2[5], D2[5];
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            G2[k] = G[k];
+            D2[k] = D[k];
+        }
+        const bool ok = learned_call(G2, D2, c, dt, m, Lm);
+        if (!rejected) {
+            if (ok) {
+#pragma unroll
+                for (int k = 0; k < 5; k++) {
+                    G[k] = G2[k];
+                    D[k] = D2[k];
+                }
+                done++;
+            } else {
+                rejected = true;
+            }
+        }
+    }
+    const bool reject = live && rejected;
     const unsigned long long mask = __ballot(reject);
     if (mask) {
         const int leader = __ffsll((long long)mask) - 1;
         unsigned long long slot = 0;
         if (lane == leader) slot = atomicAdd(qcount, (unsigned long long)__popcll(mask));
         slot = __shfl(slot, leader, 64);
-        if (reject) queue[slot + __popcll(mask & ((1ull << lane) - 1ull))] = qtag | i;
+        const int64_t tag = qtag | i | ((done == 0 && calls == 2) ? kQueueSecond : 0);
+        if (reject) queue[slot + __popcll(mask & ((1ull << lane) - 1ull))] = tag;
     }
-    if (live && !reject) drag_apply(g + i, d + i, pitch, s, fac);
+    if (live && done > 0) cell_out(g + i, d + i, pitch, G, D);
+}
+
+// The fallback of one wave of queue entries (cell pointers gi/di per lane,
+// dt per lane): the exact integrator for the rejected call, then, for
+// entries marked kQueueSecond, the learned second call and, where that is
+// rejected too, the exact integrator again. Must be called by all 64 lanes
+// together (live = the lane holds an entry). Returns the number of second
+// calls this lane sent to the exact integrator.
+__device__ __forceinline__ unsigned fallback_cell(double *gi, double *di, int64_t pitch, bool live, bool second,
+                                                  const DragConsts &c, double dt, const MlpDev &m,
+                                                  const MlpLds &Lm, bool staged, unsigned &invalid,
+                                                  unsigned &overrun)
+{
+    double G[5], D[5];
+    cell_in(gi, di, pitch, G, D);
+    exact_call(G, D, c, dt, live, invalid, overrun);
+    unsigned extra = 0;
+    if (staged && __ballot(second)) {  // wave-uniform: the network runs on the whole wave
+        double G2[5], D2[5];
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            G2[k] = G[k];
+            D2[k] = D[k];
+        }
+        const bool ok = learned_call(G2, D2, c, dt, m, Lm);
+        if (second) {
+            if (ok) {
+#pragma unroll
+                for (int k = 0; k < 5; k++) {
+                    G[k] = G2[k];
+                    D[k] = D2[k];
+                }
+            } else {
+                exact_call(G, D, c, dt, live, invalid, overrun);
+                extra = 1;
+            }
+        }
+    }
+    if (live) cell_out(gi, di, pitch, G, D);
+    return extra;
 }
 
 __global__ __launch_bounds__(256) void learned_drag_kernel(double *__restrict__ g, double *__restrict__ d,
                                                            int64_t pitch, int64_t n, DragConsts c, DragDt dtv,
-                                                           MlpDev m, int64_t *__restrict__ queue,
+                                                           MlpDev m, int calls, int64_t *__restrict__ queue,
                                                            unsigned long long *__restrict__ qcount)
 {
+    __shared__ MlpLds Lm;
+    stage_model(Lm, m);
     const double dt = drag_dt(dtv);
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
     for (int64_t base = wave * 64; base < n; base += nwaves * 64)
-        learned_cells(g, d, pitch, base, n, c, dt, m, queue, qcount, 0);
+        learned_cells(g, d, pitch, base, n, c, dt, m, Lm, queue, qcount, 0, calls);
 }
 
 __global__ __launch_bounds__(256) void fallback_drag_kernel(double *__restrict__ g, double *__restrict__ d,
                                                             int64_t pitch, int64_t n, DragConsts c, DragDt dtv,
-                                                            const int64_t *__restrict__ queue,
+                                                            MlpDev m, int calls, const int64_t *__restrict__ queue,
                                                             const unsigned long long *__restrict__ qcount,
                                                             unsigned long long *__restrict__ counts,
                                                             unsigned long long *__restrict__ err)
 {
+    __shared__ MlpLds Lm;
+    const bool staged = calls == 2;  // uniform: only a two-call pass owes learned second calls
+    if (staged) stage_model(Lm, m);
     const double dt = drag_dt(dtv);
     const unsigned long long nq = *qcount;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        atomicAdd(counts, (unsigned long long)n);
+        atomicAdd(counts, (unsigned long long)n * (unsigned long long)calls);
         if (nq) atomicAdd(counts + 1, nq);
     }
-    unsigned invalid = 0, overrun = 0;
-    for (unsigned long long t = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; t < nq;
-         t += (unsigned long long)gridDim.x * blockDim.x) {
-        const int64_t i = queue[t];
-        const DragCell s = drag_load(g + i, d + i, pitch);
-        int steps;
-        const double fac = drag_exact_ratio(c, s.rg, s.rd, s.s0, s.u0, dt, &steps);
-        invalid += (steps == kDragInvalid);
-        overrun += (steps < 0 && steps != kDragInvalid);
-        drag_apply(g + i, d + i, pitch, s, fac);
+    unsigned invalid = 0, overrun = 0, extra = 0;
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (unsigned long long base = (unsigned long long)wave * 64; base < nq; base += (unsigned long long)nwaves * 64) {
+        const unsigned long long t = base + lane;
+        const bool live = t < nq;
+        const int64_t e = queue[live ? t : base];
+        const int64_t i = e & ~kQueueSecond;
+        const bool second = live && (e & kQueueSecond);
+        extra += fallback_cell(g + i, d + i, pitch, live, second, c, dt, m, Lm, staged, invalid, overrun);
     }
     wave_count(err + ERRW_DRAG_INVALID, invalid);
     wave_count(err + ERRW_DRAG_BUDGET, overrun);
+    wave_count(counts + 1, extra);
 }
 
 __global__ void zero_counter(unsigned long long *p)
@@ -334,40 +583,51 @@ __global__ void zero_counter(unsigned long long *p)
 }
 
 // ---------------------------------------------------------------------------
-// drag over the dusty problems of an ensemble: one wave per 64-cell item,
-// dt = frac * dTime of the item's problem (read on the device)
+// drag over the dusty problems of an ensemble: persistent waves over 64-cell
+// items, dt = frac * dTime of the item's problem (read on the device), `calls`
+// (1 or 2) calls of dt each applied in one pass
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void ens_drag_kernel(const EnsDragTable *__restrict__ tab,
-                                                       const EnsDragItem *__restrict__ items, int nitems,
-                                                       const double *__restrict__ dtp, double frac, DragConsts c,
-                                                       int method, MlpDev m, int64_t *__restrict__ queue,
-                                                       unsigned long long *__restrict__ qcount,
-                                                       unsigned long long *__restrict__ err)
+template <bool LEARNED>
+__global__ __launch_bounds__(256, 2) void ens_drag_kernel(const EnsDragTable *__restrict__ tab,
+                                                          const EnsDragItem *__restrict__ items, int nitems,
+                                                          const double *__restrict__ dtp, double frac, DragConsts c,
+                                                          int method, int calls, MlpDev m,
+                                                          int64_t *__restrict__ queue,
+                                                          unsigned long long *__restrict__ qcount,
+                                                          unsigned long long *__restrict__ err)
 {
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    if (wave >= nitems) return;
-    const EnsDragItem it = items[wave];
-    const EnsDragTable T = tab[it.t];
-    const double dt = frac * dtp[T.p];
-    if (method == DRAG_LEARNED) {
-        learned_cells(T.g, T.d, T.pitch, it.base, T.n, c, dt, m, queue, qcount, (int64_t)it.t << 32);
-        return;
-    }
-    const int64_t i = it.base + (threadIdx.x & 63);
+    __shared__ MlpLds Lm;
+    if (LEARNED) stage_model(Lm, m);
+    const int wave0 = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int nwaves = (int)(((int64_t)gridDim.x * blockDim.x) >> 6);
     unsigned invalid = 0, overrun = 0;
-    if (i < T.n) {
-        const DragCell s = drag_load(T.g + i, T.d + i, T.pitch);
-        double f;
-        if (method == DRAG_EXACT) {
-            int k;
-            f = drag_exact_ratio(c, s.rg, s.rd, s.s0, s.u0, dt, &k);
-            invalid = (k == kDragInvalid);
-            overrun = (k < 0 && k != kDragInvalid);
-        } else {
-            invalid = !drag_cell_defined(s.rg, s.rd, s.s0, s.u0);
-            f = exp(-dt * drag_logtrap_rate(c, s.rg, s.rd, s.s0, s.u0, dt, method == DRAG_LOGTRAP3 ? 3 : 2));
+    for (int wave = wave0; wave < nitems; wave += nwaves) {
+        const EnsDragItem it = items[wave];
+        const EnsDragTable T = tab[it.t];
+        const double dt = frac * dtp[T.p];
+        if (LEARNED) {
+            learned_cells(T.g, T.d, T.pitch, it.base, T.n, c, dt, m, Lm, queue, qcount, (int64_t)it.t << 32, calls);
+            continue;
         }
-        drag_apply(T.g + i, T.d + i, T.pitch, s, f);
+        const int64_t i = it.base + (threadIdx.x & 63);
+        if (i >= T.n) continue;
+        double G[5], D[5];
+        cell_in(T.g + i, T.d + i, T.pitch, G, D);
+        for (int r = 0; r < calls; r++) {
+            const DragCell s = drag_load(G, D, 1);
+            double f;
+            if (method == DRAG_EXACT) {
+                int k;
+                f = drag_exact_ratio(c, s.rg, s.rd, s.s0, s.u0, dt, &k);
+                invalid += (k == kDragInvalid);
+                overrun += (k < 0 && k != kDragInvalid);
+            } else {
+                invalid += !drag_cell_defined(s.rg, s.rd, s.s0, s.u0);
+                f = exp(-dt * drag_logtrap_rate(c, s.rg, s.rd, s.s0, s.u0, dt, method == DRAG_LOGTRAP3 ? 3 : 2));
+            }
+            drag_apply(G, D, 1, s, f);
+        }
+        cell_out(T.g + i, T.d + i, T.pitch, G, D);
     }
     wave_count(err + ERRW_DRAG_INVALID, invalid);
     wave_count(err + ERRW_DRAG_BUDGET, overrun);
@@ -375,37 +635,44 @@ __global__ __launch_bounds__(256) void ens_drag_kernel(const EnsDragTable *__res
 
 __global__ __launch_bounds__(256) void ens_fallback_kernel(const EnsDragTable *__restrict__ tab,
                                                            const double *__restrict__ dtp, double frac, DragConsts c,
-                                                           const int64_t *__restrict__ queue,
+                                                           MlpDev m, int calls, const int64_t *__restrict__ queue,
                                                            const unsigned long long *__restrict__ qcount,
                                                            int64_t ncells, unsigned long long *__restrict__ counts,
                                                            unsigned long long *__restrict__ err)
 {
+    __shared__ MlpLds Lm;
+    const bool staged = calls == 2;
+    if (staged) stage_model(Lm, m);
     const unsigned long long nq = *qcount;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        atomicAdd(counts, (unsigned long long)ncells);
+        atomicAdd(counts, (unsigned long long)ncells * (unsigned long long)calls);
         if (nq) atomicAdd(counts + 1, nq);
     }
-    unsigned invalid = 0, overrun = 0;
-    for (unsigned long long t = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; t < nq;
-         t += (unsigned long long)gridDim.x * blockDim.x) {
-        const int64_t e = queue[t];
-        const EnsDragTable T = tab[(int)(e >> 32)];
-        const int64_t i = e & 0xffffffffll;
-        const DragCell s = drag_load(T.g + i, T.d + i, T.pitch);
-        int steps;
-        const double fac = drag_exact_ratio(c, s.rg, s.rd, s.s0, s.u0, frac * dtp[T.p], &steps);
-        invalid += (steps == kDragInvalid);
-        overrun += (steps < 0 && steps != kDragInvalid);
-        drag_apply(T.g + i, T.d + i, T.pitch, s, fac);
+    unsigned invalid = 0, overrun = 0, extra = 0;
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (unsigned long long base = (unsigned long long)wave * 64; base < nq; base += (unsigned long long)nwaves * 64) {
+        const unsigned long long t = base + lane;
+        const bool live = t < nq;
+        const int64_t e = queue[live ? t : base];
+        const EnsDragTable T = tab[(int)((e & ~kQueueSecond) >> 32)];
+        const int64_t i = e & kQueueIndex;
+        const bool second = live && (e & kQueueSecond);
+        extra += fallback_cell(T.g + i, T.d + i, T.pitch, live, second, c, frac * dtp[T.p], m, Lm, staged, invalid,
+                               overrun);
     }
     wave_count(err + ERRW_DRAG_INVALID, invalid);
     wave_count(err + ERRW_DRAG_BUDGET, overrun);
+    wave_count(counts + 1, extra);
 }
 
 // standalone BF16 inference (tests)
 __global__ __launch_bounds__(256) void mlp_infer_kernel(int64_t batch, const float *__restrict__ x,
                                                         float *__restrict__ y, MlpDev m)
 {
+    __shared__ MlpLds Lm;
+    stage_model(Lm, m);
     const int lane = threadIdx.x & 63;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
@@ -415,7 +682,7 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(int64_t batch, const flo
         float f[5], o[3];
 #pragma unroll
         for (int j = 0; j < 5; j++) f[j] = x[ic * 5 + j];
-        mlp_wave(m, f, o);
+        mlp_wave(Lm, f, o);
         if (i < batch) {
 #pragma unroll
             for (int j = 0; j < 3; j++) y[i * 3 + j] = o[j];
@@ -692,14 +959,15 @@ static int ensure_counts()
 }
 
 int launch_learned_drag(double *g, double *d, int64_t pitch, int64_t n, const DragConsts &c, DragDt dt,
-                        int64_t *queue, hipStream_t s)
+                        int64_t *queue, hipStream_t s, int calls)
 {
+    if (calls < 1 || calls > 2) return IM_FAIL(ERROR_INVALID_ARGS, "drag calls per pass: %d", calls);
     if (!g_model.loaded) return IM_FAIL(ERROR_INVALID_ARGS, "learned drag: no model (imogen_drag_set_model)");
     if (ensure_counts()) return ERROR_CUDA_BLEW_UP;
     int64_t waves = (n + 63) / 64;
     if (waves > 8192) waves = 8192;
     const int64_t blocks = (waves + 3) / 4;
-    if (!queue) queue = (int64_t *)scratch((size_t)n * sizeof(int64_t));
+    if (!queue) queue = (int64_t *)scratch_block((size_t)n * sizeof(int64_t));
     if (!queue) return IM_FAIL(ERROR_NOMEM, "no scratch for the drag fallback queue");
     unsigned long long *qcount = g_model.counts + 2;
     const int slot = prof_begin(PROF_MLP, s);
@@ -707,19 +975,20 @@ int launch_learned_drag(double *g, double *d, int64_t pitch, int64_t n, const Dr
     int rc = IM_LAUNCHED("zero_counter");
     if (rc) return rc;
     hipLaunchKernelGGL(learned_drag_kernel, dim3((unsigned)blocks), dim3(256), 0, s, g, d, pitch, n, c, dt,
-                       g_model.dev, queue, qcount);
+                       g_model.dev, calls, queue, qcount);
     rc = IM_LAUNCHED("learned_drag_kernel");
     if (rc) return rc;
-    hipLaunchKernelGGL(fallback_drag_kernel, dim3(1024), dim3(256), 0, s, g, d, pitch, n, c, dt, queue, qcount,
-                       g_model.counts, state()->err);
+    hipLaunchKernelGGL(fallback_drag_kernel, dim3(1024), dim3(256), 0, s, g, d, pitch, n, c, dt, g_model.dev, calls,
+                       queue, qcount, g_model.counts, state()->err);
     prof_end(slot, s);
     return IM_LAUNCHED("fallback_drag_kernel");
 }
 
 int launch_ens_drag(const EnsDragTable *tab, const EnsDragItem *items, int nitems, int64_t ncells,
                     const double *dtp, double frac, const DragConsts &c, int method, int64_t *queue,
-                    unsigned long long *qcount, hipStream_t s)
+                    unsigned long long *qcount, hipStream_t s, int calls)
 {
+    if (calls < 1 || calls > 2) return IM_FAIL(ERROR_INVALID_ARGS, "drag calls per pass: %d", calls);
     if (method != DRAG_LOGTRAP2 && method != DRAG_LOGTRAP3 && method != DRAG_EXACT && method != DRAG_LEARNED)
         return IM_FAIL(ERROR_NOIMPLEMENT, "ensemble drag method %d", method);
     if (method == DRAG_LEARNED && !g_model.loaded)
@@ -731,13 +1000,19 @@ int launch_ens_drag(const EnsDragTable *tab, const EnsDragItem *items, int nitem
         if (rc) return rc;
     }
     unsigned long long *err = state()->err;
-    hipLaunchKernelGGL(ens_drag_kernel, dim3((unsigned)((nitems + 3) / 4)), dim3(256), 0, s, tab, items, nitems, dtp,
-                       frac, c, method, g_model.dev, queue, qcount, err);
+    // persistent waves (a block stages the model once for many items)
+    const int blocks = (nitems + 3) / 4 < 2048 ? (nitems + 3) / 4 : 2048;
+    if (method == DRAG_LEARNED)
+        hipLaunchKernelGGL(ens_drag_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, s, tab, items, nitems, dtp,
+                           frac, c, method, calls, g_model.dev, queue, qcount, err);
+    else
+        hipLaunchKernelGGL(ens_drag_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, s, tab, items, nitems, dtp,
+                           frac, c, method, calls, g_model.dev, queue, qcount, err);
     int rc = IM_LAUNCHED("ens_drag_kernel");
     if (rc || method != DRAG_LEARNED) return rc;
     if (!g_model.counts) return IM_FAIL(ERROR_INVALID_ARGS, "learned drag counters not prepared (learned_drag_prepare)");
-    hipLaunchKernelGGL(ens_fallback_kernel, dim3(1024), dim3(256), 0, s, tab, dtp, frac, c, queue, qcount, ncells,
-                       g_model.counts, err);
+    hipLaunchKernelGGL(ens_fallback_kernel, dim3(1024), dim3(256), 0, s, tab, dtp, frac, c, g_model.dev, calls, queue,
+                       qcount, ncells, g_model.counts, err);
     return IM_LAUNCHED("ens_fallback_kernel");
 }
 
@@ -859,7 +1134,7 @@ int imogen_mlp_backward(const float *params, int64_t batch, const float *x, cons
     DeviceState *st = state();
     if (!st) return ERROR_CUDA_BLEW_UP;
     const int64_t blocks = (batch + kRows - 1) / kRows;
-    float *part = (float *)scratch((size_t)blocks * kNParams * sizeof(float));
+    float *part = (float *)scratch_block((size_t)blocks * kNParams * sizeof(float));
     if (!part) return IM_FAIL(ERROR_NOMEM, "no scratch for gradient partials");
     hipLaunchKernelGGL(mlp_bwd_kernel, dim3((unsigned)blocks), dim3(256), 0, st->stream, params, batch, x, act, dy,
                        part);

@@ -38,6 +38,14 @@ int fail(int status, const char *file, int line, const char *fmt, ...)
     return status;
 }
 
+// Device memory every call on a GPU can use without allocating: the
+// reductions' scalars and the error counters (zero at load, as a static
+// array). Each GPU holds its own copy of these arrays from the library's first
+// call on it until the process ends; host code addresses them only through the
+// per-device pointers that hipGetSymbolAddress returns in state().
+static __device__ __align__(256) double g_devScalars[4096];
+static __device__ __align__(256) unsigned long long g_errWord[ERRW_SLOTS];
+
 DeviceState *state()
 {
     int dev = 0;
@@ -46,22 +54,28 @@ DeviceState *state()
     DeviceState *st = &g_state[dev];
     if (st->device < 0) {
         if (IM_HIP(hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking))) return nullptr;
-        if (IM_HIP(hipHostMalloc((void **)&st->pinned, 64 * sizeof(double), hipHostMallocDefault))) return nullptr;
-        if (IM_HIP(hipMalloc((void **)&st->devScalars, 4096 * sizeof(double)))) return nullptr;
-        if (IM_HIP(hipMalloc((void **)&st->err, ERRW_SLOTS * sizeof(unsigned long long)))) return nullptr;
-        if (IM_HIP(hipHostMalloc((void **)&st->errHost, ERRW_SLOTS * sizeof(unsigned long long),
-                                 hipHostMallocDefault)))
-            return nullptr;
-        if (IM_HIP(hipMemsetAsync(st->err, 0, ERRW_SLOTS * sizeof(unsigned long long), st->stream))) return nullptr;
-        if (IM_HIP(hipStreamSynchronize(st->stream))) return nullptr;
+        if (IM_HIP(hipGetSymbolAddress((void **)&st->devScalars, HIP_SYMBOL(g_devScalars)))) return nullptr;
+        if (IM_HIP(hipGetSymbolAddress((void **)&st->err, HIP_SYMBOL(g_errWord)))) return nullptr;
         st->device = dev;
     }
     return st;
 }
 
+int pinned_buffers(DeviceState *st)
+{
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    if (!st->pinned && IM_HIP(hipHostMalloc((void **)&st->pinned, 64 * sizeof(double), hipHostMallocDefault)))
+        return ERROR_CUDA_BLEW_UP;
+    if (!st->errHost && IM_HIP(hipHostMalloc((void **)&st->errHost, ERRW_SLOTS * sizeof(unsigned long long),
+                                             hipHostMallocDefault)))
+        return ERROR_CUDA_BLEW_UP;
+    return SUCCESSFUL;
+}
+
 int take_device_errors(DeviceState *st)
 {
     if (!st) return ERROR_CUDA_BLEW_UP;
+    if (pinned_buffers(st)) return ERROR_CUDA_BLEW_UP;
     const size_t bytes = ERRW_SLOTS * sizeof(unsigned long long);
     if (IM_HIP(hipMemcpyAsync(st->errHost, st->err, bytes, hipMemcpyDeviceToHost, st->stream))) return ERROR_CUDA_BLEW_UP;
     if (IM_HIP(hipStreamSynchronize(st->stream))) return ERROR_CUDA_BLEW_UP;
@@ -80,7 +94,7 @@ int take_device_errors(DeviceState *st)
                    c0, c1, c2, c3);
 }
 
-void *scratch(size_t bytes)
+void *scratch_block(size_t bytes)
 {
     DeviceState *st = state();
     if (!st) return nullptr;

@@ -112,6 +112,83 @@ __global__ void bc_face_yz(double *__restrict__ q, int64_t pitch, int nx, int64_
     }
 }
 
+// Outflow ghosts (BCModeTypes outflow, gpuclass/cudaCommon.h:46) of the faces
+// side0 .. side0 + nsides - 1 of direction dir, all five slabs: one thread per
+// (line along dir, side), the line's decision taken once from the normal
+// momentum of the interior cell next to the face (cell d or N-1-d). Outward
+// flow: the ghosts copy that cell. Otherwise the rules of cukern_SetOutflowX's
+// +x branch (cudaStatics.cu:486-531), which the fluid step applies along every
+// direction after rotating it into memory x: density copied, slab 2 zeroed,
+// slab 3 set to rho times the rest-frame speed on x faces (1.05 on -x,
+// -1.198 - 0.0047 i on +x, :382-385) or to rho times the cell's own velocity
+// elsewhere, slab 4 zeroed, and the total energy changed by the kinetic
+// energy of each replaced momentum in that order. Reads and writes stay
+// inside the line (the reference's -x kernel starts depth cells before it
+// and its y/z kernels read the wrong rows, cudaStatics.cu:539-560, 581, 612).
+__global__ void bc_outflow(double *__restrict__ q, int64_t pitch, int nx, int64_t n2, int64_t stride,
+                           int64_t ostride, int N, int dir, int side0, int nsides, int d)
+{
+    // evaluated as written (no FMA contraction): the energy update uses the
+    // stored momenta p2 = rho v, as the reference's kernel does
+#pragma clang fp contract(off)
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t lines = (int64_t)((dir == 1) ? 1 : nx) * n2;  // dir 1: rows (y, z); else (x, other)
+    if (tid >= lines * nsides) return;
+    const int side = side0 + (int)(tid / lines);
+    const int64_t l = tid % lines;
+    double *r = (dir == 1) ? q + l * ostride : q + (l % nx) + (l / nx) * ostride;
+    const int64_t c = (side ? N - 1 - d : d) * stride;
+    const double a = r[(1 + dir) * pitch + c];
+    const double dirsign = side ? 1.0 : -1.0;
+    if (a * dirsign > 0) {
+        for (int sl = 0; sl < 5; sl++) {
+            const double v = r[sl * pitch + c];
+            for (int k = 0; k < d; k++) r[sl * pitch + (side ? N - d + k : k) * stride] = v;
+        }
+        return;
+    }
+    const double rho = r[c], E0 = r[pitch + c];
+    const double px = r[2 * pitch + c], py = r[3 * pitch + c], pz = r[4 * pitch + c];
+    for (int k = 0; k < d; k++) {
+        const int64_t gi = (side ? N - d + k : k) * stride;
+        double E = E0;
+        double p2 = rho * 0.0;
+        r[gi] = rho;
+        r[2 * pitch + gi] = p2;
+        E += .5 * (p2 - px) * (p2 + px) / rho;
+        const double v = (dir == 1) ? (side ? -1.198 - .0047 * k : 1.05) : py / rho;
+        p2 = rho * v;
+        r[3 * pitch + gi] = p2;
+        E += .5 * (p2 - py) * (p2 + py) / rho;
+        p2 = rho * 0.0;
+        r[4 * pitch + gi] = p2;
+        E += .5 * (p2 - pz) * (p2 + pz) / rho;
+        r[pitch + gi] = E;
+    }
+}
+
+int launch_outflow_bc(double *base, int64_t pitch, int nx, int ny, int nz, int dir, int side0, int nsides,
+                      int depth, hipStream_t s)
+{
+    const int N = (dir == 1) ? nx : (dir == 2 ? ny : nz);
+    if (N < depth + 2) return IM_FAIL(ERROR_INVALID_ARGS, "outflow BC needs >= %d cells (got %d)", depth + 2, N);
+    if (nsides == 2 && N < 2 * depth + 2) return IM_FAIL(ERROR_INVALID_ARGS, "outflow faces overlap");
+    int64_t n2, stride, ostride;
+    if (dir == 1) {
+        n2 = (int64_t)ny * nz;
+        stride = 1;
+        ostride = nx;
+    } else {
+        n2 = (dir == 2) ? nz : ny;
+        stride = (dir == 2) ? (int64_t)nx : (int64_t)nx * ny;
+        ostride = (dir == 2) ? (int64_t)nx * ny : (int64_t)nx;
+    }
+    const int64_t threads = ((dir == 1) ? 1 : (int64_t)nx) * n2 * nsides;
+    hipLaunchKernelGGL(bc_outflow, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, base, pitch, nx, n2,
+                       stride, ostride, N, dir, side0, nsides, depth);
+    return IM_LAUNCHED("bc_outflow");
+}
+
 int launch_fluid_bc(double *base, int64_t pitch, int nx, int ny, int nz, int dir, const int *modes10,
                     int depth, hipStream_t s, int xrules)
 {

@@ -138,8 +138,9 @@ static CflArgs cfl_args(const TimeloopImpl *L, int f)
 }
 
 // sources/src0001.m: the drag over fraction * dTime as two calls of half that
-// interval (dt read from the device clock), then the boundary conditions of
-// both fluids in the directions FluidManager.setBoundaries visits
+// interval (dt read from the device clock; one pass over memory applies both
+// calls), then the boundary conditions of both fluids in the directions
+// FluidManager.setBoundaries visits
 static int source(TimeloopImpl *L, double fraction, hipStream_t s)
 {
     if (L->dragMethod < 0) return SUCCESSFUL;
@@ -150,10 +151,13 @@ static int source(TimeloopImpl *L, double fraction, hipStream_t s)
     if (rc) return rc;
     const int64_t n = (int64_t)L->fluid[0][0].dim[0] * L->fluid[0][0].dim[1] * L->fluid[0][0].dim[2];
     const DragDt dt{0.0, L->dbuf + 1, fraction / 2};
-    for (int call = 0; call < 2; call++) {
+    // the learned method takes its two calls as two launches (a cell its
+    // first call rejects must meet the second call's network as a separate
+    // launch would)
+    for (int call = 0; call < (L->dragMethod == DRAG_LEARNED ? 2 : 1); call++) {
         const int slot = prof_begin(PROF_DRAG, s);
-        rc = (L->dragMethod == DRAG_LEARNED) ? launch_learned_drag(g, d, pg, n, L->dc, dt, nullptr, s)
-                                             : launch_drag(g, d, pg, n, L->dc, dt, L->dragMethod, s);
+        rc = (L->dragMethod == DRAG_LEARNED) ? launch_learned_drag(g, d, pg, n, L->dc, dt, nullptr, s, 1)
+                                             : launch_drag(g, d, pg, n, L->dc, dt, L->dragMethod, s, 2);
         prof_end(slot, s);
         if (rc) return rc;
     }

@@ -296,7 +296,7 @@ int launch_xinjin_sweep(double *base, int64_t pitch, int nx, int ny, int nz, int
     XjConsts k{gamma, gamma - 1.0, rhomin, 1e-5 / (gamma - 1.0)};
     const int64_t nlines = (dir == 1) ? (int64_t)ny * nz : (dir == 2 ? (int64_t)nx * nz : (int64_t)nx * ny);
     // temporary: 5 slabs of U at the fluid's pitch, then the per-line freezing speeds
-    double *U = (double *)scratch((size_t)(5 * pitch + nlines + 32) * sizeof(double));
+    double *U = (double *)scratch_block((size_t)(5 * pitch + nlines + 32) * sizeof(double));
     if (!U) return IM_FAIL(ERROR_NOMEM, "no scratch for the Xin-Jin stages");
     double *cf = U + 5 * pitch;
     const int slot = prof_begin(PROF_SWEEP_X + dir - 1, s);

@@ -840,14 +840,79 @@ static int array_face_bc(double *phi, const orc_grid *g, int side, int memdir, i
     return ORC_OK;
 }
 
+/* Outflow face (BCModeTypes outflow, cudaCommon.h:46: scalars and transverse
+ * properties constant, normal momentum constant when outward, zero when
+ * inward), with the per-slab rules of cukern_SetOutflowX's in-bounds +x branch
+ * (cudaStatics.cu:486-531) on every face and both sides: the normal momentum
+ * is read at the interior cell next to the face (cell depth or N-1-depth
+ * along dir). Outward (sign of the face times that momentum > 0): all five
+ * slabs of the ghosts copy that cell. Otherwise the ghosts get its density,
+ * slab 2 (the reference's radial momentum) zero, slab 3 rho * v with v the
+ * rest-frame speed on x faces (1.05 on -x, -1.198 - 0.0047 i for ghost i of
+ * +x counted outward, cudaStatics.cu:382-385, 471-474) and the cell's own
+ * velocity elsewhere, slab 4 zero, and the cell's total energy changed by
+ * the kinetic energy of each replaced momentum, in that order (:510-518).
+ * Ghost cells are 0..depth-1 and N-depth..N-1 of each line along dir. */
+int orc_outflow_face(double *q, const orc_grid *g, int dir, int side, int depth)
+{
+    const long n3[3] = {g->nx, g->ny, g->nz};
+    const long s3[3] = {1, (long)g->nx, (long)g->nx * g->ny};
+    const int a1 = (dir == 1) ? 1 : 0, a2 = (dir == 3) ? 1 : 2;
+    if (dir < 1 || dir > 3 || side < 0 || side > 1) return ORC_EINVAL;
+    const long N = n3[dir - 1], st = s3[dir - 1], P = g->pitch;
+    const int d = depth;
+    if (N < d + 2) return ORC_EINVAL;
+    const long c = side ? N - 1 - d : d; /* interior cell next to the face */
+    const double dirsign = side ? 1.0 : -1.0;
+    long i, j;
+    int k;
+    for (j = 0; j < n3[a2]; j++)
+        for (i = 0; i < n3[a1]; i++) {
+            double *r = q + i * s3[a1] + j * s3[a2];
+            const double a = r[(1 + dir) * P + c * st];
+            for (k = 0; k < d; k++) {
+                const long gi = side ? N - d + k : k; /* ghost cell */
+                if (a * dirsign > 0) {
+                    int sl;
+                    for (sl = 0; sl < 5; sl++) r[sl * P + gi * st] = r[sl * P + c * st];
+                } else {
+                    const double rho = r[c * st];
+                    double E = r[P + c * st], p1, p2, v;
+                    r[gi * st] = rho;
+                    p1 = r[2 * P + c * st];
+                    p2 = rho * 0.0;
+                    r[2 * P + gi * st] = p2;
+                    E += .5 * (p2 - p1) * (p2 + p1) / rho;
+                    p1 = r[3 * P + c * st];
+                    if (dir == 1) v = side ? -1.198 - .0047 * k : 1.05;
+                    else v = p1 / rho;
+                    p2 = rho * v;
+                    r[3 * P + gi * st] = p2;
+                    E += .5 * (p2 - p1) * (p2 + p1) / rho;
+                    p1 = r[4 * P + c * st];
+                    p2 = rho * 0.0;
+                    r[4 * P + gi * st] = p2;
+                    E += .5 * (p2 - p1) * (p2 + p1) / rho;
+                    r[P + gi * st] = E;
+                }
+            }
+        }
+    return ORC_OK;
+}
+
 int orc_set_fluid_boundary(double *q, const orc_grid *g, int dir, const int *modes, int depth)
 {
     if (dir < 1 || dir > 3) return ORC_EINVAL;
     int side, s;
     for (side = 0; side < 2; side++) {
         int bt = modes[2 * (dir - 1) + side];
-        if (bt == ORC_BC_OUTFLOW || bt == ORC_BC_FREEBALANCE || bt == ORC_BC_WALL) return ORC_ENOIMPL;
+        if (bt == ORC_BC_FREEBALANCE || bt == ORC_BC_WALL) return ORC_ENOIMPL;
         if (bt == ORC_BC_CIRCULAR || bt == ORC_BC_STATIONARY) continue;
+        if (bt == ORC_BC_OUTFLOW) {
+            int rc = orc_outflow_face(q, g, dir, side, depth);
+            if (rc) return rc;
+            continue;
+        }
         for (s = 0; s < 5; s++) {
             int sas;
             int veccomp = (s < 2) ? 0 : s - 1; /* px,py,pz carry vectorComponent 1,2,3 */
@@ -904,8 +969,13 @@ int orc_fluid_boundary_swept(double *q, const orc_grid *g, int dir, const int *m
     int side, s;
     for (side = 0; side < 2; side++) {
         int bt = modes[2 * (dir - 1) + side];
-        if (bt == ORC_BC_OUTFLOW || bt == ORC_BC_FREEBALANCE || bt == ORC_BC_WALL) return ORC_ENOIMPL;
+        if (bt == ORC_BC_FREEBALANCE || bt == ORC_BC_WALL) return ORC_ENOIMPL;
         if (bt == ORC_BC_CIRCULAR || bt == ORC_BC_STATIONARY) continue;
+        if (bt == ORC_BC_OUTFLOW) {
+            int rc = orc_outflow_face(q, g, dir, side, depth);
+            if (rc) return rc;
+            continue;
+        }
         for (s = 0; s < 5; s++) {
             int sas;
             int veccomp = (s < 2) ? 0 : s - 1;

@@ -64,6 +64,8 @@ int orc_xj_predict_1d(double *q, const orc_grid *g, int dir, double lambda, cons
 /* Boundary conditions for direction dir on both faces. modes[6] = [-x,+x,-y,+y,-z,+z],
  * depth = halo depth (4). Follows setFluidBoundary (gpuclass/cudaStatics.cu:112-196). */
 int orc_set_fluid_boundary(double *q, const orc_grid *g, int dir, const int *modes, int depth);
+/* outflow ghosts of one face (side 0 = minus, 1 = plus) of all 5 slabs */
+int orc_outflow_face(double *q, const orc_grid *g, int dir, int side, int depth);
 /* the boundaries orc_fluidstep sets after sweeping dir: memory-x conventions
  * along dir (flux.cu:99-107 after the rotations of flux.cu:46-48) */
 int orc_fluid_boundary_swept(double *q, const orc_grid *g, int dir, const int *modes, int depth);
