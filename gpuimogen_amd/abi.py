@@ -146,6 +146,7 @@ SIGNATURES = {
     "GPU_ctrl_reset": (c_int, []),
     "GPU_ctrl_info": (c_int, [c_int, ctypes.c_char_p, c_int]),
     "performFluidUpdate_3D": (c_int, [_MGP, _P(ParallelTopology), FluidStepParams, _MGP]),
+    "performFluidUpdate_3D_noalloc": (c_int, [_MGP, _P(ParallelTopology), FluidStepParams]),
     "performFluidUpdate_1D": (c_int, [_MGP, FluidStepParams, _P(ParallelTopology), _MGP]),
     "setFluidBoundary": (c_int, [_MGP, _P(GeometryParams), c_int]),
     "setArrayBoundaryConditions": (c_int, [_MGP, _P(GeometryParams), c_int, c_int]),

@@ -216,6 +216,14 @@ int64_t sweep_waves(int nx, int ny, int nz, int dir)
     return (int64_t)g.blocksPerRow * g.nother * 4;
 }
 
+int64_t sweep_scratch_bytes(int nx, int ny, int nz, int dir)
+{
+    const int N = (dir == 1) ? nx : (dir == 2 ? ny : nz);
+    if (dir < 1 || dir > 3 || N < 3) return 0;
+    if (dir == 1) return x_geom(nx, (int64_t)ny * nz).blocks * 256 * 5 * (int64_t)sizeof(double);
+    return march_geom(nx, ny, nz, dir).nthr * 25 * (int64_t)sizeof(double);
+}
+
 int launch_fluid_sweep(double *base, int64_t pitch, int nx, int ny, int nz, int dir, int method, double lambda,
                        double gamma, double rhomin, double ciso, int limiter, hipStream_t s, const double *lamp,
                        const CflFuse *cfl)

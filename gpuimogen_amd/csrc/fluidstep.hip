@@ -348,6 +348,44 @@ int performFluidUpdate_3D(MGArray *fluid, ParallelTopology *parallelTopo, FluidS
     return imogen::update_3d(fluid, fsp, nullptr);
 }
 
+int performFluidUpdate_3D_noalloc(MGArray *fluid, ParallelTopology *parallelTopo, FluidStepParams fsp)
+{
+    int rc = check_topology(parallelTopo);
+    if (rc) return rc;
+    double *base;
+    int64_t pitch;
+    rc = fluid_view(fluid, &base, &pitch);
+    if (rc) return rc;
+    if (!is_xyz(fluid))
+        return IM_FAIL(ERROR_INVALID_ARGS,
+                       "the allocation-free fluid step takes XYZ-ordered arrays (a rotated canister is stepped "
+                       "on a transposed copy, which allocates)");
+    const int method = (int)fsp.stepMethod;
+    if (method != METHOD_HLLC && method != METHOD_HLL && method != METHOD_XINJIN)
+        return IM_FAIL(ERROR_INVALID_ARGS, "unknown fluid method %d", method);
+    // every sweep update_3d will launch must fit the fixed scratch array
+    const int *d3 = fluid->dim;
+    const bool line = d3[1] == 1 && d3[2] == 1;
+    int64_t need = 0;
+    for (int d = 1; d <= 3; d++) {
+        if (line ? d != 1 : d3[d - 1] <= 3) continue;
+        const int64_t b = (method == METHOD_XINJIN) ? xinjin_scratch_bytes(pitch, d3[0], d3[1], d3[2], d)
+                                                    : sweep_scratch_bytes(d3[0], d3[1], d3[2], d);
+        if (b > need) need = b;
+    }
+    if (need > (int64_t)kFixedScratchBytes)
+        return IM_FAIL(ERROR_NOMEM,
+                       "a %d x %d x %d fluid step needs %lld bytes of scratch; the allocation-free step has a fixed "
+                       "%zu (use performFluidUpdate_3D)",
+                       d3[0], d3[1], d3[2], (long long)need, kFixedScratchBytes);
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    st->fixedScratchOn = 1;
+    rc = imogen::update_3d(fluid, fsp, nullptr);
+    st->fixedScratchOn = 0;
+    return rc;
+}
+
 int calculateSoundspeed(MGArray *fluid, MGArray *mag, MGArray *cs, double gamma)
 {
     if (mag) return IM_FAIL(ERROR_NOIMPLEMENT, "MHD sound speed is not built");

@@ -32,6 +32,10 @@ enum DeviceErrorSlot {
     ERRW_SLOTS = 8
 };
 
+// Size of the fixed per-device scratch array that the allocation-free entry
+// points use in place of the growable scratch block.
+constexpr size_t kFixedScratchBytes = (size_t)64 << 20;
+
 // Per-device state: one stream all kernels go to, a growable scratch block,
 // the device addresses of the library's static device arrays, and small
 // pinned host buffers for scalar read-back.
@@ -40,6 +44,8 @@ struct DeviceState {
     hipStream_t stream = nullptr;
     void *scratch = nullptr;
     size_t scratchBytes = 0;
+    double *fixedScratch = nullptr;  // this device's copy of the fixed scratch array (kFixedScratchBytes)
+    int fixedScratchOn = 0;          // scratch_block hands out fixedScratch (never allocates) while set
     double *pinned = nullptr;   // 64 doubles of pinned host memory (pinned_buffers)
     double *devScalars = nullptr;  // 4096 doubles of device memory for reductions
     unsigned long long *err = nullptr;      // ERRW_SLOTS device counters, zero when no error is pending
@@ -66,7 +72,14 @@ void prof_end(int slot, hipStream_t s);
 // Nested suspension of the event timing (kept off while a graph is captured).
 void prof_suspend(int on);
 // Device scratch of at least `bytes`; contents are undefined on return.
+// While st->fixedScratchOn is set it returns the fixed array (null with an
+// ERROR_NOMEM message when bytes is above kFixedScratchBytes) and never
+// allocates.
 void *scratch_block(size_t bytes);
+// Scratch bytes launch_fluid_sweep / launch_xinjin_sweep take for one sweep
+// (the launch shapes of fluid_sweep.hip and xinjin.hip).
+int64_t sweep_scratch_bytes(int nx, int ny, int nz, int dir);
+int64_t xinjin_scratch_bytes(int64_t pitch, int nx, int ny, int nz, int dir);
 
 // Validates a 5-array fluid canister (slab views of one 5-slab owner, XYZ
 // order, one GPU) and returns its base pointer and slab pitch in doubles.

@@ -39,10 +39,12 @@ int fail(int status, const char *file, int line, const char *fmt, ...)
 }
 
 // Device memory every call on a GPU can use without allocating: the
-// reductions' scalars and the error counters (zero at load, as a static
-// array). Each GPU holds its own copy of these arrays from the library's first
-// call on it until the process ends; host code addresses them only through the
-// per-device pointers that hipGetSymbolAddress returns in state().
+// reductions' scalars, the error counters (zero at load, as a static array) and
+// the fixed scratch of the allocation-free entry points. Each GPU holds its own
+// copy of these arrays from the library's first call on it until the process
+// ends; host code addresses them only through the per-device pointers that
+// hipGetSymbolAddress returns in state().
+static __device__ __align__(256) double scratch[kFixedScratchBytes / sizeof(double)];
 static __device__ __align__(256) double g_devScalars[4096];
 static __device__ __align__(256) unsigned long long g_errWord[ERRW_SLOTS];
 
@@ -54,6 +56,7 @@ DeviceState *state()
     DeviceState *st = &g_state[dev];
     if (st->device < 0) {
         if (IM_HIP(hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking))) return nullptr;
+        if (IM_HIP(hipGetSymbolAddress((void **)&st->fixedScratch, HIP_SYMBOL(scratch)))) return nullptr;
         if (IM_HIP(hipGetSymbolAddress((void **)&st->devScalars, HIP_SYMBOL(g_devScalars)))) return nullptr;
         if (IM_HIP(hipGetSymbolAddress((void **)&st->err, HIP_SYMBOL(g_errWord)))) return nullptr;
         st->device = dev;
@@ -98,6 +101,12 @@ void *scratch_block(size_t bytes)
 {
     DeviceState *st = state();
     if (!st) return nullptr;
+    if (st->fixedScratchOn) {
+        if (bytes <= kFixedScratchBytes) return st->fixedScratch;
+        IM_FAIL(ERROR_NOMEM, "%zu bytes of scratch asked for inside an allocation-free call (fixed scratch: %zu)",
+                bytes, kFixedScratchBytes);
+        return nullptr;
+    }
     if (bytes <= st->scratchBytes) return st->scratch;
     if (st->scratch) {
         if (IM_HIP(hipStreamSynchronize(st->stream))) return nullptr;

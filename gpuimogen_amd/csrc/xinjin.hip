@@ -286,6 +286,14 @@ static int xj_stage(const double *src, const double *base, double *dst, int64_t 
     return IM_LAUNCHED("xj_stage_march");
 }
 
+int64_t xinjin_scratch_bytes(int64_t pitch, int nx, int ny, int nz, int dir)
+{
+    const int N = (dir == 1) ? nx : (dir == 2 ? ny : nz);
+    if (dir < 1 || dir > 3 || N < 3) return 0;
+    const int64_t nlines = (dir == 1) ? (int64_t)ny * nz : (dir == 2 ? (int64_t)nx * nz : (int64_t)nx * ny);
+    return (5 * pitch + nlines + 32) * (int64_t)sizeof(double);
+}
+
 int launch_xinjin_sweep(double *base, int64_t pitch, int nx, int ny, int nz, int dir, double lambda,
                         double gamma, double rhomin, hipStream_t s, const double *lamp)
 {
@@ -296,7 +304,7 @@ int launch_xinjin_sweep(double *base, int64_t pitch, int nx, int ny, int nz, int
     XjConsts k{gamma, gamma - 1.0, rhomin, 1e-5 / (gamma - 1.0)};
     const int64_t nlines = (dir == 1) ? (int64_t)ny * nz : (dir == 2 ? (int64_t)nx * nz : (int64_t)nx * ny);
     // temporary: 5 slabs of U at the fluid's pitch, then the per-line freezing speeds
-    double *U = (double *)scratch_block((size_t)(5 * pitch + nlines + 32) * sizeof(double));
+    double *U = (double *)scratch_block((size_t)xinjin_scratch_bytes(pitch, nx, ny, nz, dir));
     if (!U) return IM_FAIL(ERROR_NOMEM, "no scratch for the Xin-Jin stages");
     double *cf = U + 5 * pitch;
     const int slot = prof_begin(PROF_SWEEP_X + dir - 1, s);

@@ -250,6 +250,21 @@ int GPU_ctrl_info(int device, char *text, int len);
 int performFluidUpdate_3D(MGArray *fluid, ParallelTopology *parallelTopo, FluidStepParams fsp,
                           MGArray *tempStorage);
 
+/* No counterpart in the reference: performFluidUpdate_3D for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc). It takes its sweeps' scratch from a fixed
+ * 64 MiB device array that each GPU the library runs on holds from the
+ * library's first call until the process ends, and calls no hipMalloc/hipFree.
+ * Limit: every sweep's scratch must fit 64 MiB (HLLC/HLL: 40 B per X-sweep
+ * thread and 200 B per Y/Z marching lane, which fits every cube up to 256^3,
+ * e.g. 29.5 MB at 192^3; Xin-Jin: 5 slabs, up to about 1.6 M cells); larger
+ * steps return ERROR_NOMEM before any launch. XYZ-ordered canisters only (ERROR_INVALID_ARGS for rotated ones).
+ * All calls on a GPU share the array: they are ordered by the library's
+ * stream; two executions that overlap on one GPU (e.g. a captured graph of
+ * this call replayed on another stream while the library's stream runs one)
+ * race. Results are bitwise those of performFluidUpdate_3D. */
+int performFluidUpdate_3D_noalloc(MGArray *fluid, ParallelTopology *parallelTopo, FluidStepParams fsp);
+
 /* gpuclass/cudaFluidStep.h:56-58, cudaFluidStep.cu:233-663: one directional
  * predictor+corrector along params.stepDirection (1/2/3 = X/Y/Z). */
 int performFluidUpdate_1D(MGArray *fluid, FluidStepParams params, ParallelTopology *topo,
