@@ -3,11 +3,13 @@
 // FP32 kernels that train the network on the GPU.
 //
 // Network 5 -> 64 -> 64 -> 3 with GELU (tanh form). Inference keeps the cells
-// of a wave on the N side of v_mfma_f32_16x16x16_bf16 (16 cells per tile) and
-// the hidden units on M/K, so a layer's D tile is, lane for lane, the B
-// operand of the next layer: no data moves between layers, the FP32
-// accumulators only pass through GELU and a BF16 rounding. Biases enter as the
-// accumulator's initial value.
+// of a wave on the N side of the BF16 MFMAs (16 cells per tile; K = 16 for
+// layer 1, gfx950's K = 32 for layers 2 and 3) and the hidden units on M/K,
+// so a layer's D tiles are, lane for lane, the B operand of the next layer: no
+// data moves between layers, the FP32 accumulators only pass through GELU and
+// a BF16 rounding. Biases enter as the accumulator's initial value. Weights
+// and biases are read from an LDS copy each block makes once; rejected cells
+// reach the fallback queue through per-wave LDS batches.
 //
 // Per cell the surrogate replaces the adaptive integrator of the relaxation
 // problem (drag_device.h). With k0 = k(|dv_0|, u_0), tau = dt k0 and the
@@ -33,6 +35,8 @@ namespace imogen {
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bfv4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x8s __attribute__((ext_vector_type(8)));
+typedef __bf16 bfv8 __attribute__((ext_vector_type(8)));
 
 constexpr int kIn = 5, kHid = 64, kOut = 3;
 constexpr int kOffW1 = 0, kOffB1 = kIn * kHid, kOffW2 = kOffB1 + kHid, kOffB2 = kOffW2 + kHid * kHid,
@@ -77,6 +81,16 @@ __device__ __forceinline__ float gelu_grad(float x)
     const float a = 0.7978845608028654f * (x + 0.044715f * x * x * x);
     const float t = 1.0f - 2.0f / (__expf(2.0f * a) + 1.0f);
     return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * 0.7978845608028654f * (1.0f + 3.0f * 0.044715f * x * x);
+}
+
+// Eight BF16 K-values of a 16x16x32 MFMA operand from two 16x16x16 ones:
+// elements 0..3 from the lower K-tile, 4..7 from the upper. A and B operands
+// are built alike, so each product pairs the same hidden unit whatever K
+// index the hardware gives each element.
+__device__ __forceinline__ bfv8 cat8(const bf16x4 &a, const bf16x4 &b)
+{
+    const bf16x8s v = {a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]};
+    return __builtin_bit_cast(bfv8, v);
 }
 
 __device__ __forceinline__ bf16x4 load_frag(const unsigned short *w, int ld, int row, int col)
@@ -167,7 +181,9 @@ __device__ __forceinline__ void mlp_wave(const MlpLds &L, const float (&f)[5], f
                                                            0);
             h1[mt] = act_to_b(acc);
         }
-        // layer 2: the four output tiles' accumulators advance together, so
+        // layer 2 with gfx950's K = 32 MFMA: hidden tiles 2b and 2b+1 form one
+        // K block (B operand: this lane's two D fragments of layer 1, as they
+        // are); the four output tiles' accumulators advance together, so
         // consecutive MFMAs never wait on each other's results
         f32x4 acc2[4];
 #pragma unroll
@@ -176,20 +192,25 @@ __device__ __forceinline__ void mlp_wave(const MlpLds &L, const float (&f)[5], f
             acc2[mt] = f32x4{b2[u], b2[u + 1], b2[u + 2], b2[u + 3]};
         }
 #pragma unroll
-        for (int kt = 0; kt < 4; kt++)
+        for (int b = 0; b < 2; b++) {
+            const bfv8 hb = cat8(h1[2 * b], h1[2 * b + 1]);
 #pragma unroll
             for (int mt = 0; mt < 4; mt++)
-                acc2[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
-                    load_frag(w2t, 64, 16 * mt + col, 16 * kt + 4 * q), h1[kt], acc2[mt], 0, 0, 0);
+                acc2[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    cat8(load_frag(w2t, 64, 16 * mt + col, 32 * b + 4 * q),
+                         load_frag(w2t, 64, 16 * mt + col, 32 * b + 16 + 4 * q)),
+                    hb, acc2[mt], 0, 0, 0);
+        }
         bf16x4 h2[4];
 #pragma unroll
         for (int mt = 0; mt < 4; mt++) h2[mt] = act_to_b(acc2[mt]);
         // layer 3: one tile, outputs 0..2 in rows 0..2 (lanes 0..15)
         f32x4 acc = {b3[4 * q], b3[4 * q + 1], b3[4 * q + 2], b3[4 * q + 3]};
 #pragma unroll
-        for (int kt = 0; kt < 4; kt++)
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(load_frag(w3t, 64, col, 16 * kt + 4 * q), h2[kt], acc,
-                                                           0, 0, 0);
+        for (int b = 0; b < 2; b++)
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                cat8(load_frag(w3t, 64, col, 32 * b + 4 * q), load_frag(w3t, 64, col, 32 * b + 16 + 4 * q)),
+                cat8(h2[2 * b], h2[2 * b + 1]), acc, 0, 0, 0);
         res[g][0] = acc[0];
         res[g][1] = acc[1];
         res[g][2] = acc[2];
@@ -433,6 +454,43 @@ __device__ __forceinline__ void exact_call(double (&G)[5], double (&D)[5], const
     drag_apply(G, D, 1, s, fac);
 }
 
+// Rejected cells are collected per wave in LDS and appended to the global
+// fallback queue in batches: one atomic per batch of up to kWaveQueue entries
+// instead of one per wave of cells (a single counter that every wave hits
+// serialises the waves on the atomic's round trip).
+constexpr int kWaveQueue = 448;
+
+struct WaveQueue {
+    int64_t *buf;  // this wave's kWaveQueue LDS entries
+    int count;     // entries held (wave-uniform)
+};
+
+__device__ __forceinline__ void wq_flush(WaveQueue &w, int64_t *__restrict__ queue,
+                                         unsigned long long *__restrict__ qcount)
+{
+    if (w.count == 0) return;
+    const int lane = threadIdx.x & 63;
+    unsigned long long slot = 0;
+    if (lane == 0) slot = atomicAdd(qcount, (unsigned long long)w.count);
+    slot = __shfl(slot, 0, 64);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // other lanes' LDS entries
+    for (int k = lane; k < w.count; k += 64) queue[slot + k] = w.buf[k];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // read before the buffer is reused
+    w.count = 0;
+}
+
+__device__ __forceinline__ void wq_push(WaveQueue &w, bool reject, int64_t tag, int64_t *__restrict__ queue,
+                                        unsigned long long *__restrict__ qcount)
+{
+    const unsigned long long mask = __ballot(reject);
+    if (!mask) return;
+    const int n = __popcll(mask);
+    if (w.count + n > kWaveQueue) wq_flush(w, queue, qcount);
+    const int lane = threadIdx.x & 63;
+    if (reject) w.buf[w.count + __popcll(mask & ((1ull << lane) - 1ull))] = tag;
+    w.count += n;
+}
+
 // One wave of 64 cells [base, base+64) of an n-cell range: `calls` (1 or 2)
 // learned drag calls of dt each, in registers. A cell that one call does not
 // accept is stored as the accepted calls left it and queued (one atomic per
@@ -442,8 +500,9 @@ __device__ __forceinline__ void exact_call(double (&G)[5], double (&D)[5], const
 // check), so both forms give the same result.
 __device__ __forceinline__ void learned_cells(double *__restrict__ g, double *__restrict__ d, int64_t pitch,
                                               int64_t base, int64_t n, const DragConsts &c, double dt,
-                                              const MlpDev &m, const MlpLds &Lm, int64_t *__restrict__ queue,
-                                              unsigned long long *__restrict__ qcount, int64_t qtag, int calls)
+                                              const MlpDev &m, const MlpLds &Lm, WaveQueue &wq,
+                                              int64_t *__restrict__ queue, unsigned long long *__restrict__ qcount,
+                                              int64_t qtag, int calls)
 {
     const int lane = threadIdx.x & 63;
     const int64_t i = base + lane;
@@ -478,15 +537,7 @@ __device__ __forceinline__ void learned_cells(double *__restrict__ g, double *__
         }
     }
     const bool reject = live && rejected;
-    const unsigned long long mask = __ballot(reject);
-    if (mask) {
-        const int leader = __ffsll((long long)mask) - 1;
-        unsigned long long slot = 0;
-        if (lane == leader) slot = atomicAdd(qcount, (unsigned long long)__popcll(mask));
-        slot = __shfl(slot, leader, 64);
-        const int64_t tag = qtag | i | ((done == 0 && calls == 2) ? kQueueSecond : 0);
-        if (reject) queue[slot + __popcll(mask & ((1ull << lane) - 1ull))] = tag;
-    }
+    wq_push(wq, reject, qtag | i | ((done == 0 && calls == 2) ? kQueueSecond : 0), queue, qcount);
     if (live && done > 0) cell_out(g + i, d + i, pitch, G, D);
 }
 
@@ -530,18 +581,21 @@ __device__ __forceinline__ unsigned fallback_cell(double *gi, double *di, int64_
     return extra;
 }
 
-__global__ __launch_bounds__(256) void learned_drag_kernel(double *__restrict__ g, double *__restrict__ d,
+__global__ __launch_bounds__(256, 2) void learned_drag_kernel(double *__restrict__ g, double *__restrict__ d,
                                                            int64_t pitch, int64_t n, DragConsts c, DragDt dtv,
                                                            MlpDev m, int calls, int64_t *__restrict__ queue,
                                                            unsigned long long *__restrict__ qcount)
 {
     __shared__ MlpLds Lm;
+    __shared__ int64_t qbuf[4][kWaveQueue];
     stage_model(Lm, m);
+    WaveQueue wq{qbuf[threadIdx.x >> 6], 0};
     const double dt = drag_dt(dtv);
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
     for (int64_t base = wave * 64; base < n; base += nwaves * 64)
-        learned_cells(g, d, pitch, base, n, c, dt, m, Lm, queue, qcount, 0, calls);
+        learned_cells(g, d, pitch, base, n, c, dt, m, Lm, wq, queue, qcount, 0, calls);
+    wq_flush(wq, queue, qcount);
 }
 
 __global__ __launch_bounds__(256) void fallback_drag_kernel(double *__restrict__ g, double *__restrict__ d,
@@ -597,7 +651,9 @@ __global__ __launch_bounds__(256, 2) void ens_drag_kernel(const EnsDragTable *__
                                                           unsigned long long *__restrict__ err)
 {
     __shared__ MlpLds Lm;
+    __shared__ int64_t qbuf[LEARNED ? 4 : 1][LEARNED ? kWaveQueue : 1];
     if (LEARNED) stage_model(Lm, m);
+    WaveQueue wq{qbuf[LEARNED ? threadIdx.x >> 6 : 0], 0};
     const int wave0 = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const int nwaves = (int)(((int64_t)gridDim.x * blockDim.x) >> 6);
     unsigned invalid = 0, overrun = 0;
@@ -606,7 +662,8 @@ __global__ __launch_bounds__(256, 2) void ens_drag_kernel(const EnsDragTable *__
         const EnsDragTable T = tab[it.t];
         const double dt = frac * dtp[T.p];
         if (LEARNED) {
-            learned_cells(T.g, T.d, T.pitch, it.base, T.n, c, dt, m, Lm, queue, qcount, (int64_t)it.t << 32, calls);
+            learned_cells(T.g, T.d, T.pitch, it.base, T.n, c, dt, m, Lm, wq, queue, qcount, (int64_t)it.t << 32,
+                          calls);
             continue;
         }
         const int64_t i = it.base + (threadIdx.x & 63);
@@ -629,6 +686,7 @@ __global__ __launch_bounds__(256, 2) void ens_drag_kernel(const EnsDragTable *__
         }
         cell_out(T.g + i, T.d + i, T.pitch, G, D);
     }
+    if (LEARNED) wq_flush(wq, queue, qcount);
     wave_count(err + ERRW_DRAG_INVALID, invalid);
     wave_count(err + ERRW_DRAG_BUDGET, overrun);
 }
