@@ -491,23 +491,20 @@ __device__ __forceinline__ void wq_push(WaveQueue &w, bool reject, int64_t tag, 
     w.count += n;
 }
 
-// One wave of 64 cells [base, base+64) of an n-cell range: `calls` (1 or 2)
-// learned drag calls of dt each, in registers. A cell that one call does not
-// accept is stored as the accepted calls left it and queued (one atomic per
-// wave): its remaining calls are taken by the fallback kernel exactly as two
-// separate launches would take them (exact integrator for the rejected call,
-// then, if the first was rejected, the learned second call with its own
-// check), so both forms give the same result.
+// One wave of learned drag calls: lane cells i (live lanes; the others work
+// on cell ic and store nothing). `calls` (1 or 2) calls of dt each are taken
+// in registers. A cell that one call does not accept is stored as the
+// accepted calls left it and queued: the fallback kernel integrates the
+// rejected call exactly and, when the cell's first call was the rejected one
+// (kQueueSecond), lists the cell for a second pass of this kernel (list
+// mode) that takes its second call, so every call of every cell runs the same
+// instructions as in two separate launches and gives the same result.
 __device__ __forceinline__ void learned_cells(double *__restrict__ g, double *__restrict__ d, int64_t pitch,
-                                              int64_t base, int64_t n, const DragConsts &c, double dt,
+                                              int64_t i, int64_t ic, bool live, const DragConsts &c, double dt,
                                               const MlpDev &m, const MlpLds &Lm, WaveQueue &wq,
                                               int64_t *__restrict__ queue, unsigned long long *__restrict__ qcount,
                                               int64_t qtag, int calls)
 {
-    const int lane = threadIdx.x & 63;
-    const int64_t i = base + lane;
-    const bool live = i < n;
-    const int64_t ic = live ? i : n - 1;  // lanes past the end work on a valid cell and store nothing
     double G[5], D[5];
     cell_in(g + ic, d + ic, pitch, G, D);
     int done = 0;  // calls accepted before the first rejection
@@ -541,80 +538,83 @@ __device__ __forceinline__ void learned_cells(double *__restrict__ g, double *__
     if (live && done > 0) cell_out(g + i, d + i, pitch, G, D);
 }
 
-// The fallback of one wave of queue entries (cell pointers gi/di per lane,
-// dt per lane): the exact integrator for the rejected call, then, for
-// entries marked kQueueSecond, the learned second call and, where that is
-// rejected too, the exact integrator again. Must be called by all 64 lanes
-// together (live = the lane holds an entry). Returns the number of second
-// calls this lane sent to the exact integrator.
-__device__ __forceinline__ unsigned fallback_cell(double *gi, double *di, int64_t pitch, bool live, bool second,
-                                                  const DragConsts &c, double dt, const MlpDev &m,
-                                                  const MlpLds &Lm, bool staged, unsigned &invalid,
-                                                  unsigned &overrun)
+// The exact call of one queued cell; returns whether the cell is listed for
+// a learned second call.
+__device__ __forceinline__ void fallback_cell(double *gi, double *di, int64_t pitch, bool live, const DragConsts &c,
+                                              double dt, unsigned &invalid, unsigned &overrun)
 {
     double G[5], D[5];
     cell_in(gi, di, pitch, G, D);
     exact_call(G, D, c, dt, live, invalid, overrun);
-    unsigned extra = 0;
-    if (staged && __ballot(second)) {  // wave-uniform: the network runs on the whole wave
-        double G2[5], D2[5];
-#pragma unroll
-        for (int k = 0; k < 5; k++) {
-            G2[k] = G[k];
-            D2[k] = D[k];
-        }
-        const bool ok = learned_call(G2, D2, c, dt, m, Lm);
-        if (second) {
-            if (ok) {
-#pragma unroll
-                for (int k = 0; k < 5; k++) {
-                    G[k] = G2[k];
-                    D[k] = D2[k];
-                }
-            } else {
-                exact_call(G, D, c, dt, live, invalid, overrun);
-                extra = 1;
-            }
-        }
-    }
     if (live) cell_out(gi, di, pitch, G, D);
-    return extra;
 }
 
+// Appends the live lanes' entries to a global list (one atomic per wave).
+__device__ __forceinline__ void wave_append(bool put, int64_t v, int64_t *__restrict__ list,
+                                            unsigned long long *__restrict__ count)
+{
+    const unsigned long long mask = __ballot(put);
+    if (!mask) return;
+    const int lane = threadIdx.x & 63;
+    unsigned long long slot = 0;
+    if (lane == 0) slot = atomicAdd(count, (unsigned long long)__popcll(mask));
+    slot = __shfl(slot, 0, 64);
+    if (put) list[slot + __popcll(mask & ((1ull << lane) - 1ull))] = v;
+}
+
+// Dense mode (list null): cells [0, n). List mode: the *listCount cells of
+// list (every entry valid).
 __global__ __launch_bounds__(256, 2) void learned_drag_kernel(double *__restrict__ g, double *__restrict__ d,
-                                                           int64_t pitch, int64_t n, DragConsts c, DragDt dtv,
-                                                           MlpDev m, int calls, int64_t *__restrict__ queue,
-                                                           unsigned long long *__restrict__ qcount)
+                                                              int64_t pitch, int64_t n, DragConsts c, DragDt dtv,
+                                                              MlpDev m, int calls, int64_t *__restrict__ queue,
+                                                              unsigned long long *__restrict__ qcount,
+                                                              const int64_t *__restrict__ list,
+                                                              const unsigned long long *__restrict__ listCount)
 {
     __shared__ MlpLds Lm;
     __shared__ int64_t qbuf[4][kWaveQueue];
     stage_model(Lm, m);
     WaveQueue wq{qbuf[threadIdx.x >> 6], 0};
     const double dt = drag_dt(dtv);
+    const int lane = threadIdx.x & 63;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-    for (int64_t base = wave * 64; base < n; base += nwaves * 64)
-        learned_cells(g, d, pitch, base, n, c, dt, m, Lm, wq, queue, qcount, 0, calls);
+    const int64_t total = list ? (int64_t)*listCount : n;
+    for (int64_t base = wave * 64; base < total; base += nwaves * 64) {
+        const int64_t t = base + lane;
+        const bool live = t < total;
+        int64_t i, ic;
+        if (list) {
+            i = list[live ? t : base];
+            ic = i;
+        } else {
+            i = t;
+            ic = live ? t : n - 1;
+        }
+        learned_cells(g, d, pitch, i, ic, live, c, dt, m, Lm, wq, queue, qcount, 0, calls);
+    }
     wq_flush(wq, queue, qcount);
 }
 
+// Exact integration of the queued cells in dense waves. cells: the calls'
+// cell count to add to counts[0]; entries marked kQueueSecond are appended to
+// listOut for the learned second call.
 __global__ __launch_bounds__(256) void fallback_drag_kernel(double *__restrict__ g, double *__restrict__ d,
-                                                            int64_t pitch, int64_t n, DragConsts c, DragDt dtv,
-                                                            MlpDev m, int calls, const int64_t *__restrict__ queue,
+                                                            int64_t pitch, int64_t cells, DragConsts c, DragDt dtv,
+                                                            const int64_t *__restrict__ queue,
                                                             const unsigned long long *__restrict__ qcount,
                                                             unsigned long long *__restrict__ counts,
-                                                            unsigned long long *__restrict__ err)
+                                                            unsigned long long *__restrict__ err,
+                                                            int64_t *__restrict__ listOut,
+                                                            unsigned long long *__restrict__ listCount)
 {
-    __shared__ MlpLds Lm;
-    const bool staged = calls == 2;  // uniform: only a two-call pass owes learned second calls
-    if (staged) stage_model(Lm, m);
     const double dt = drag_dt(dtv);
     const unsigned long long nq = *qcount;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        atomicAdd(counts, (unsigned long long)n * (unsigned long long)calls);
+        if (cells) atomicAdd(counts, (unsigned long long)cells);
         if (nq) atomicAdd(counts + 1, nq);
     }
-    unsigned invalid = 0, overrun = 0, extra = 0;
+    unsigned invalid = 0, overrun = 0;
     const int lane = threadIdx.x & 63;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
@@ -623,17 +623,16 @@ __global__ __launch_bounds__(256) void fallback_drag_kernel(double *__restrict__
         const bool live = t < nq;
         const int64_t e = queue[live ? t : base];
         const int64_t i = e & ~kQueueSecond;
-        const bool second = live && (e & kQueueSecond);
-        extra += fallback_cell(g + i, d + i, pitch, live, second, c, dt, m, Lm, staged, invalid, overrun);
+        fallback_cell(g + i, d + i, pitch, live, c, dt, invalid, overrun);
+        if (listOut) wave_append(live && (e & kQueueSecond), i, listOut, listCount);
     }
     wave_count(err + ERRW_DRAG_INVALID, invalid);
     wave_count(err + ERRW_DRAG_BUDGET, overrun);
-    wave_count(counts + 1, extra);
 }
 
-__global__ void zero_counter(unsigned long long *p)
+__global__ void zero_counters(unsigned long long *p, int n)
 {
-    *p = 0;
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -662,8 +661,10 @@ __global__ __launch_bounds__(256, 2) void ens_drag_kernel(const EnsDragTable *__
         const EnsDragTable T = tab[it.t];
         const double dt = frac * dtp[T.p];
         if (LEARNED) {
-            learned_cells(T.g, T.d, T.pitch, it.base, T.n, c, dt, m, Lm, wq, queue, qcount, (int64_t)it.t << 32,
-                          calls);
+            const int64_t i = it.base + (threadIdx.x & 63);
+            const bool live = i < T.n;
+            learned_cells(T.g, T.d, T.pitch, i, live ? i : T.n - 1, live, c, dt, m, Lm, wq, queue, qcount,
+                          (int64_t)it.t << 32, calls);
             continue;
         }
         const int64_t i = it.base + (threadIdx.x & 63);
@@ -693,20 +694,17 @@ __global__ __launch_bounds__(256, 2) void ens_drag_kernel(const EnsDragTable *__
 
 __global__ __launch_bounds__(256) void ens_fallback_kernel(const EnsDragTable *__restrict__ tab,
                                                            const double *__restrict__ dtp, double frac, DragConsts c,
-                                                           MlpDev m, int calls, const int64_t *__restrict__ queue,
+                                                           const int64_t *__restrict__ queue,
                                                            const unsigned long long *__restrict__ qcount,
-                                                           int64_t ncells, unsigned long long *__restrict__ counts,
+                                                           int64_t cells, unsigned long long *__restrict__ counts,
                                                            unsigned long long *__restrict__ err)
 {
-    __shared__ MlpLds Lm;
-    const bool staged = calls == 2;
-    if (staged) stage_model(Lm, m);
     const unsigned long long nq = *qcount;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        atomicAdd(counts, (unsigned long long)ncells * (unsigned long long)calls);
+        atomicAdd(counts, (unsigned long long)cells);
         if (nq) atomicAdd(counts + 1, nq);
     }
-    unsigned invalid = 0, overrun = 0, extra = 0;
+    unsigned invalid = 0, overrun = 0;
     const int lane = threadIdx.x & 63;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
@@ -716,13 +714,10 @@ __global__ __launch_bounds__(256) void ens_fallback_kernel(const EnsDragTable *_
         const int64_t e = queue[live ? t : base];
         const EnsDragTable T = tab[(int)((e & ~kQueueSecond) >> 32)];
         const int64_t i = e & kQueueIndex;
-        const bool second = live && (e & kQueueSecond);
-        extra += fallback_cell(T.g + i, T.d + i, T.pitch, live, second, c, frac * dtp[T.p], m, Lm, staged, invalid,
-                               overrun);
+        fallback_cell(T.g + i, T.d + i, T.pitch, live, c, frac * dtp[T.p], invalid, overrun);
     }
     wave_count(err + ERRW_DRAG_INVALID, invalid);
     wave_count(err + ERRW_DRAG_BUDGET, overrun);
-    wave_count(counts + 1, extra);
 }
 
 // standalone BF16 inference (tests)
@@ -1010,36 +1005,52 @@ static int ensure_counts()
     if (g_model.counts) return SUCCESSFUL;
     DeviceState *st = state();
     if (!st) return ERROR_CUDA_BLEW_UP;
-    if (IM_HIP(hipMalloc(&g_model.counts, 4 * sizeof(unsigned long long)))) return ERROR_CUDA_BLEW_UP;
-    if (IM_HIP(hipMemsetAsync(g_model.counts, 0, 4 * sizeof(unsigned long long), st->stream)))
+    if (IM_HIP(hipMalloc(&g_model.counts, 8 * sizeof(unsigned long long)))) return ERROR_CUDA_BLEW_UP;
+    if (IM_HIP(hipMemsetAsync(g_model.counts, 0, 8 * sizeof(unsigned long long), st->stream)))
         return ERROR_CUDA_BLEW_UP;
     return IM_HIP(hipStreamSynchronize(st->stream));
 }
 
+// counts: [0] cells, [1] fallbacks (imogen_drag_learned_stats), [2] queue
+// length, [3] second-call list length, [4] the second pass's queue length.
+// Two calls: learned (both calls) -> fallback (exact; first-call rejections
+// listed) -> learned over the list (second call) -> fallback.
 int launch_learned_drag(double *g, double *d, int64_t pitch, int64_t n, const DragConsts &c, DragDt dt,
                         int64_t *queue, hipStream_t s, int calls)
 {
     if (calls < 1 || calls > 2) return IM_FAIL(ERROR_INVALID_ARGS, "drag calls per pass: %d", calls);
     if (!g_model.loaded) return IM_FAIL(ERROR_INVALID_ARGS, "learned drag: no model (imogen_drag_set_model)");
     if (ensure_counts()) return ERROR_CUDA_BLEW_UP;
+    if (queue && calls == 2) return IM_FAIL(ERROR_INVALID_ARGS, "a caller's queue serves one call per pass");
     int64_t waves = (n + 63) / 64;
     if (waves > 8192) waves = 8192;
     const int64_t blocks = (waves + 3) / 4;
-    if (!queue) queue = (int64_t *)scratch_block((size_t)n * sizeof(int64_t));
+    if (!queue) queue = (int64_t *)scratch_block((size_t)n * calls * sizeof(int64_t));
     if (!queue) return IM_FAIL(ERROR_NOMEM, "no scratch for the drag fallback queue");
-    unsigned long long *qcount = g_model.counts + 2;
+    int64_t *list = calls == 2 ? queue + n : nullptr;
+    unsigned long long *cnt = g_model.counts;
+    unsigned long long *err = state()->err;
     const int slot = prof_begin(PROF_MLP, s);
-    hipLaunchKernelGGL(zero_counter, dim3(1), dim3(1), 0, s, qcount);
-    int rc = IM_LAUNCHED("zero_counter");
+    hipLaunchKernelGGL(zero_counters, dim3(1), dim3(64), 0, s, cnt + 2, 3);
+    int rc = IM_LAUNCHED("zero_counters");
     if (rc) return rc;
     hipLaunchKernelGGL(learned_drag_kernel, dim3((unsigned)blocks), dim3(256), 0, s, g, d, pitch, n, c, dt,
-                       g_model.dev, calls, queue, qcount);
-    rc = IM_LAUNCHED("learned_drag_kernel");
-    if (rc) return rc;
-    hipLaunchKernelGGL(fallback_drag_kernel, dim3(1024), dim3(256), 0, s, g, d, pitch, n, c, dt, g_model.dev, calls,
-                       queue, qcount, g_model.counts, state()->err);
+                       g_model.dev, calls, queue, cnt + 2, (const int64_t *)nullptr,
+                       (const unsigned long long *)nullptr);
+    if ((rc = IM_LAUNCHED("learned_drag_kernel"))) return rc;
+    hipLaunchKernelGGL(fallback_drag_kernel, dim3(1024), dim3(256), 0, s, g, d, pitch, n * calls, c, dt, queue,
+                       cnt + 2, cnt, err, list, cnt + 3);
+    if ((rc = IM_LAUNCHED("fallback_drag_kernel"))) return rc;
+    if (calls == 2) {
+        hipLaunchKernelGGL(learned_drag_kernel, dim3(256), dim3(256), 0, s, g, d, pitch, n, c, dt, g_model.dev, 1,
+                           queue, cnt + 4, (const int64_t *)list, (const unsigned long long *)(cnt + 3));
+        if ((rc = IM_LAUNCHED("learned_drag_kernel"))) return rc;
+        hipLaunchKernelGGL(fallback_drag_kernel, dim3(1024), dim3(256), 0, s, g, d, pitch, (int64_t)0, c, dt, queue,
+                           cnt + 4, cnt, err, (int64_t *)nullptr, (unsigned long long *)nullptr);
+        if ((rc = IM_LAUNCHED("fallback_drag_kernel"))) return rc;
+    }
     prof_end(slot, s);
-    return IM_LAUNCHED("fallback_drag_kernel");
+    return SUCCESSFUL;
 }
 
 int launch_ens_drag(const EnsDragTable *tab, const EnsDragItem *items, int nitems, int64_t ncells,
@@ -1051,10 +1062,12 @@ int launch_ens_drag(const EnsDragTable *tab, const EnsDragItem *items, int nitem
         return IM_FAIL(ERROR_NOIMPLEMENT, "ensemble drag method %d", method);
     if (method == DRAG_LEARNED && !g_model.loaded)
         return IM_FAIL(ERROR_INVALID_ARGS, "learned drag: no model (imogen_drag_set_model)");
+    if (method == DRAG_LEARNED && calls != 1)
+        return IM_FAIL(ERROR_INVALID_ARGS, "ensemble learned drag takes one call per launch");
     if (nitems <= 0) return SUCCESSFUL;
     if (method == DRAG_LEARNED) {
-        hipLaunchKernelGGL(zero_counter, dim3(1), dim3(1), 0, s, qcount);
-        int rc = IM_LAUNCHED("zero_counter");
+        hipLaunchKernelGGL(zero_counters, dim3(1), dim3(64), 0, s, qcount, 1);
+        int rc = IM_LAUNCHED("zero_counters");
         if (rc) return rc;
     }
     unsigned long long *err = state()->err;
@@ -1069,8 +1082,8 @@ int launch_ens_drag(const EnsDragTable *tab, const EnsDragItem *items, int nitem
     int rc = IM_LAUNCHED("ens_drag_kernel");
     if (rc || method != DRAG_LEARNED) return rc;
     if (!g_model.counts) return IM_FAIL(ERROR_INVALID_ARGS, "learned drag counters not prepared (learned_drag_prepare)");
-    hipLaunchKernelGGL(ens_fallback_kernel, dim3(1024), dim3(256), 0, s, tab, dtp, frac, c, g_model.dev, calls, queue,
-                       qcount, ncells, g_model.counts, err);
+    hipLaunchKernelGGL(ens_fallback_kernel, dim3(1024), dim3(256), 0, s, tab, dtp, frac, c, queue, qcount,
+                       ncells * calls, g_model.counts, err);
     return IM_LAUNCHED("ens_fallback_kernel");
 }
 

@@ -151,12 +151,9 @@ static int source(TimeloopImpl *L, double fraction, hipStream_t s)
     if (rc) return rc;
     const int64_t n = (int64_t)L->fluid[0][0].dim[0] * L->fluid[0][0].dim[1] * L->fluid[0][0].dim[2];
     const DragDt dt{0.0, L->dbuf + 1, fraction / 2};
-    // the learned method takes its two calls as two launches (a cell its
-    // first call rejects must meet the second call's network as a separate
-    // launch would)
-    for (int call = 0; call < (L->dragMethod == DRAG_LEARNED ? 2 : 1); call++) {
+    {
         const int slot = prof_begin(PROF_DRAG, s);
-        rc = (L->dragMethod == DRAG_LEARNED) ? launch_learned_drag(g, d, pg, n, L->dc, dt, nullptr, s, 1)
+        rc = (L->dragMethod == DRAG_LEARNED) ? launch_learned_drag(g, d, pg, n, L->dc, dt, nullptr, s, 2)
                                              : launch_drag(g, d, pg, n, L->dc, dt, L->dragMethod, s, 2);
         prof_end(slot, s);
         if (rc) return rc;
