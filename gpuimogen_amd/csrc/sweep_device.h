@@ -144,9 +144,13 @@ __device__ __forceinline__ double slope_limit(double a, double b, int rt)
     const int which = (LIM >= 0) ? LIM : rt;
     switch (which) {
     case 0: {  // Ospre: .75 A B (A+B) / (A^2 + AB + B^2)
+        // With A B > 0 the denominator falls below the normal range only for
+        // |A|, |B| < 1.5e-154, where the numerator has already underflowed to
+        // zero and the quotient is +-0; the hardware reciprocal of such a
+        // denominator is infinite, so it is raised to the smallest normal.
         double r = a * b;
         double s = a + b;
-        double v = div_fast(.75 * r * s, s * s - r);
+        double v = div_fast(.75 * r * s, fmax(s * s - r, 0x1p-1022));
         return (r > 0) ? v : 0.0;
     }
     case 1: {  // van Leer: A B / (A+B)

@@ -25,8 +25,11 @@ struct XjConsts {
 // Ospre flux limiter, gpuclass/limiter_functions.h:36-42
 __device__ __forceinline__ double xj_ospre(double a, double b)
 {
+    // a^2 + ab + b^2 below the normal range (|a|, |b| < 1.5e-154, numerator
+    // already +-0) is raised to the smallest normal: its hardware reciprocal
+    // would be infinite and the quotient NaN instead of +-0
     const double r = a * b;
-    const double v = div_nr(1.5 * r * (a + b), a * a + r + b * b);
+    const double v = div_nr(1.5 * r * (a + b), fmax(a * a + r + b * b, 0x1p-1022));
     return (r <= 0.0) ? 0.0 : v;
 }
 
