@@ -98,7 +98,9 @@ def run_device_loop(args, q0, h):
     f = gc.FluidManager(q0, 5.0 / 3.0, minmass=1e-8, bc_modes=(abi.extrapConstant,) * 6)
     gm = gc.GeometryManager((n, n, n), h)
     loop = imogen.DeviceLoop([f], [abi.METHOD_HLLC], gm, cfl=args.cfl, limiter=LIMITERS[args.limiter])
-    loop.run(args.warmup, graph=True)  # with warmup >= 6 this captures all six sweep-order graphs
+    # untimed: at least six iterations, so that all six sweep-order graphs are
+    # captured before the timed region whatever --warmup is
+    loop.run(max(args.warmup, 6), graph=True)
     gc.synchronize()
     t0 = time.perf_counter()
     done = 0
@@ -258,29 +260,65 @@ def run_ensemble(args):
                      if dust else None)}
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def run_cpu_baseline(args, q0, h):
     from oracle import pyoracle as po
 
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     threads = max(1, min(threads, os.cpu_count() or 1))
-    po.set_threads(threads)
-    q = q0.copy()
-    n = q.shape[-1]
+    n = q0.shape[-1]
     gamma = 5.0 / 3.0
     lim = po.LIM_VANLEER if args.limiter == "vanleer" else po.LIM_OSPRE
-    t0 = time.perf_counter()
-    for it in range(args.cpu_iters):
-        cs = po.soundspeed(q, gamma)
-        tau = po.cfl_timestep(q, cs, h, po.HLLC, (n, n, n))
-        dt = args.cfl * tau
-        for direction in (1, -1):
-            po.fluidstep(q, h, dt, direction, it, gamma, po.HLLC, lim, modes=(po.BC_CONST,) * 6)
-    el = time.perf_counter() - t0
-    cu = q.shape[1] * q.shape[2] * q.shape[3] * 2 * args.cpu_iters
-    return {"value": cu / el, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+
+    def sample(iters, nthreads):
+        po.set_threads(nthreads)
+        q = q0.copy()
+        t0 = time.perf_counter()
+        for it in range(iters):
+            cs = po.soundspeed(q, gamma)
+            tau = po.cfl_timestep(q, cs, h, po.HLLC, (n, n, n))
+            dt = args.cfl * tau
+            for direction in (1, -1):
+                po.fluidstep(q, h, dt, direction, it, gamma, po.HLLC, lim, modes=(po.BC_CONST,) * 6)
+        return q.shape[1] * q.shape[2] * q.shape[3] * 2 * iters / (time.perf_counter() - t0), time.perf_counter() - t0
+
+    cu, el = sample(args.cpu_iters, threads)
+    cu1, el1 = sample(1, 1)
+    po.set_threads(threads)
+    return {"value": cu, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(),
             "sample": f"{args.cpu_iters} iterations (CFL + 2 fluidsteps) of the same {n}^3 Sedov HLLC "
                       f"workload from the same initial state, FP64 oracle, {threads} OpenMP threads, "
-                      f"{el:.2f} s"}
+                      f"{el:.2f} s",
+            "single_thread": {"value": cu1, "unit": "cell-updates/s", "cores": 1,
+                              "sample": f"1 iteration of the same workload on one thread, {el1:.2f} s"}}
+
+
+def run_ospre(args, q0, h):
+    """The same Sedov workload with the reference's compiled-in limiter (Ospre,
+    gpuclass/cudaFluidStep.cu:132): device-loop throughput and the sweeps'
+    HIP-event averages from a short host loop."""
+    a = argparse.Namespace(**vars(args))
+    a.limiter = "ospre"
+    a.steps = max(20, args.steps // 4)
+    _, kern, _, _ = run_gpu(a, q0, h)
+    el, _ = run_device_loop(a, q0, h)
+    cells = q0.shape[-1] ** 3
+    sweeps = {k: v["avg_ms"] for k, v in kern.items() if k.startswith("hllc_sweep")}
+    dom = max(sweeps, key=sweeps.get)
+    return {"limiter": "ospre", "iterations": a.steps, "cell_updates_per_s": cells * 2 * a.steps / el,
+            "ms_per_step": el / a.steps * 1e3, "sweep_avg_ms": sweeps,
+            "roofline_frac": BYTES_PER_CELL_SWEEP * cells / (sweeps[dom] * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
 
 def copy_probe(reps=10, ndoubles=1 << 27):
@@ -332,6 +370,7 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-no-dust", action="store_true", help="ensemble without the dusty half")
+    ap.add_argument("--no-ospre", action="store_true", help="skip the Ospre-limiter Sedov leg")
     ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args()
     if args.copy_probe:
@@ -369,6 +408,11 @@ def main():
         ok = bool(np.all(np.isfinite(final)) and np.all(final[0] > 0))
         log(f"[bench] final state finite/positive: {ok}")
 
+    ospre = None
+    if args.limiter != "ospre" and not args.no_ospre:
+        log("[bench] Sedov with the Ospre limiter beside van Leer")
+        ospre = run_ospre(args, q0, h)
+
     gasdust = None
     if not args.no_gasdust:
         log(f"[bench] gas-dust {args.gasdust_n}^3: {args.gasdust_steps} iterations, exact vs learned drag")
@@ -396,7 +440,7 @@ def main():
                           "sim_time_advanced": simtime,
                           "note": "CFL read back by the host each iteration; per-kernel times below are from this run"},
             "roofline": roofline, "cpu_baseline": cpu,
-            "kernels": kern,
+            "kernels": kern, "ospre": ospre,
             "drag_speedup_learned_vs_exact": gasdust["step_speedup"] if gasdust else None,
             "gasdust": gasdust, "ensemble": ensemble}
     print(json.dumps(line), flush=True)
