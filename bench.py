@@ -195,17 +195,22 @@ def run_gasdust(args):
 def run_ensemble(args):
     """The batched ensemble: `count` independent 2D four-quadrant HLLC problems
     with log-uniform sizes, per-problem CFL 0.4 on the device, one launch per
-    sweep direction over all problems, each iteration a replayed hipGraph; the
-    odd problems carry dust (dust-to-gas 0.5) coupled by the learned drag with
-    exact fallback. The march segment counts are autotuned on the ensemble
-    itself, then the initial states are reloaded. cell_updates_per_s counts
-    the problems' cells (2 per cell per iteration)."""
+    sweep direction and K class over all problems, each iteration a replayed
+    hipGraph. First the hydrodynamics alone, then with dust on the odd problems
+    (dust-to-gas 0.5) coupled by the learned drag with exact fallback, both
+    from the same initial states. The march segment counts are autotuned on the
+    ensemble itself. cell_updates_per_s counts the problems' cells (2 per cell
+    per iteration); sweep_hbm_frac is the sweeps' algorithmic bytes (80 B per
+    cell and sweep, 4 sweeps per iteration, gas and dust) over the whole
+    iteration time, against 8 TB/s."""
     from gpuimogen_amd import abi, ensemble as ens, gpuclass as gc, learned_drag as L
 
     sizes, states = ens.quadrant_problems(args.ensemble_count, seed=args.seed)
     dust = {p: ens.quadrant_dust(states[p], seed=args.seed + 7919 * (p + 1))
             for p in range(len(sizes)) if p % 2 == 1 and not args.ensemble_no_dust}
     e = ens.Ensemble(sizes, sizes, 1.0 / sizes, 1.0 / sizes)
+    cells = int(np.sum(sizes.astype(np.int64) ** 2))
+    dust_cells = int(sum(int(sizes[p]) ** 2 for p in dust))
 
     def load(with_dust):
         for p, s in enumerate(states):
@@ -213,51 +218,61 @@ def run_ensemble(args):
         for p, s in (dust.items() if with_dust else ()):
             e.set_dust_state(p, s)
 
+    def timed(label, fluid_cells):
+        e.iteration = 0
+        e.run(args.warmup, graph=True)
+        gc.synchronize()
+        t_before = e.times()
+        t0 = time.perf_counter()
+        done = 0
+        chunk = max(1, args.ensemble_steps // 4)
+        while done < args.ensemble_steps:
+            k = min(chunk, args.ensemble_steps - done)
+            e.run(k, graph=True)
+            done += k
+            log(f"[bench] ensemble ({label}): iteration {done}/{args.ensemble_steps}")
+        gc.synchronize()
+        el = time.perf_counter() - t0
+        adv = e.times() - t_before
+        sweep_gbs = fluid_cells * 4 * BYTES_PER_CELL_SWEEP * args.ensemble_steps / el / 1e9
+        return el, {"seconds": el, "ms_per_iteration": el / args.ensemble_steps * 1e3,
+                    "cell_updates_per_s": cells * 2 * args.ensemble_steps / el,
+                    "sweep_bytes_GBs": sweep_gbs, "sweep_hbm_frac": sweep_gbs / HBM_PEAK_GBS,
+                    "sim_time_advanced_range": [float(adv.min()), float(adv.max())]}
+
     # the march segments are tuned on the hydrodynamics alone (the drag
-    # source would only add noise to the timings), then the dust is attached
+    # source would only add noise to the timings)
     load(False)
     t0 = time.perf_counter()
     chosen = e.autotune(trials=2)
     tune_s = time.perf_counter() - t0
+    load(False)
+    _, hydro = timed("hydrodynamics", cells)
+    ok = all(bool(np.all(np.isfinite(e.state(p)))) for p in (0, len(sizes) // 2, len(sizes) - 1))
+    dusty = None
     if dust:
         params, norm, _ = L.load_model()
         L.set_model(params, norm)
         tg, td = L.cold_h2(), L.iron_dust()
         tg.gamma = 1.4  # the quadrant problems' gas
         e.set_dust([p in dust for p in range(len(sizes))], tg, td, 1.01, abi.DRAG_LEARNED)
-    load(True)
-    e.iteration = 0
-    e.run(args.warmup, graph=True)
-    gc.synchronize()
-    t0 = time.perf_counter()
-    done = 0
-    chunk = max(1, args.ensemble_steps // 4)
-    while done < args.ensemble_steps:
-        k = min(chunk, args.ensemble_steps - done)
-        e.run(k, graph=True)
-        done += k
-        log(f"[bench] ensemble: iteration {done}/{args.ensemble_steps}")
-    gc.synchronize()
-    el = time.perf_counter() - t0
-    cells = int(np.sum(sizes.astype(np.int64) ** 2))
-    times = e.times()
-    ok = all(bool(np.all(np.isfinite(e.state(p)))) for p in (0, len(sizes) // 2, len(sizes) - 1))
-    if dust:
+        load(True)
+        L.learned_stats()
+        el, dusty = timed("with dust", cells + dust_cells)
+        lc, lf = L.learned_stats()
         ok = ok and all(bool(np.all(np.isfinite(e.dust_state(p)))) for p in list(dust)[:3])
-    dust_cells = int(sum(int(sizes[p]) ** 2 for p in dust))
+        dusty.update({"problems": len(dust), "dust_to_gas": 0.5, "dust_cells": dust_cells,
+                      "drag": "learned + exact fallback",
+                      "fluid_cell_updates_per_s": (cells + dust_cells) * 2 * args.ensemble_steps / el,
+                      "learned_calls": lc, "fallback_share": lf / lc if lc else None})
     e.free()
-    cu = cells * 2 * args.ensemble_steps
+    head = dusty if dusty else hydro
     return {"workload": f"ensemble2d_quadrant_{args.ensemble_count}", "problems": len(sizes),
             "sizes": [int(sizes.min()), int(sizes.max())], "cells": cells, "iterations": args.ensemble_steps,
-            "warmup": args.warmup, "seconds": el, "cell_updates_per_s": cu / el,
-            "ms_per_iteration": el / args.ensemble_steps * 1e3,
-            "sweep_bytes_GBs": cells * 4 * BYTES_PER_CELL_SWEEP * args.ensemble_steps / el / 1e9,
+            "warmup": args.warmup, "seconds": head["seconds"], "cell_updates_per_s": head["cell_updates_per_s"],
+            "ms_per_iteration": head["ms_per_iteration"],
             "segments_by_ny_class": {"<=128": chosen[0], "<=256": chosen[1], "<=512": chosen[2]},
-            "autotune_s": tune_s, "sim_time_range": [float(times.min()), float(times.max())],
-            "finite": ok,
-            "dust": ({"problems": len(dust), "dust_to_gas": 0.5, "cells": dust_cells, "drag": "learned + exact fallback",
-                      "fluid_cell_updates_per_s": (cells + dust_cells) * 2 * args.ensemble_steps / el}
-                     if dust else None)}
+            "autotune_s": tune_s, "finite": ok, "hydro": hydro, "dust": dusty}
 
 
 def cpu_model():

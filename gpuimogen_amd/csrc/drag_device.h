@@ -352,7 +352,8 @@ int launch_learned_drag(double *g, double *d, int64_t pitch, int64_t n, const Dr
 // table entry t = (gas, dust, slab pitch, cells, problem index for dtp), items
 // = 64-cell chunks (t, first cell). Each cell gets dt = frac * dtp[problem].
 // Methods 4, 5, 6 and 7 (learned, with the queue of rejected cells integrated
-// exactly); queue needs one entry per cell of all tables.
+// exactly). queue: one entry per cell of all tables per call (2 x cells for
+// calls = 2); qcount: 3 counters.
 struct EnsDragTable {
     double *g, *d;
     int64_t pitch, n;

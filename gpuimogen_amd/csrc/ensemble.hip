@@ -395,16 +395,12 @@ static int ens_bc(EnsembleImpl *e, int dir, hipStream_t s, int xrules)
 static int ens_source(EnsembleImpl *e, double fraction, hipStream_t s)
 {
     if (!e->Pd) return SUCCESSFUL;
-    // both half-interval calls in one pass over memory (two launches for the
-    // learned method, as the single-grid device loop)
-    const bool learned = e->dragMethod == DRAG_LEARNED;
-    for (int call = 0; call < (learned ? 2 : 1); call++) {
-        int rc = launch_ens_drag(e->dragTab, e->dragItems, e->nDragItems, e->dragCells, e->dtp, fraction / 2, e->dc,
-                                 e->dragMethod, e->queue, e->qcount, s, learned ? 1 : 2);
-        if (rc) return rc;
-    }
+    // both half-interval calls in one pass over memory
+    int rc = launch_ens_drag(e->dragTab, e->dragItems, e->nDragItems, e->dragCells, e->dtp, fraction / 2, e->dc,
+                             e->dragMethod, e->queue, e->qcount, s, 2);
+    if (rc) return rc;
     for (int d = 1; d <= 2; d++) {
-        int rc = ens_bc(e, d, s, 0);
+        rc = ens_bc(e, d, s, 0);
         if (rc) return rc;
     }
     return SUCCESSFUL;
@@ -624,8 +620,8 @@ int imogen_ensemble_set_dust(ImogenEnsemble *E, const int *hasDust, double dustG
     if (!rc) rc = IM_HIP(hipMemcpy(tab, htab.data(), htab.size() * sizeof(EnsDragTable), hipMemcpyHostToDevice));
     if (!rc) rc = IM_HIP(hipMalloc(&items, hitems.size() * sizeof(EnsDragItem)));
     if (!rc) rc = IM_HIP(hipMemcpy(items, hitems.data(), hitems.size() * sizeof(EnsDragItem), hipMemcpyHostToDevice));
-    if (!rc) rc = IM_HIP(hipMalloc(&queue, (size_t)cells * sizeof(int64_t)));
-    if (!rc) rc = IM_HIP(hipMalloc(&qcount, sizeof(unsigned long long)));
+    if (!rc) rc = IM_HIP(hipMalloc(&queue, (size_t)cells * 2 * sizeof(int64_t)));  // queue + second-call list
+    if (!rc) rc = IM_HIP(hipMalloc(&qcount, 3 * sizeof(unsigned long long)));
     if (rc) {
         void *bufs[] = {dpool, dprobs, lam, part, tab, items, queue, qcount};
         for (void *b : bufs)

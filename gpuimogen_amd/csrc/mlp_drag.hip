@@ -640,6 +640,10 @@ __global__ void zero_counters(unsigned long long *p, int n)
 // items, dt = frac * dTime of the item's problem (read on the device), `calls`
 // (1 or 2) calls of dt each applied in one pass
 // ---------------------------------------------------------------------------
+// Learned list mode (list not null): the *listCount queue entries of list
+// ((table entry << 32) | cell, every entry valid), one call each; the lanes of
+// a wave may then belong to different problems. Both modes run the same
+// learned_cells code.
 template <bool LEARNED>
 __global__ __launch_bounds__(256, 2) void ens_drag_kernel(const EnsDragTable *__restrict__ tab,
                                                           const EnsDragItem *__restrict__ items, int nitems,
@@ -647,27 +651,48 @@ __global__ __launch_bounds__(256, 2) void ens_drag_kernel(const EnsDragTable *__
                                                           int method, int calls, MlpDev m,
                                                           int64_t *__restrict__ queue,
                                                           unsigned long long *__restrict__ qcount,
-                                                          unsigned long long *__restrict__ err)
+                                                          unsigned long long *__restrict__ err,
+                                                          const int64_t *__restrict__ list,
+                                                          const unsigned long long *__restrict__ listCount)
 {
     __shared__ MlpLds Lm;
     __shared__ int64_t qbuf[LEARNED ? 4 : 1][LEARNED ? kWaveQueue : 1];
     if (LEARNED) stage_model(Lm, m);
     WaveQueue wq{qbuf[LEARNED ? threadIdx.x >> 6 : 0], 0};
+    const int lane = threadIdx.x & 63;
     const int wave0 = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     const int nwaves = (int)(((int64_t)gridDim.x * blockDim.x) >> 6);
     unsigned invalid = 0, overrun = 0;
+    if (LEARNED) {
+        const int64_t total = list ? (int64_t)*listCount : (int64_t)nitems * 64;
+        for (int64_t base = (int64_t)wave0 * 64; base < total; base += (int64_t)nwaves * 64) {
+            int tt;
+            int64_t i;
+            bool live;
+            if (list) {
+                const int64_t t = base + lane;
+                live = t < total;
+                const int64_t e = list[live ? t : base];
+                tt = (int)(e >> 32);
+                i = e & kQueueIndex;
+            } else {
+                const EnsDragItem it = items[base >> 6];
+                tt = it.t;
+                i = it.base + lane;
+                live = i < tab[tt].n;
+            }
+            const EnsDragTable T = tab[tt];
+            learned_cells(T.g, T.d, T.pitch, i, live ? i : T.n - 1, live, c, frac * dtp[T.p], m, Lm, wq, queue,
+                          qcount, (int64_t)tt << 32, list ? 1 : calls);
+        }
+        wq_flush(wq, queue, qcount);
+        return;
+    }
     for (int wave = wave0; wave < nitems; wave += nwaves) {
         const EnsDragItem it = items[wave];
         const EnsDragTable T = tab[it.t];
         const double dt = frac * dtp[T.p];
-        if (LEARNED) {
-            const int64_t i = it.base + (threadIdx.x & 63);
-            const bool live = i < T.n;
-            learned_cells(T.g, T.d, T.pitch, i, live ? i : T.n - 1, live, c, dt, m, Lm, wq, queue, qcount,
-                          (int64_t)it.t << 32, calls);
-            continue;
-        }
-        const int64_t i = it.base + (threadIdx.x & 63);
+        const int64_t i = it.base + lane;
         if (i >= T.n) continue;
         double G[5], D[5];
         cell_in(T.g + i, T.d + i, T.pitch, G, D);
@@ -687,21 +712,24 @@ __global__ __launch_bounds__(256, 2) void ens_drag_kernel(const EnsDragTable *__
         }
         cell_out(T.g + i, T.d + i, T.pitch, G, D);
     }
-    if (LEARNED) wq_flush(wq, queue, qcount);
     wave_count(err + ERRW_DRAG_INVALID, invalid);
     wave_count(err + ERRW_DRAG_BUDGET, overrun);
 }
 
+// Exact integration of the queued calls; entries marked kQueueSecond are
+// appended to listOut for the learned second call (listOut null: none are).
 __global__ __launch_bounds__(256) void ens_fallback_kernel(const EnsDragTable *__restrict__ tab,
                                                            const double *__restrict__ dtp, double frac, DragConsts c,
                                                            const int64_t *__restrict__ queue,
                                                            const unsigned long long *__restrict__ qcount,
                                                            int64_t cells, unsigned long long *__restrict__ counts,
-                                                           unsigned long long *__restrict__ err)
+                                                           unsigned long long *__restrict__ err,
+                                                           int64_t *__restrict__ listOut,
+                                                           unsigned long long *__restrict__ listCount)
 {
     const unsigned long long nq = *qcount;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        atomicAdd(counts, (unsigned long long)cells);
+        if (cells) atomicAdd(counts, (unsigned long long)cells);
         if (nq) atomicAdd(counts + 1, nq);
     }
     unsigned invalid = 0, overrun = 0;
@@ -715,6 +743,7 @@ __global__ __launch_bounds__(256) void ens_fallback_kernel(const EnsDragTable *_
         const EnsDragTable T = tab[(int)((e & ~kQueueSecond) >> 32)];
         const int64_t i = e & kQueueIndex;
         fallback_cell(T.g + i, T.d + i, T.pitch, live, c, frac * dtp[T.p], invalid, overrun);
+        if (listOut) wave_append(live && (e & kQueueSecond), e & ~kQueueSecond, listOut, listCount);
     }
     wave_count(err + ERRW_DRAG_INVALID, invalid);
     wave_count(err + ERRW_DRAG_BUDGET, overrun);
@@ -1062,28 +1091,41 @@ int launch_ens_drag(const EnsDragTable *tab, const EnsDragItem *items, int nitem
         return IM_FAIL(ERROR_NOIMPLEMENT, "ensemble drag method %d", method);
     if (method == DRAG_LEARNED && !g_model.loaded)
         return IM_FAIL(ERROR_INVALID_ARGS, "learned drag: no model (imogen_drag_set_model)");
-    if (method == DRAG_LEARNED && calls != 1)
-        return IM_FAIL(ERROR_INVALID_ARGS, "ensemble learned drag takes one call per launch");
     if (nitems <= 0) return SUCCESSFUL;
-    if (method == DRAG_LEARNED) {
-        hipLaunchKernelGGL(zero_counters, dim3(1), dim3(64), 0, s, qcount, 1);
+    const bool learned = method == DRAG_LEARNED;
+    if (learned && !g_model.counts)
+        return IM_FAIL(ERROR_INVALID_ARGS, "learned drag counters not prepared (learned_drag_prepare)");
+    if (learned) {
+        hipLaunchKernelGGL(zero_counters, dim3(1), dim3(64), 0, s, qcount, 3);
         int rc = IM_LAUNCHED("zero_counters");
         if (rc) return rc;
     }
     unsigned long long *err = state()->err;
     // persistent waves (a block stages the model once for many items)
     const int blocks = (nitems + 3) / 4 < 2048 ? (nitems + 3) / 4 : 2048;
-    if (method == DRAG_LEARNED)
+    const int64_t *noList = nullptr;
+    const unsigned long long *noCount = nullptr;
+    if (learned)
         hipLaunchKernelGGL(ens_drag_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, s, tab, items, nitems, dtp,
-                           frac, c, method, calls, g_model.dev, queue, qcount, err);
+                           frac, c, method, calls, g_model.dev, queue, qcount, err, noList, noCount);
     else
         hipLaunchKernelGGL(ens_drag_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, s, tab, items, nitems, dtp,
-                           frac, c, method, calls, g_model.dev, queue, qcount, err);
+                           frac, c, method, calls, g_model.dev, queue, qcount, err, noList, noCount);
     int rc = IM_LAUNCHED("ens_drag_kernel");
-    if (rc || method != DRAG_LEARNED) return rc;
-    if (!g_model.counts) return IM_FAIL(ERROR_INVALID_ARGS, "learned drag counters not prepared (learned_drag_prepare)");
+    if (rc || !learned) return rc;
+    // calls = 2: the first pass left cells whose first call was rejected
+    // without their second call; the fallback lists them, a list-mode pass
+    // gives them a learned second call and a second fallback takes its rejects
+    int64_t *list = calls == 2 ? queue + ncells : nullptr;
     hipLaunchKernelGGL(ens_fallback_kernel, dim3(1024), dim3(256), 0, s, tab, dtp, frac, c, queue, qcount,
-                       ncells * calls, g_model.counts, err);
+                       ncells * calls, g_model.counts, err, list, qcount + 1);
+    if ((rc = IM_LAUNCHED("ens_fallback_kernel")) || calls == 1) return rc;
+    hipLaunchKernelGGL(ens_drag_kernel<true>, dim3(256), dim3(256), 0, s, tab, items, 0, dtp, frac, c, method, 1,
+                       g_model.dev, queue, qcount + 2, err, (const int64_t *)list,
+                       (const unsigned long long *)(qcount + 1));
+    if ((rc = IM_LAUNCHED("ens_drag_kernel"))) return rc;
+    hipLaunchKernelGGL(ens_fallback_kernel, dim3(1024), dim3(256), 0, s, tab, dtp, frac, c, queue, qcount + 2,
+                       (int64_t)0, g_model.counts, err, (int64_t *)nullptr, (unsigned long long *)nullptr);
     return IM_LAUNCHED("ens_fallback_kernel");
 }
 
