@@ -49,7 +49,11 @@ struct EnsItem {
 };
 
 constexpr int kEnsCflBlocks = 32;  // CFL partial minima per problem
-static const int kXClassK[6] = {1, 2, 3, 4, 6, 8};
+// X line classes: K cells per lane; classes 6 and 7 split each line into two
+// halves swept by the two waves of a workgroup pair (ens_sweep_xh)
+constexpr int kXClasses = 8;
+static const int kXClassK[kXClasses] = {1, 2, 3, 4, 6, 8, 4, 3};
+static const bool kXClassSplit[kXClasses] = {false, false, false, false, false, false, true, true};
 
 static int x_class_of(int N)
 {
@@ -57,7 +61,9 @@ static int x_class_of(int N)
     if (N <= 128) return 1;
     if (N <= 192) return 2;
     if (N <= 256) return 3;
+    if (N <= 3 * 64 * 2 - 16) return 7;  // halves of <= 184 cells + 8 halo cells in 3-cell lanes
     if (N <= 384) return 4;
+    if (N <= 4 * 64 * 2 - 16) return 6;  // halves of <= 248 cells + 8 halo cells in 4-cell lanes
     if (N <= 512) return 5;
     return -1;
 }
@@ -201,6 +207,78 @@ __global__ __launch_bounds__(256, (K <= 6) ? 2 : 1) void ens_sweep_x(const EnsPr
     }
 }
 
+// X sweep of lines split into two halves, one wave per half. A wave holds its
+// half and the 4 cells beyond each end (the two stages' stencil, periodic
+// within the line, as the whole-line sweep reads them), so it forms its half's
+// outputs alone; the window's outer cells only feed discarded values. The two
+// waves of a line are in one workgroup (4 waves: 2 lines), and every load of
+// the workgroup precedes the barrier while every store follows it, so no wave
+// overwrites a cell its partner has still to read. One work item (problem,
+// line) per wave pair; lines of at most 2 (64 K - 8) cells.
+template <int K, int LIM>
+__global__ __launch_bounds__(256, 2) void ens_sweep_xh(const EnsProblem *__restrict__ probs,
+                                                    const EnsItem *__restrict__ work, int nwork,
+                                                    const double *__restrict__ lam, int rt,
+                                                    double *__restrict__ dummy)
+{
+    const int lane = threadIdx.x & 63;
+    const int wib = threadIdx.x >> 6;
+    int item = (int)blockIdx.x * 2 + (wib >> 1);
+    const bool itemValid = item < nwork;
+    item = itemValid ? item : nwork - 1;  // waves past the list still meet the barrier
+    const int part = wib & 1;
+    const EnsItem it = work[item];
+    const EnsProblem P = probs[it.p];
+    const HydroConsts hc = ens_consts(P.gamma);
+    const double lambda = lam[2 * it.p];
+    const int N = P.nx;
+    const int h = (N + 1) >> 1;
+    const int s0 = part ? h : 0;          // first output cell of this half
+    const int Wn = (part ? N - h : h) + 8;  // window: the half and 4 cells beyond each end
+    const int nl = (Wn + K - 1) / K;
+    const bool laneUsed = lane < nl;
+    const int srcL = laneUsed ? (lane == 0 ? nl - 1 : lane - 1) : lane;
+    const int srcR = laneUsed ? (lane == nl - 1 ? 0 : lane + 1) : lane;
+    const int w0 = lane * K;
+    int cnt = Wn - w0;
+    cnt = cnt < K ? cnt : K;
+    if (!laneUsed) cnt = 0;
+    double *row = P.q + (int64_t)it.a * N;
+    V5 Q[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        int g = s0 - 4 + w0 + k;  // window cell -> line cell, periodic
+        g = (g < 0) ? g + N : g;
+        g = (g >= N) ? g - N : g;
+        g = (g < N) ? g : N - 1;
+        Q[k] = (k < cnt) ? load_cell<Hllc, 1>(row + g, P.pitch) : V5{1.0, 1.0, 0.0, 0.0, 0.0};
+    }
+    __syncthreads();
+    V5 F[K], Fm;
+    x_fluxes<Hllc, K, LIM>(hc, rt, Q, cnt, srcL, srcR, F, Fm);
+    V5 U1[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) U1[k] = axpy_update(Q[k], lambda, F[k], k == 0 ? Fm : F[k > 0 ? k - 1 : 0]);
+    x_fluxes<Hllc, K, LIM>(hc, rt, U1, cnt, srcL, srcR, F, Fm);
+    const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nthr = (int64_t)gridDim.x * blockDim.x;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        if (k < cnt) {
+            const int w = w0 + k;
+            const bool own = itemValid && w >= 4 && w < Wn - 4;
+            const V5 o = avg_update(add5(U1[k], Q[k]), lambda, F[k], k == 0 ? Fm : F[k > 0 ? k - 1 : 0]);
+            store_cell<Hllc, 1>(own ? row + (s0 + w - 4) : dummy + gtid, own ? P.pitch : nthr, o);
+        }
+    }
+}
+
+// workgroups of an X class launch over n work items
+static int64_t x_blocks(int cls, int64_t n)
+{
+    return kXClassSplit[cls] ? (n + 1) / 2 : (n + 3) / 4;
+}
+
 // Y sweep: column march, one work item (problem, first column, S) per wave
 template <int LIM>
 __global__ __launch_bounds__(256) void ens_sweep_y(const EnsProblem *__restrict__ probs,
@@ -304,8 +382,8 @@ struct EnsembleImpl {
     double *pool = nullptr;
     EnsProblem *dprobs = nullptr;
     double *lam = nullptr, *clock = nullptr, *cflPart = nullptr;
-    EnsItem *xwork[6] = {nullptr};
-    int nx_work[6] = {0};
+    EnsItem *xwork[kXClasses] = {nullptr};
+    int nx_work[kXClasses] = {0};
     EnsItem *ywork = nullptr;
     int ny_work = 0;
     double *xdummy = nullptr, *ydefer = nullptr;
@@ -340,19 +418,19 @@ static int build_work(EnsembleImpl *e)
     if (!st) return ERROR_CUDA_BLEW_UP;
     // the tables below are replaced while no kernel can still read them
     if (IM_HIP(hipStreamSynchronize(st->stream))) return ERROR_CUDA_BLEW_UP;
-    std::vector<EnsItem> xw[6], yw;
+    std::vector<EnsItem> xw[kXClasses], yw;
     for (int p = 0; p < e->NE; p++) {
         const EnsProblem &P = e->host[p];
         const int cls = x_class_of(P.nx);
         const int K = kXClassK[cls];
         const int nl = (P.nx + K - 1) / K;
-        const int lpw = 64 / nl;
+        const int lpw = kXClassSplit[cls] ? 1 : 64 / nl;  // split classes: one item per line
         for (int l = 0; l < P.ny; l += lpw) xw[cls].push_back(EnsItem{p, l, 0, 0});
         const int S = e->segS[e->host[p].owner];
         const int W = 64 / S;
         for (int x = 0; x < P.nx; x += W) yw.push_back(EnsItem{p, x, S, 0});
     }
-    for (int c = 0; c < 6; c++) {
+    for (int c = 0; c < kXClasses; c++) {
         if (e->xwork[c]) (void)hipFree(e->xwork[c]);
         e->xwork[c] = nullptr;
         e->nx_work[c] = (int)xw[c].size();
@@ -367,9 +445,9 @@ static int build_work(EnsembleImpl *e)
     if (IM_HIP(hipMemcpy(e->ywork, yw.data(), yw.size() * sizeof(EnsItem), hipMemcpyHostToDevice)))
         return ERROR_CUDA_BLEW_UP;
     // per-lane scratch: X dummy slots (5 doubles), Y deferred outputs (25 doubles)
-    int64_t xmax = 0;
-    for (int c = 0; c < 6; c++) xmax = std::max<int64_t>(xmax, e->nx_work[c]);
-    const int64_t xt = ((xmax + 3) / 4) * 256, yt = (((int64_t)e->ny_work + 3) / 4) * 256;
+    int64_t xt = 0;
+    for (int c = 0; c < kXClasses; c++) xt = std::max<int64_t>(xt, x_blocks(c, e->nx_work[c]) * 256);
+    const int64_t yt = (((int64_t)e->ny_work + 3) / 4) * 256;
     if (xt > e->xThreads) {
         if (e->xdummy) (void)hipFree(e->xdummy);
         if (IM_HIP(hipMalloc(&e->xdummy, (size_t)xt * 5 * sizeof(double)))) return ERROR_CUDA_BLEW_UP;
@@ -410,21 +488,32 @@ template <int LIM>
 static int ens_sweep(EnsembleImpl *e, int dir, int rt, hipStream_t s)
 {
     if (dir == 1) {
-        for (int c = 0; c < 6; c++) {
+        for (int c = 0; c < kXClasses; c++) {
             if (!e->nx_work[c]) continue;
-            const unsigned blocks = (unsigned)((e->nx_work[c] + 3) / 4);
+            const unsigned blocks = (unsigned)x_blocks(c, e->nx_work[c]);
 #define IM_EX(KK)                                                                                              \
     hipLaunchKernelGGL((ens_sweep_x<KK, LIM>), dim3(blocks), dim3(256), 0, s, e->dprobs, e->xwork[c],           \
                        e->nx_work[c], e->lam, rt, e->xdummy)
-            switch (kXClassK[c]) {
-            case 1: IM_EX(1); break;
-            case 2: IM_EX(2); break;
-            case 3: IM_EX(3); break;
-            case 4: IM_EX(4); break;
-            case 6: IM_EX(6); break;
-            case 8: IM_EX(8); break;
+#define IM_EXH(KK)                                                                                             \
+    hipLaunchKernelGGL((ens_sweep_xh<KK, LIM>), dim3(blocks), dim3(256), 0, s, e->dprobs, e->xwork[c],          \
+                       e->nx_work[c], e->lam, rt, e->xdummy)
+            if (kXClassSplit[c]) {
+                switch (kXClassK[c]) {
+                case 3: IM_EXH(3); break;
+                case 4: IM_EXH(4); break;
+                }
+            } else {
+                switch (kXClassK[c]) {
+                case 1: IM_EX(1); break;
+                case 2: IM_EX(2); break;
+                case 3: IM_EX(3); break;
+                case 4: IM_EX(4); break;
+                case 6: IM_EX(6); break;
+                case 8: IM_EX(8); break;
+                }
             }
 #undef IM_EX
+#undef IM_EXH
             int rc = IM_LAUNCHED("ens_sweep_x");
             if (rc) return rc;
         }
@@ -790,7 +879,7 @@ int imogen_ensemble_destroy(ImogenEnsemble *E)
     DeviceState *st = state();
     if (st) (void)hipStreamSynchronize(st->stream);
     free_graphs(e);
-    for (int c = 0; c < 6; c++)
+    for (int c = 0; c < kXClasses; c++)
         if (e->xwork[c]) (void)hipFree(e->xwork[c]);
     void *bufs[] = {e->pool,   e->dprobs,   e->lam,       e->clock, e->cflPart, e->ywork, e->xdummy,
                     e->ydefer, e->dpool,    e->dtp,       e->dragTab, e->dragItems, e->queue, e->qcount};

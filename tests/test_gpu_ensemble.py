@@ -8,7 +8,10 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-SIZES = [64, 77, 100, 128, 150, 200, 256, 300]
+# every X line class: whole lines of 1-8 cells per lane, and lines split into
+# halves (257-368 cells: 3 per lane; 385-496: 4 per lane), with both edges of
+# the split classes and odd lengths
+SIZES = [64, 77, 100, 128, 150, 200, 256, 300, 368, 369, 385, 451, 496, 509]
 
 
 def make(seed=3):
