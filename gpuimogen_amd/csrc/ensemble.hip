@@ -49,11 +49,12 @@ struct EnsItem {
 };
 
 constexpr int kEnsCflBlocks = 32;  // CFL partial minima per problem
-// X line classes: K cells per lane; classes 6 and 7 split each line into two
-// halves swept by the two waves of a workgroup pair (ens_sweep_xh)
-constexpr int kXClasses = 8;
-static const int kXClassK[kXClasses] = {1, 2, 3, 4, 6, 8, 4, 3};
-static const bool kXClassSplit[kXClasses] = {false, false, false, false, false, false, true, true};
+// X line classes: K cells per lane; lines of up to 256 cells are swept whole
+// (several short lines per wave), longer ones split into two halves swept by
+// a pair of waves (ens_sweep_xh)
+constexpr int kXClasses = 7;
+static const int kXClassK[kXClasses] = {1, 2, 3, 4, 3, 4, 5};
+static const bool kXClassSplit[kXClasses] = {false, false, false, false, true, true, true};
 
 static int x_class_of(int N)
 {
@@ -61,10 +62,9 @@ static int x_class_of(int N)
     if (N <= 128) return 1;
     if (N <= 192) return 2;
     if (N <= 256) return 3;
-    if (N <= 3 * 64 * 2 - 16) return 7;  // halves of <= 184 cells + 8 halo cells in 3-cell lanes
-    if (N <= 384) return 4;
-    if (N <= 4 * 64 * 2 - 16) return 6;  // halves of <= 248 cells + 8 halo cells in 4-cell lanes
-    if (N <= 512) return 5;
+    if (N <= 3 * 64 * 2 - 16) return 4;  // halves of <= 184 cells + 8 halo cells in 3-cell lanes
+    if (N <= 4 * 64 * 2 - 16) return 5;  // halves of <= 248 cells + 8 halo cells in 4-cell lanes
+    if (N <= 512) return 6;              // halves of <= 256 cells + 8 halo cells in 5-cell lanes
     return -1;
 }
 
@@ -151,12 +151,10 @@ __device__ __forceinline__ HydroConsts ens_consts(double gamma)
     return hc;
 }
 
-// X sweep of whole lines, K cells per lane; one work item (problem, first line) per wave
-// K <= 6 held at 2 waves per SIMD (K = 6 spills 88 bytes and still runs the
-// hydrodynamic iteration of the 512-problem ensemble 4 % faster than at one
-// wave: 5.83 against 6.09 ms); K = 8 would spill 344 bytes and stays at one
+// X sweep of whole lines (K <= 4 cells per lane, two waves per SIMD); one
+// work item (problem, first line) per wave
 template <int K, int LIM>
-__global__ __launch_bounds__(256, (K <= 6) ? 2 : 1) void ens_sweep_x(const EnsProblem *__restrict__ probs,
+__global__ __launch_bounds__(256, 2) void ens_sweep_x(const EnsProblem *__restrict__ probs,
                                                    const EnsItem *__restrict__ work, int nwork,
                                                    const double *__restrict__ lam, int rt,
                                                    double *__restrict__ dummy)
@@ -501,6 +499,7 @@ static int ens_sweep(EnsembleImpl *e, int dir, int rt, hipStream_t s)
                 switch (kXClassK[c]) {
                 case 3: IM_EXH(3); break;
                 case 4: IM_EXH(4); break;
+                case 5: IM_EXH(5); break;
                 }
             } else {
                 switch (kXClassK[c]) {
@@ -508,8 +507,6 @@ static int ens_sweep(EnsembleImpl *e, int dir, int rt, hipStream_t s)
                 case 2: IM_EX(2); break;
                 case 3: IM_EX(3); break;
                 case 4: IM_EX(4); break;
-                case 6: IM_EX(6); break;
-                case 8: IM_EX(8); break;
                 }
             }
 #undef IM_EX
