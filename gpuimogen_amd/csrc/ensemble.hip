@@ -316,7 +316,7 @@ __global__ __launch_bounds__(256) void ens_sweep_y(const EnsProblem *__restrict_
 
 // constant-extrapolation ghost cells (cudaStatics.cu:1339-1366, 1440-1466 with
 // the +y innermost-ghost convention), depth d, both faces of one direction;
-// one block per problem
+// workgroups (entry, 4)
 // xrules: the +y face copies the last interior cell, as the fluid step sets
 // its boundaries (flux.cu:99-107, the swept direction rotated into memory x);
 // 0: the y kernel's innermost-ghost copy (cudaStatics.cu:1461-1466), as
@@ -329,31 +329,40 @@ __global__ __launch_bounds__(256) void ens_bc_const(const EnsProblem *__restrict
     if (N < d + 2) return;
     // Both faces in one pass only when the cells one face reads are disjoint
     // from those the other writes (N >= 3 d + 2); otherwise the minus face
-    // first, then the plus face, as the reference's two calls run.
+    // first, then the plus face, as the reference's two calls run, by the
+    // entry's first workgroup alone (the passes are ordered by its barrier).
     const bool together = N >= 3 * d + 2;
+    if (!together && blockIdx.y > 0) return;
+    const int nsides = together ? 2 : 1;
+    const int t0 = together ? (int)(blockIdx.y * blockDim.x + threadIdx.x) : (int)threadIdx.x;
+    const int tstride = together ? (int)(gridDim.y * blockDim.x) : (int)blockDim.x;
     for (int pass = 0; pass < (together ? 1 : 2); pass++) {
         if (pass) __syncthreads();
-        const int nsides = together ? 2 : 1;
         if (dir == 1) {
-            const int64_t total = (int64_t)ny * 5 * nsides * d;
-            for (int64_t t = threadIdx.x; t < total; t += blockDim.x) {
-                const int g = (int)(t % d);
-                int64_t r = t / d;
-                const int side = together ? (int)(r % 2) : pass;
-                r /= nsides;
-                const int slab = (int)(r % 5);
-                const int64_t row = r / 5;
-                double *a = P.q + slab * P.pitch + row * nx;
-                if (side == 0) a[g] = a[d];
-                else a[nx - d + g] = a[nx - d - 1];
+            // a thread per (row, slab, side): the d ghosts of that row end
+            const int total = ny * 5 * nsides;
+            for (int t = t0; t < total; t += tstride) {
+                const int side = together ? (t & 1) : pass;
+                const int r = together ? (t >> 1) : t;
+                const int slab = r % 5;
+                const int row = r / 5;
+                double *a = P.q + slab * P.pitch + (int64_t)row * nx;
+                if (side == 0) {
+                    const double f = a[d];
+                    for (int g = 0; g < d; g++) a[g] = f;
+                } else {
+                    const double f = a[nx - d - 1];
+                    for (int g = 0; g < d; g++) a[nx - d + g] = f;
+                }
             }
         } else {
-            const int64_t total = (int64_t)nx * 5 * nsides;
-            for (int64_t t = threadIdx.x; t < total; t += blockDim.x) {
-                const int x = (int)(t % nx);
-                int64_t r = t / nx;
-                const int side = together ? (int)(r % 2) : pass;
-                const int slab = (int)(r / nsides);
+            // a thread per (column, slab, side)
+            const int total = nx * 5 * nsides;
+            for (int t = t0; t < total; t += tstride) {
+                const int x = t % nx;
+                const int r = t / nx;
+                const int side = together ? (r & 1) : pass;
+                const int slab = together ? (r >> 1) : r;
                 double *a = P.q + slab * P.pitch + x;
                 if (side == 0) {
                     const double f = a[(int64_t)d * nx];
@@ -429,6 +438,8 @@ static int build_work(EnsembleImpl *e)
         for (int x = 0; x < P.nx; x += W) yw.push_back(EnsItem{p, x, S, 0});
     }
     for (int c = 0; c < kXClasses; c++) {
+        std::stable_sort(xw[c].begin(), xw[c].end(),
+                         [e](const EnsItem &a, const EnsItem &b) { return e->host[a.p].nx > e->host[b.p].nx; });
         if (e->xwork[c]) (void)hipFree(e->xwork[c]);
         e->xwork[c] = nullptr;
         e->nx_work[c] = (int)xw[c].size();
@@ -437,6 +448,10 @@ static int build_work(EnsembleImpl *e)
         if (IM_HIP(hipMemcpy(e->xwork[c], xw[c].data(), xw[c].size() * sizeof(EnsItem), hipMemcpyHostToDevice)))
             return ERROR_CUDA_BLEW_UP;
     }
+    // longest marches first, so that the short ones fill in behind them
+    std::stable_sort(yw.begin(), yw.end(), [e](const EnsItem &a, const EnsItem &b) {
+        return (e->host[a.p].ny + a.b - 1) / a.b > (e->host[b.p].ny + b.b - 1) / b.b;
+    });
     if (e->ywork) (void)hipFree(e->ywork);
     e->ny_work = (int)yw.size();
     if (IM_HIP(hipMalloc(&e->ywork, yw.size() * sizeof(EnsItem)))) return ERROR_CUDA_BLEW_UP;
@@ -462,7 +477,7 @@ static int build_work(EnsembleImpl *e)
 
 static int ens_bc(EnsembleImpl *e, int dir, hipStream_t s, int xrules)
 {
-    hipLaunchKernelGGL(ens_bc_const, dim3((unsigned)e->NE), dim3(256), 0, s, e->dprobs, dir, 4, xrules);
+    hipLaunchKernelGGL(ens_bc_const, dim3((unsigned)e->NE, 4), dim3(256), 0, s, e->dprobs, dir, 4, xrules);
     return IM_LAUNCHED("ens_bc_const");
 }
 
