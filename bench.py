@@ -336,13 +336,16 @@ def run_ospre(args, q0, h):
             "roofline_frac": BYTES_PER_CELL_SWEEP * cells / (sweeps[dom] * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
 
-def copy_probe(reps=10, ndoubles=1 << 27):
+def copy_probe(reps=10, ndoubles=1 << 27, quiet=False):
     """Copies 1 GiB `reps` times with the library's copy kernel: the known-size
-    reference that calibrates the FETCH_SIZE / WRITE_SIZE counters."""
+    reference that calibrates the FETCH_SIZE / WRITE_SIZE counters, and the
+    practical HBM ceiling the sweeps are also compared with. Returns GB/s
+    (bytes read + written)."""
     from gpuimogen_amd import abi, gpuclass as gc
 
     a = gc.GPU_Type(dims=(ndoubles, 1, 1))
     b = gc.GPU_Type(dims=(ndoubles, 1, 1))
+    abi.call("GPU_copy", b._tp(), a._tp())  # warm
     gc.synchronize()
     t0 = time.perf_counter()
     for _ in range(reps):
@@ -350,10 +353,13 @@ def copy_probe(reps=10, ndoubles=1 << 27):
     gc.synchronize()
     el = time.perf_counter() - t0
     nbytes = ndoubles * 8
-    print(json.dumps({"copy_probe": True, "bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes,
-                      "launches": reps, "GBs": 2 * nbytes * reps / el / 1e9}), flush=True)
+    gbs = 2 * nbytes * reps / el / 1e9
+    if not quiet:
+        print(json.dumps({"copy_probe": True, "bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes,
+                          "launches": reps, "GBs": gbs}), flush=True)
     a.free()
     b.free()
+    return gbs
 
 
 def load_traffic():
@@ -415,10 +421,12 @@ def main():
     for k, v in sweeps.items():
         v["achieved_GBs"] = bytes_per_launch / (v["avg_ms"] * 1e-3) / 1e9
     traffic = load_traffic()
+    copy_gbs = copy_probe(reps=20, quiet=True)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS,
                 "traffic": traffic.get(dom) if isinstance(traffic, dict) else None,
-                "algorithmic_bytes_per_launch": bytes_per_launch}
+                "algorithmic_bytes_per_launch": bytes_per_launch,
+                "copy_kernel_GBs": copy_gbs, "frac_of_copy": achieved / copy_gbs}
     if args.check:
         ok = bool(np.all(np.isfinite(final)) and np.all(final[0] > 0))
         log(f"[bench] final state finite/positive: {ok}")

@@ -395,6 +395,11 @@ struct EnsembleImpl {
     int ny_work = 0;
     double *xdummy = nullptr, *ydefer = nullptr;
     int64_t xThreads = 0, yThreads = 0;
+    int64_t xcells[kXClasses] = {0};  // cells of each X line class (the stream balance)
+    // The X line classes sweep disjoint lines: their launches fork onto these
+    // streams (default priority) and join back, parallel branches of the graph
+    hipStream_t side[2] = {nullptr, nullptr};
+    hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
     hipGraphExec_t graph[6] = {nullptr};
     int64_t cells = 0;
     // dust (imogen_ensemble_set_dust)
@@ -426,6 +431,7 @@ static int build_work(EnsembleImpl *e)
     // the tables below are replaced while no kernel can still read them
     if (IM_HIP(hipStreamSynchronize(st->stream))) return ERROR_CUDA_BLEW_UP;
     std::vector<EnsItem> xw[kXClasses], yw;
+    for (int c = 0; c < kXClasses; c++) e->xcells[c] = 0;
     for (int p = 0; p < e->NE; p++) {
         const EnsProblem &P = e->host[p];
         const int cls = x_class_of(P.nx);
@@ -433,6 +439,7 @@ static int build_work(EnsembleImpl *e)
         const int nl = (P.nx + K - 1) / K;
         const int lpw = kXClassSplit[cls] ? 1 : 64 / nl;  // split classes: one item per line
         for (int l = 0; l < P.ny; l += lpw) xw[cls].push_back(EnsItem{p, l, 0, 0});
+        e->xcells[cls] += (int64_t)P.nx * P.ny;
         const int S = e->segS[e->host[p].owner];
         const int W = 64 / S;
         for (int x = 0; x < P.nx; x += W) yw.push_back(EnsItem{p, x, S, 0});
@@ -501,14 +508,26 @@ template <int LIM>
 static int ens_sweep(EnsembleImpl *e, int dir, int rt, hipStream_t s)
 {
     if (dir == 1) {
-        for (int c = 0; c < kXClasses; c++) {
-            if (!e->nx_work[c]) continue;
+        // classes to the three streams, largest first to the least loaded
+        int order[kXClasses], nc = 0;
+        for (int c = 0; c < kXClasses; c++)
+            if (e->nx_work[c]) order[nc++] = c;
+        std::sort(order, order + nc, [e](int a, int b) { return e->xcells[a] > e->xcells[b]; });
+        int64_t load[3] = {0, 0, 0};
+        if (IM_HIP(hipEventRecord(e->fork, s))) return ERROR_CUDA_BLEW_UP;
+        for (int k = 0; k < 2; k++)
+            if (IM_HIP(hipStreamWaitEvent(e->side[k], e->fork, 0))) return ERROR_CUDA_BLEW_UP;
+        for (int i = 0; i < nc; i++) {
+            const int c = order[i];
+            const int sl = (int)(std::min_element(load, load + 3) - load);
+            load[sl] += e->xcells[c];
+            hipStream_t sx = sl == 0 ? s : e->side[sl - 1];
             const unsigned blocks = (unsigned)x_blocks(c, e->nx_work[c]);
 #define IM_EX(KK)                                                                                              \
-    hipLaunchKernelGGL((ens_sweep_x<KK, LIM>), dim3(blocks), dim3(256), 0, s, e->dprobs, e->xwork[c],           \
+    hipLaunchKernelGGL((ens_sweep_x<KK, LIM>), dim3(blocks), dim3(256), 0, sx, e->dprobs, e->xwork[c],          \
                        e->nx_work[c], e->lam, rt, e->xdummy)
 #define IM_EXH(KK)                                                                                             \
-    hipLaunchKernelGGL((ens_sweep_xh<KK, LIM>), dim3(blocks), dim3(256), 0, s, e->dprobs, e->xwork[c],          \
+    hipLaunchKernelGGL((ens_sweep_xh<KK, LIM>), dim3(blocks), dim3(256), 0, sx, e->dprobs, e->xwork[c],         \
                        e->nx_work[c], e->lam, rt, e->xdummy)
             if (kXClassSplit[c]) {
                 switch (kXClassK[c]) {
@@ -528,6 +547,10 @@ static int ens_sweep(EnsembleImpl *e, int dir, int rt, hipStream_t s)
 #undef IM_EXH
             int rc = IM_LAUNCHED("ens_sweep_x");
             if (rc) return rc;
+        }
+        for (int k = 0; k < 2; k++) {
+            if (IM_HIP(hipEventRecord(e->join[k], e->side[k]))) return ERROR_CUDA_BLEW_UP;
+            if (IM_HIP(hipStreamWaitEvent(s, e->join[k], 0))) return ERROR_CUDA_BLEW_UP;
         }
     } else {
         const unsigned blocks = (unsigned)((e->ny_work + 3) / 4);
@@ -621,6 +644,11 @@ int imogen_ensemble_create(int nprob, const int *nx, const int *ny, const double
     if (!rc && IM_HIP(hipMemcpy(e->dprobs, e->host.data(), nprob * sizeof(EnsProblem), hipMemcpyHostToDevice)))
         rc = ERROR_CUDA_BLEW_UP;
     if (!rc && IM_HIP(hipMalloc(&e->lam, 2 * nprob * sizeof(double)))) rc = ERROR_CUDA_BLEW_UP;
+    for (int k = 0; k < 2 && !rc; k++) {
+        if (IM_HIP(hipStreamCreateWithFlags(&e->side[k], hipStreamNonBlocking))) rc = ERROR_CUDA_BLEW_UP;
+        if (!rc && IM_HIP(hipEventCreateWithFlags(&e->join[k], hipEventDisableTiming))) rc = ERROR_CUDA_BLEW_UP;
+    }
+    if (!rc && IM_HIP(hipEventCreateWithFlags(&e->fork, hipEventDisableTiming))) rc = ERROR_CUDA_BLEW_UP;
     if (!rc && IM_HIP(hipMalloc(&e->clock, nprob * sizeof(double)))) rc = ERROR_CUDA_BLEW_UP;
     if (!rc && IM_HIP(hipMemset(e->clock, 0, nprob * sizeof(double)))) rc = ERROR_CUDA_BLEW_UP;
     if (!rc && IM_HIP(hipMalloc(&e->dtp, nprob * sizeof(double)))) rc = ERROR_CUDA_BLEW_UP;
@@ -897,6 +925,11 @@ int imogen_ensemble_destroy(ImogenEnsemble *E)
                     e->ydefer, e->dpool,    e->dtp,       e->dragTab, e->dragItems, e->queue, e->qcount};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
+    for (int k = 0; k < 2; k++) {
+        if (e->side[k]) (void)hipStreamDestroy(e->side[k]);
+        if (e->join[k]) (void)hipEventDestroy(e->join[k]);
+    }
+    if (e->fork) (void)hipEventDestroy(e->fork);
     delete E;
     return SUCCESSFUL;
 }

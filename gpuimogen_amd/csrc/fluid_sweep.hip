@@ -166,16 +166,25 @@ static int launch_march(double *q, int64_t pitch, int nx, int ny, int nz, int di
     // 4 deferred output slots + 1 dummy slot of 5 doubles per lane
     double *defer = (double *)scratch_block((size_t)nthr * 25 * sizeof(double));
     if (!defer) return IM_FAIL(ERROR_NOMEM, "no scratch for deferred outputs");
-#define IM_MARCH(D, C)                                                                                  \
-    hipLaunchKernelGGL((sweep_march<M, D, LIM, C>), grid, block, 0, s, q, pitch, nx, sstride, ostride, N, S, \
+    // rows of a multiple of 16 cells in a 256-byte-aligned slab: every segment
+    // row (16, 32 or 64 columns) is whole 128-byte lines -> nontemporal hint
+    const bool lines = (nx % 16) == 0 && (pitch % 32) == 0 && ((uintptr_t)q % 256) == 0;
+#define IM_MARCH(D, C, T)                                                                                  \
+    hipLaunchKernelGGL((sweep_march<M, D, LIM, C, T>), grid, block, 0, s, q, pitch, nx, sstride, ostride, N, S, \
                        seglen, wavesPerRow, lambda, lamp, hc, rt, defer, cf)
+#define IM_MARCH_NT(D, C)             \
+    do {                              \
+        if (lines) IM_MARCH(D, C, true);  \
+        else IM_MARCH(D, C, false);       \
+    } while (0)
     if (dir == 2) {
-        if (cf.part) IM_MARCH(2, true);
-        else IM_MARCH(2, false);
+        if (cf.part) IM_MARCH_NT(2, true);
+        else IM_MARCH_NT(2, false);
     } else {
-        if (cf.part) IM_MARCH(3, true);
-        else IM_MARCH(3, false);
+        if (cf.part) IM_MARCH_NT(3, true);
+        else IM_MARCH_NT(3, false);
     }
+#undef IM_MARCH_NT
 #undef IM_MARCH
     return IM_LAUNCHED("sweep_march");
 }
