@@ -65,6 +65,13 @@ class MGArray(ctypes.Structure):
                 ("vectorComponent", c_int)]
 
 
+class ImogenBoundaryData(ctypes.Structure):
+    """What BCSettings.externalData points to: the freebalance potential and
+    adiabatic index, and an array's static-cell table."""
+    _fields_ = [("gravpot", ctypes.POINTER(MGArray)), ("gamma", c_double), ("statics", ctypes.POINTER(MGArray)),
+                ("compOffset", c_double * (12 * MAX_GPUS_USED))]
+
+
 class GeometryParams(ctypes.Structure):
     _fields_ = [("shape", c_int), ("h", c_double * 3), ("x0", c_double), ("y0", c_double),
                 ("z0", c_double), ("Rinner", c_double), ("frameRotateCenter", c_double * 3),
@@ -112,7 +119,8 @@ class ParametricRadiation(ctypes.Structure):
 STRUCTS = {"BCSettings": BCSettings, "MGArray": MGArray, "GeometryParams": GeometryParams,
            "ThermoDetails": ThermoDetails, "FluidStepParams": FluidStepParams,
            "ParallelTopology": ParallelTopology, "DragModelNorm": DragModelNorm, "GravityData": GravityData,
-           "GridFluid": GridFluid, "ParametricRadiation": ParametricRadiation}
+           "GridFluid": GridFluid, "ParametricRadiation": ParametricRadiation,
+           "ImogenBoundaryData": ImogenBoundaryData}
 
 _P = ctypes.POINTER
 _DP = _P(c_double)
@@ -150,6 +158,8 @@ SIGNATURES = {
     "performFluidUpdate_1D": (c_int, [_MGP, FluidStepParams, _P(ParallelTopology), _MGP]),
     "setFluidBoundary": (c_int, [_MGP, _P(GeometryParams), c_int]),
     "setArrayBoundaryConditions": (c_int, [_MGP, _P(GeometryParams), c_int, c_int]),
+    "setFreeBalanceCondition": (c_int, [_MGP, _P(GeometryParams), c_int, c_int, _MGP, c_double]),
+    "applyStaticCells": (c_int, [_MGP, _MGP, _P(c_double)]),
     "calculateSoundspeed": (c_int, [_MGP, _MGP, _MGP, c_double]),
     "computeLocalCFLTimestep": (c_int, [_MGP, _MGP, _P(GeometryParams), c_int, _DP, _DP]),
     "imogen_cflTimestepFused": (c_int, [_MGP, _P(GeometryParams), c_int, c_double, _DP, _DP]),

@@ -123,24 +123,118 @@ static int on_xyz_copy(MGArray *fluid, Fn &&fn)
     return rc;
 }
 
-int fluid_boundary(MGArray *fluid, int direction, int xrules)
+// Static cells of one array (setArrayStaticCells, cudaStatics.cu:288-350): the
+// entries listed for its current orientation, 2 (perm[0]-1) + (perm[1] > perm[2]).
+static int static_cells(MGArray *phi, MGArray *statics, const double *compOffset, hipStream_t s)
+{
+    if (!statics || !compOffset) return IM_FAIL(ERROR_NULL_POINTER, "null static-cell table");
+    if (phi->nGPUs != 1 || statics->nGPUs != 1 || !statics->devicePtr[0])
+        return IM_FAIL(ERROR_NOIMPLEMENT, "single-GPU static-cell tables only");
+    const int *perm = phi->currentPermutation;
+    const int oi = 2 * (perm[0] - 1) + (perm[1] > perm[2] ? 1 : 0);
+    if (oi < 0 || oi > 5) return IM_FAIL(ERROR_INVALID_ARGS, "array permutation %d %d %d", perm[0], perm[1], perm[2]);
+    const int64_t off = (int64_t)compOffset[2 * oi];
+    const int64_t n = (int64_t)compOffset[2 * oi + 1];
+    const int64_t blk = statics->dim[0];
+    if (n <= 0) return SUCCESSFUL;
+    if (off < 0 || off + n > blk || statics->numel < 3 * blk || n > INT32_MAX)
+        return IM_FAIL(ERROR_INVALID_ARGS, "static cells %lld..%lld outside a table of %lld rows", (long long)off,
+                       (long long)(off + n), (long long)blk);
+    return launch_static_fade(phi->devicePtr[0], statics->devicePtr[0] + off, (int)n, blk, phi->numel, s);
+}
+
+// the static cells of an array whose externalData lists some
+static int array_statics(MGArray *phi, hipStream_t s)
+{
+    const ImogenBoundaryData *bd = (const ImogenBoundaryData *)phi->boundaryConditions.externalData;
+    if (!bd || !bd->statics) return SUCCESSFUL;
+    return static_cells(phi, bd->statics, bd->compOffset, s);
+}
+
+// setFreeBalanceCondition's host constants (cudaStatics.cu:710-745) for one face
+static FreeBalanceConsts freebalance_consts(const MGArray *fluid, const GeometryParams *geom, int side, int depth,
+                                            double gamma)
+{
+    FreeBalanceConsts fc{};
+    const int nx0 = fluid->dim[0];
+    const double h0 = geom->h[0];
+    const double rc = side ? geom->Rinner + (nx0 - 1 - depth) * h0 : geom->Rinner + depth * h0;
+    fc.vFrame = geom->frameOmega * rc;
+    for (int k = 0; k < depth; k++) {
+        const double r = side ? geom->Rinner + (nx0 - depth + k) * h0 : geom->Rinner + k * h0;
+        const double q = r / rc;
+        fc.x0[k] = (q - 1) / q;
+        fc.qh[k] = pow(q, -0.5);
+        fc.qq[k] = q;
+    }
+    fc.invgm1 = 1 / (gamma - 1);
+    return fc;
+}
+
+// Freebalance face of the fluid whose density array is `rho` (the reference
+// reads only that array's struct and reaches the other four slabs through its
+// slab pitch, cudaStatics.cu:780-842): the owner of the 5-slab block or its
+// slab-0 view.
+static int freebalance_face(MGArray *rho, const GeometryParams *geom, int side, int direction,
+                            const MGArray *gravpot, double gamma, hipStream_t s)
+{
+    MGArray *fluid = rho;
+    if (fluid->nGPUs != 1 || !fluid->devicePtr[0]) return IM_FAIL(ERROR_NOIMPLEMENT, "single-GPU arrays only");
+    if (!(fluid->numSlabs == 0 || fluid->numSlabs >= 5))
+        return IM_FAIL(ERROR_INVALID_ARGS, "freebalance needs the density array of a 5-slab fluid (numSlabs %d)",
+                       fluid->numSlabs);
+    double *base = fluid->devicePtr[0];
+    const int64_t pitch = fluid->slabPitch[0] / (int64_t)sizeof(double);
+    if (pitch < fluid->numel) return IM_FAIL(ERROR_INVALID_ARGS, "slab pitch below the array size");
+    if (!geom) return IM_FAIL(ERROR_NULL_POINTER, "freebalance needs the geometry");
+    if (!gravpot || gravpot->nGPUs != 1 || !gravpot->devicePtr[0])
+        return IM_FAIL(ERROR_INVALID_ARGS, "freebalance needs a single-GPU gravity potential array");
+    if (!is_xyz(fluid) || !is_xyz(gravpot))
+        return IM_FAIL(ERROR_NOIMPLEMENT, "freebalance boundaries of rotated arrays are not built");
+    for (int a = 0; a < 3; a++)
+        if (gravpot->dim[a] != fluid->dim[a])
+            return IM_FAIL(ERROR_INVALID_ARGS, "gravity potential %d x %d x %d, fluid %d x %d x %d", gravpot->dim[0],
+                           gravpot->dim[1], gravpot->dim[2], fluid->dim[0], fluid->dim[1], fluid->dim[2]);
+    if (direction < 1 || direction > 3 || side < 0 || side > 1)
+        return IM_FAIL(ERROR_INVALID_ARGS, "direction %d side %d", direction, side);
+    if (!(gamma > 1.0)) return IM_FAIL(ERROR_INVALID_ARGS, "freebalance needs gamma > 1 (got %g)", gamma);
+    const int depth = fluid->haloSize > 0 ? fluid->haloSize : 4;
+    if (depth > kMaxFreeBalanceDepth)
+        return IM_FAIL(ERROR_NOIMPLEMENT, "freebalance halo depth %d (at most %d)", depth, kMaxFreeBalanceDepth);
+    const FreeBalanceConsts fc = freebalance_consts(fluid, geom, side, depth, gamma);
+    const int *d3 = fluid->dim;
+    return launch_freebalance_bc(base, pitch, gravpot->devicePtr[0], d3[0], d3[1], d3[2], direction, side, depth, fc,
+                                 s);
+}
+
+int fluid_boundary(MGArray *fluid, int direction, int xrules, const GeometryParams *geo)
 {
     double *base;
     int64_t pitch;
     int rc = fluid_view(fluid, &base, &pitch);
     if (rc) return rc;
-    if (direction < 1 || direction > 3)
+    if (direction < 0 || direction > 3)
         return IM_FAIL(ERROR_INVALID_ARGS, "direction %d is not one of 1 (X) 2 (Y) 3 (Z)", direction);
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    bool statics = false;
+    for (int i = 0; i < 5; i++) {
+        const ImogenBoundaryData *bd = (const ImogenBoundaryData *)fluid[i].boundaryConditions.externalData;
+        statics = statics || (bd && bd->statics);
+    }
+    if (direction == 0) {  // statics only (cudaStatics.cu:209-211)
+        for (int i = 0; i < 5 && !rc; i++) rc = array_statics(&fluid[i], st->stream);
+        return rc;
+    }
     int modes10[10];
-    bool out[2] = {false, false};
+    bool out[2] = {false, false}, fb[2] = {false, false};
     for (int side = 0; side < 2; side++) {
         // the density array's mode decides (cudaStatics.cu:140-152)
         const BCModeTypes bt = fluid[0].boundaryConditions.mode[2 * (direction - 1) + side];
-        if (bt == freebalance)
-            return IM_FAIL(ERROR_NOIMPLEMENT, "freebalance boundaries are not built (they need the gravity potential)");
         out[side] = (bt == outflow);
+        fb[side] = (bt == freebalance);
         for (int i = 0; i < 5; i++) {
-            if (out[side]) {
+            if (out[side] || fb[side]) {
                 modes10[2 * i + side] = -1;
                 continue;
             }
@@ -148,26 +242,33 @@ int fluid_boundary(MGArray *fluid, int direction, int xrules)
             if (rc) return rc;
         }
     }
+    const ImogenBoundaryData *bd0 = (const ImogenBoundaryData *)fluid[0].boundaryConditions.externalData;
+    if ((fb[0] || fb[1]) && (!bd0 || !bd0->gravpot))
+        return IM_FAIL(ERROR_INVALID_ARGS, "freebalance needs the gravity potential: the density array's "
+                                           "boundaryConditions.externalData must point to an ImogenBoundaryData");
     // modes and mirror signs follow the physical direction, the kernels the
     // memory axis that holds it (cudaStatics.cu:230)
     const int memdir = is_xyz(fluid) ? direction : MGA_dir2memdir(fluid->currentPermutation, direction);
     if (memdir < 1) return IM_FAIL(ERROR_INVALID_ARGS, "no memory axis holds direction %d", direction);
     if ((out[0] || out[1]) && !is_xyz(fluid))
         return IM_FAIL(ERROR_NOIMPLEMENT, "outflow boundaries of rotated arrays are not built");
-    DeviceState *st = state();
-    if (!st) return ERROR_CUDA_BLEW_UP;
     const int depth = fluid->haloSize > 0 ? fluid->haloSize : 4;
     const int *d3 = fluid->dim;
     const int slot = prof_begin(PROF_BC, st->stream);
-    if (!out[0] && !out[1]) {
+    if (!statics && !fb[0] && !fb[1] && !out[0] && !out[1]) {
         rc = launch_fluid_bc(base, pitch, d3[0], d3[1], d3[2], memdir, modes10, depth, st->stream, xrules);
-    } else if (out[0] && out[1] && d3[direction - 1] >= 2 * depth + 2) {
+    } else if (!statics && !fb[0] && !fb[1] && out[0] && out[1] && d3[direction - 1] >= 2 * depth + 2) {
         rc = launch_outflow_bc(base, pitch, d3[0], d3[1], d3[2], direction, 0, 2, depth, st->stream);
     } else {
-        // the minus face, then the plus face (setFluidBoundary's side loop)
+        // the minus face, then the plus face (setFluidBoundary's side loop),
+        // each after the static cells of every array (:141-189)
         rc = SUCCESSFUL;
         for (int side = 0; side < 2 && !rc; side++) {
-            if (out[side]) {
+            for (int i = 0; i < 5 && !rc; i++) rc = array_statics(&fluid[i], st->stream);
+            if (rc) break;
+            if (fb[side]) {
+                rc = freebalance_face(fluid, geo, side, direction, bd0->gravpot, bd0->gamma, st->stream);
+            } else if (out[side]) {
                 rc = launch_outflow_bc(base, pitch, d3[0], d3[1], d3[2], direction, side, 1, depth, st->stream);
             } else {
                 int one[10];
@@ -279,7 +380,7 @@ int update_3d(MGArray *fluid, const FluidStepParams &fsp, const double *lampBase
         p.stepDirection = 1;
         rc = update_1d(fluid, p, lampBase, lastCfl ? &cf : nullptr);
         if (rc) return rc;
-        return fluid_boundary(fluid, 1, 1);
+        return fluid_boundary(fluid, 1, 1, &fsp.geometry);
     }
     int sweep = (fsp.stepNumber + 3 * (fsp.stepDirection > 0)) % 6;
     if (sweep < 0) sweep += 6;
@@ -291,7 +392,7 @@ int update_3d(MGArray *fluid, const FluidStepParams &fsp, const double *lampBase
             if (rc) return rc;
             // the reference sets these boundaries with direction d rotated into
             // memory x (flux.cu:99-107): the x kernels' conventions along d
-            rc = fluid_boundary(fluid, d, 1);
+            rc = fluid_boundary(fluid, d, 1, &fsp.geometry);
             if (rc) return rc;
         }
     }
@@ -304,18 +405,46 @@ extern "C" {
 
 int setFluidBoundary(MGArray *fluid, GeometryParams *geo, int direction)
 {
-    (void)geo;
-    return fluid_boundary(fluid, direction, 0);
+    return fluid_boundary(fluid, direction, 0, geo);
+}
+
+int setFreeBalanceCondition(MGArray *fluid, GeometryParams *geom, int rightside, int direction, MGArray *gravpot,
+                            double fluidGamma)
+{
+    if (!fluid || !geom || !gravpot) return IM_FAIL(ERROR_NULL_POINTER, "null argument");
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    return freebalance_face(fluid, geom, rightside, direction, gravpot, fluidGamma, st->stream);
+}
+
+int applyStaticCells(MGArray *phi, MGArray *statics, const double *compOffset)
+{
+    if (!phi || !statics || !compOffset) return IM_FAIL(ERROR_NULL_POINTER, "null argument");
+    DeviceState *st = state();
+    if (!st) return ERROR_CUDA_BLEW_UP;
+    return static_cells(phi, statics, compOffset, st->stream);
 }
 
 int setArrayBoundaryConditions(MGArray *phi, GeometryParams *geo, int direction, int side)
 {
-    (void)geo;
     if (!phi) return IM_FAIL(ERROR_NULL_POINTER, "null array");
     if (phi->nGPUs != 1) return IM_FAIL(ERROR_NOIMPLEMENT, "single-GPU arrays only");
-    if (direction == 0) return SUCCESSFUL;
-    if (direction < 1 || direction > 3 || side < 0 || side > 1)
+    if (direction < 0 || direction > 3 || side < 0 || side > 1)
         return IM_FAIL(ERROR_INVALID_ARGS, "direction %d side %d", direction, side);
+    {
+        DeviceState *st0 = state();
+        if (!st0) return ERROR_CUDA_BLEW_UP;
+        int rs = array_statics(phi, st0->stream);  // cudaStatics.cu:208
+        if (rs || direction == 0) return rs;
+    }
+    if (phi->boundaryConditions.mode[2 * (direction - 1) + side] == freebalance) {
+        // set through the density array alone (numSlabs 0, :259-281); the
+        // reference leaves the face as it is without a potential (:264)
+        const ImogenBoundaryData *bd = (const ImogenBoundaryData *)phi->boundaryConditions.externalData;
+        if (phi->numSlabs != 0 || !bd || !bd->gravpot) return SUCCESSFUL;
+        DeviceState *st1 = state();
+        return freebalance_face(phi, geo, side, direction, bd->gravpot, bd->gamma, st1->stream);
+    }
     int code;
     int rc = sas_code(phi, direction, side, &code);
     if (rc) return rc;

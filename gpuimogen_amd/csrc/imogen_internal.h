@@ -104,7 +104,7 @@ int update_1d(MGArray *fluid, const FluidStepParams &params, const double *lampB
 int update_3d(MGArray *fluid, const FluidStepParams &fsp, const double *lampBase, const CflFuse *lastCfl = nullptr);
 // setFluidBoundary with a choice of kernel conventions (launch_fluid_bc's
 // xrules); update_3d uses xrules = 1 after every sweep.
-int fluid_boundary(MGArray *fluid, int direction, int xrules);
+int fluid_boundary(MGArray *fluid, int direction, int xrules, const GeometryParams *geo);
 // Direction swept last by update_3d for (stepNumber, stepDirection), 0 if none.
 int last_sweep_dir(const MGArray *fluid, int stepNumber, int stepDirection);
 // Cells [lo, hi) of direction dir whose swept values survive the boundary
@@ -134,6 +134,21 @@ int launch_fluid_bc(double *base, int64_t pitch, int nx, int ny, int nz, int dir
 // slabs; both faces together only when they are disjoint, N >= 2 depth + 2).
 int launch_outflow_bc(double *base, int64_t pitch, int nx, int ny, int nz, int dir, int side0, int nsides,
                       int depth, hipStream_t s);
+// Freebalance ghosts of one face (side 0/1) of direction dir: the host-side
+// constants of setFreeBalanceCondition (cudaStatics.cu:710-745) per ghost k.
+constexpr int kMaxFreeBalanceDepth = 8;
+struct FreeBalanceConsts {
+    double x0[kMaxFreeBalanceDepth];  // (q-1)/q, q = ghost radius / reference cell radius
+    double qh[kMaxFreeBalanceDepth];  // q^-1/2
+    double qq[kMaxFreeBalanceDepth];  // q
+    double vFrame;                    // frame velocity at the reference cell
+    double invgm1;                    // 1 / (gamma - 1)
+};
+int launch_freebalance_bc(double *base, int64_t pitch, const double *phi, int nx, int ny, int nz, int dir,
+                          int side, int depth, const FreeBalanceConsts &fc, hipStream_t s);
+// Static-cell fade of nspec entries of table (columns of blk doubles) on an
+// array of numel cells.
+int launch_static_fade(double *phi, const double *table, int nspec, int64_t blk, int64_t numel, hipStream_t s);
 int launch_soundspeed(const double *base, int64_t pitch, int64_t n, double gamma, double *cs,
                       hipStream_t s);
 int launch_cfl(const double *base, int64_t pitch, int64_t n, const double *cs, double gamma, int fused,

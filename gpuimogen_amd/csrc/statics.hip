@@ -189,6 +189,107 @@ int launch_outflow_bc(double *base, int64_t pitch, int nx, int ny, int nz, int d
     return IM_LAUNCHED("bc_outflow");
 }
 
+// Freebalance ghosts of one face (cukern_SetFreeBalance{X,Y,Z}, cudaStatics.cu:859-1003):
+// one thread per face cell computes all its ghosts from the interior cell next
+// to the face, in the reference's operation order (no FMA contraction).
+__global__ void bc_freebalance(double *__restrict__ q, int64_t pitch, const double *__restrict__ phi, int nx,
+                               int64_t n2, int64_t stride, int64_t ostride, int N, int dir, int side, int d,
+                               FreeBalanceConsts fc)
+{
+#pragma clang fp contract(off)
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t lines = (int64_t)((dir == 1) ? 1 : nx) * n2;
+    if (tid >= lines) return;
+    const int64_t off = (dir == 1) ? tid * ostride : (tid % nx) + (tid / nx) * ostride;
+    double *r = q + off;
+    const double *ph = phi + off;
+    const int64_t c = (int64_t)(side ? N - 1 - d : d) * stride;
+    const double rho = r[c];
+    const double vx = r[2 * pitch + c] / rho;
+    const double vy = r[3 * pitch + c] / rho;
+    double vz = r[4 * pitch + c] / rho;
+    const double E = r[pitch + c] - .5 * rho * (vx * vx + vy * vy + vz * vz);
+    const double T = E / rho;
+    const double phi0 = ph[c];
+    const double direct = side ? 1.0 : -1.0;
+    for (int k = 0; k < d; k++) {
+        const int64_t gi = (int64_t)(side ? N - d + k : k) * stride;
+        double phi1 = ph[gi], rho2, th;
+        if (rho < 1e-7) {  // the tenuous (solar-wind) side of the disk setups, :908-910
+            phi1 = phi0;
+            vz = (vz * direct < 0) ? 0 : vz;
+        }
+        if (dir != 1) {
+            rho2 = rho * exp(-fc.invgm1 * (phi1 - phi0) / T);
+            th = vy;
+        } else {
+            th = vy + fc.vFrame;
+            const double cp = th * th * fc.x0[k];
+            th = th * fc.qh[k] - fc.vFrame * fc.qq[k];
+            rho2 = rho * exp(-fc.invgm1 * (phi1 - phi0 - cp) / T);
+        }
+        r[gi] = rho2;
+        r[2 * pitch + gi] = rho2 * vx;
+        r[3 * pitch + gi] = rho2 * th;
+        r[4 * pitch + gi] = rho2 * vz;
+        r[pitch + gi] = rho2 * T + .5 * rho2 * (vx * vx + th * th + vz * vz);
+    }
+}
+
+int launch_freebalance_bc(double *base, int64_t pitch, const double *phi, int nx, int ny, int nz, int dir,
+                          int side, int depth, const FreeBalanceConsts &fc, hipStream_t s)
+{
+    const int N = (dir == 1) ? nx : (dir == 2 ? ny : nz);
+    if (depth < 1 || depth > kMaxFreeBalanceDepth)
+        return IM_FAIL(ERROR_INVALID_ARGS, "freebalance depth %d (1..%d)", depth, kMaxFreeBalanceDepth);
+    if (N < depth + 2) return IM_FAIL(ERROR_INVALID_ARGS, "freebalance BC needs >= %d cells (got %d)", depth + 2, N);
+    int64_t n2, stride, ostride;
+    if (dir == 1) {
+        n2 = (int64_t)ny * nz;
+        stride = 1;
+        ostride = nx;
+    } else {
+        n2 = (dir == 2) ? nz : ny;
+        stride = (dir == 2) ? (int64_t)nx : (int64_t)nx * ny;
+        ostride = (dir == 2) ? (int64_t)nx * ny : (int64_t)nx;
+    }
+    const int64_t threads = ((dir == 1) ? 1 : (int64_t)nx) * n2;
+    hipLaunchKernelGGL(bc_freebalance, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, base, pitch, phi,
+                       nx, n2, stride, ostride, N, dir, side, depth, fc);
+    return IM_LAUNCHED("bc_freebalance");
+}
+
+// Static cells of one array (cukern_applySpecial_fade, cudaStatics.cu:1251-1281):
+// entry s holds the cell index table[s], the value f0 = table[s + blk] and the
+// coefficient c = table[s + 2 blk]. Indices outside [0, numel) are skipped
+// (the reference does not check them).
+__global__ void static_fade(double *__restrict__ phi, const double *__restrict__ table, int nspec, int64_t blk,
+                            int64_t numel)
+{
+#pragma clang fp contract(off)
+    const int s = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (s >= nspec) return;
+    const double xi = table[s];
+    if (!(xi >= 0.0 && xi < (double)numel)) return;
+    const int64_t x = (int64_t)xi;
+    const double f0 = table[s + blk], c = table[s + 2 * blk];
+    if (c != 1.0) {
+        double w = f0 * c + (1.0 - c) * phi[x];
+        if (w != w) w = f0 * c;
+        phi[x] = w;
+    } else {
+        phi[x] = f0;
+    }
+}
+
+int launch_static_fade(double *phi, const double *table, int nspec, int64_t blk, int64_t numel, hipStream_t s)
+{
+    if (nspec <= 0) return SUCCESSFUL;
+    hipLaunchKernelGGL(static_fade, dim3((unsigned)((nspec + 255) / 256)), dim3(256), 0, s, phi, table, nspec, blk,
+                       numel);
+    return IM_LAUNCHED("static_fade");
+}
+
 int launch_fluid_bc(double *base, int64_t pitch, int nx, int ny, int nz, int dir, const int *modes10,
                     int depth, hipStream_t s, int xrules)
 {

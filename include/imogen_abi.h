@@ -270,11 +270,40 @@ int performFluidUpdate_3D_noalloc(MGArray *fluid, ParallelTopology *parallelTopo
 int performFluidUpdate_1D(MGArray *fluid, FluidStepParams params, ParallelTopology *topo,
                           MGArray *tmpst);
 
+/* What the reference's boundary code reads from the MATLAB class's
+ * boundaryData struct through BCSettings.externalData (cudaStatics.cu:165-176,
+ * 262-272, 293-321) for the parts that need more than the arrays: the
+ * gravity potential and adiabatic index of the freebalance mode, and an
+ * array's static cells. Here BCSettings.externalData points to one of these
+ * (or is NULL: no statics, and a freebalance face is an error); the MEX
+ * gateways fill it from boundaryData. The freebalance fields are read from
+ * the density array's, the statics of each array from its own. */
+typedef struct ImogenBoundaryData {
+    MGArray *gravpot;  /* boundaryData.gravpot: one slab of the fluid's dims, or NULL */
+    double gamma;      /* boundaryData.gamma */
+    MGArray *statics;  /* boundaryData.staticsData (table columns: cell index, value, coefficient), or NULL */
+    double compOffset[12 * MAX_GPUS_USED]; /* boundaryData.compOffset: per GPU, 6 orientations x (offset, count) */
+} ImogenBoundaryData;
+
 /* gpuclass/cudaStatics.h: setFluidBoundary (cudaStatics.cu:112-196) and
  * setArrayBoundaryConditions (cudaStatics.cu:199-286). direction 1/2/3,
- * side 0 = minus face, 1 = plus face. */
+ * side 0 = minus face, 1 = plus face; direction 0 applies only the static
+ * cells. Each array's static cells (externalData) are applied first. */
 int setFluidBoundary(MGArray *fluid, GeometryParams *geo, int direction);
 int setArrayBoundaryConditions(MGArray *phi, GeometryParams *geo, int direction, int side);
+
+/* cudaStatics.cu:700-857: freebalance ghosts of one face (rightside 0/1) of
+ * the 5-array fluid along direction 1/2/3, balanced against the gravity
+ * potential gravpot (same dims) at the interior cell's temperature; direction
+ * 1 adds the centripetal potential of a Kepler-extrapolated azimuthal
+ * velocity (geom Rinner, h[0], frameOmega). XYZ-ordered arrays. */
+int setFreeBalanceCondition(MGArray *fluid, GeometryParams *geom, int rightside, int direction, MGArray *gravpot,
+                            double fluidGamma);
+
+/* cudaStatics.cu:288-350 (setArrayStaticCells) with the boundaryData fields
+ * passed explicitly: the fade of phi's static cells listed in statics for
+ * phi's current orientation (compOffset as in ImogenBoundaryData). */
+int applyStaticCells(MGArray *phi, MGArray *statics, const double *compOffset);
 
 /* gpuclass/cudaSoundspeed.h, cudaSoundspeed.cu:107-152 (mag must be NULL: hydro only) */
 int calculateSoundspeed(MGArray *fluid, MGArray *mag, MGArray *cs, double gamma);

@@ -107,6 +107,61 @@ static int read_bc_modes(const mxArray *m, int idx, BCSettings *b)
     return SUCCESSFUL;
 }
 
+// The boundaryData fields the library reads through externalData (the
+// freebalance potential and gamma, the static cells), for one array, kept in a
+// ring of slots that outlives the gateway call that fills it (a gateway
+// passes at most a few fluids' 5 arrays).
+struct BoundaryDataSlot {
+    ImogenBoundaryData bd;
+    MGArray gravpot, statics;
+};
+static BoundaryDataSlot g_bdSlots[64];
+static int g_bdNext = 0;
+
+static int array_of(const mxArray *m, MGArray *out)
+{
+    int64_t tag[GPU_TAG_MAXLEN];
+    int rc = imogen_mex_tag(m, 0, tag);
+    return rc ? rc : deserializeTagToMGArray(tag, out);
+}
+
+// boundaryData.gravpot.field and .gamma (cudaStatics.cu:165-176), .staticsData
+// and .compOffset (:302-321)
+static int read_boundary_data(const mxArray *m, int idx, BCSettings *b)
+{
+    if (mxGetClassID(m) == mxINT64_CLASS || mxIsStruct(m)) return SUCCESSFUL;
+    Prop bd(m, idx, "boundaryData");
+    if (!bd.v || !mxIsStruct(bd.v)) return SUCCESSFUL;
+    const mxArray *gp = mxGetField(bd.v, 0, "gravpot");
+    const mxArray *st = mxGetField(bd.v, 0, "staticsData");
+    const mxArray *co = mxGetField(bd.v, 0, "compOffset");
+    const bool grav = gp && mxGetNumberOfElements(gp) > 0;
+    const bool stat = st && mxGetNumberOfElements(st) > 0 && co && mxIsDouble(co) && mxGetNumberOfElements(co) >= 12;
+    if (!grav && !stat) return SUCCESSFUL;
+    BoundaryDataSlot &s = g_bdSlots[g_bdNext];
+    g_bdNext = (g_bdNext + 1) % 64;
+    memset(&s, 0, sizeof(s));
+    if (grav) {
+        Prop field(gp, 0, "field");
+        int rc = array_of(field.v ? field.v : gp, &s.gravpot);
+        if (rc) return rc;
+        int ok;
+        s.bd.gravpot = &s.gravpot;
+        s.bd.gamma = imogen_mex_scalar(bd.v, 0, "gamma", &ok);
+        if (!ok) return set_fail(ERROR_INVALID_ARGS, "boundaryData.gamma missing (freebalance)");
+    }
+    if (stat) {
+        int rc = array_of(st, &s.statics);
+        if (rc) return rc;
+        s.bd.statics = &s.statics;
+        size_t n = mxGetNumberOfElements(co);
+        if (n > 12 * MAX_GPUS_USED) n = 12 * MAX_GPUS_USED;
+        for (size_t k = 0; k < n; k++) s.bd.compOffset[k] = mxGetPr(co)[k];
+    }
+    b->externalData = &s.bd;
+    return SUCCESSFUL;
+}
+
 int imogen_mex_array(const mxArray *m, int idx, MGArray *out)
 {
     int64_t tag[GPU_TAG_MAXLEN];
@@ -114,9 +169,10 @@ int imogen_mex_array(const mxArray *m, int idx, MGArray *out)
     if (rc) return rc;
     rc = deserializeTagToMGArray(tag, out);
     if (rc) return rc;
-    out->boundaryConditions.externalData = nullptr;  // the library reads no MATLAB data
+    out->boundaryConditions.externalData = nullptr;
     out->boundaryConditions.extIndex = idx;
-    return read_bc_modes(m, idx, &out->boundaryConditions);
+    rc = read_bc_modes(m, idx, &out->boundaryConditions);
+    return rc ? rc : read_boundary_data(m, idx, &out->boundaryConditions);
 }
 
 int imogen_mex_fluid(const mxArray *fm, int fluidIdx, MGArray fluid[5])

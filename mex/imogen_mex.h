@@ -25,7 +25,8 @@
 int imogen_mex_tag(const mxArray *m, int idx, int64_t *tag);
 
 /* MGArray of argument m (element idx), with its boundary modes when m carries
- * boundaryData.bcModes. */
+ * boundaryData.bcModes, and its ImogenBoundaryData (externalData) when
+ * boundaryData holds a gravity potential or static cells. */
 int imogen_mex_array(const mxArray *m, int idx, MGArray *out);
 
 /* The five slab views of fluid fluidIdx of a FluidManager (array). */

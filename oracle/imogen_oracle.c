@@ -900,6 +900,103 @@ int orc_outflow_face(double *q, const orc_grid *g, int dir, int side, int depth)
     return ORC_OK;
 }
 
+/* Freebalance face (BCModeTypes freebalance, cudaCommon.h:48; setFreeBalanceCondition and
+ * cukern_SetFreeBalance{X,Y,Z}, gpuclass/cudaStatics.cu:700-1003). The interior cell next to
+ * the face (cell depth, or N-1-depth, along dir) gives rho, v = p/rho, the internal energy
+ * density e = E - rho|v|^2/2 and T = e/rho. Each ghost holds the same velocity and T with the
+ * density of isothermal balance in the potential: rho_g = rho exp(-(phi_g - phi_c) / ((gamma-1) T))
+ * (:916-922). For dir 1 (radial in the disk setups) the azimuthal velocity vy, taken to the
+ * inertial frame (vy + Omega r_c), is extrapolated as a Kepler curve and its centripetal
+ * potential enters the balance (:924-937): with q = r_g / r_c, the potential (q-1)/q w^2 and the
+ * ghost's frame velocity w q^-1/2 - Omega r_c q (hostRadQ, :718-745; r = Rinner + i h0, the
+ * reference cell at i = depth, or nx0 - 1 - depth on the plus face). A cell with rho < 1e-7
+ * (the solar-wind side of those setups, :908-910) keeps phi_g = phi_c and loses a z velocity
+ * that points into the domain. The ghost energy is rho_g T + rho_g |v|^2 / 2 (:945). */
+int orc_freebalance_face(double *q, const orc_grid *g, const double *phi, int dir, int side, int depth,
+                         double gamma, double Rinner, double h0, double frameOmega, int nx0)
+{
+    const long n3[3] = {g->nx, g->ny, g->nz};
+    const long s3[3] = {1, (long)g->nx, (long)g->nx * g->ny};
+    const int a1 = (dir == 1) ? 1 : 0, a2 = (dir == 3) ? 1 : 2;
+    if (dir < 1 || dir > 3 || side < 0 || side > 1 || depth < 1 || depth > 8) return ORC_EINVAL;
+    const long N = n3[dir - 1], st = s3[dir - 1], P = g->pitch;
+    const int d = depth;
+    if (N < d + 2) return ORC_EINVAL;
+    /* radial constants of this face (hostRadQ) and the frame velocity at the reference cell */
+    double x0[8], qh[8], qq[8];
+    const double rc = side ? Rinner + (nx0 - 1 - d) * h0 : Rinner + d * h0;
+    const double vFrame = frameOmega * rc;
+    int k;
+    for (k = 0; k < d; k++) {
+        const double r = side ? Rinner + (nx0 - d + k) * h0 : Rinner + k * h0;
+        const double qr = r / rc;
+        x0[k] = (qr - 1) / qr;
+        qh[k] = pow(qr, -0.5);
+        qq[k] = qr;
+    }
+    const double invgm1 = 1 / (gamma - 1);
+    const long c = side ? N - 1 - d : d;
+    const double direct = side ? 1.0 : -1.0;
+    long i, j;
+    for (j = 0; j < n3[a2]; j++)
+        for (i = 0; i < n3[a1]; i++) {
+            double *r = q + i * s3[a1] + j * s3[a2];
+            const double *ph = phi + i * s3[a1] + j * s3[a2];
+            const double rho = r[c * st];
+            const double vx = r[2 * P + c * st] / rho;
+            const double vy = r[3 * P + c * st] / rho;
+            double vz = r[4 * P + c * st] / rho;
+            const double E = r[P + c * st] - .5 * rho * (vx * vx + vy * vy + vz * vz);
+            const double T = E / rho;
+            const double phi0 = ph[c * st];
+            for (k = 0; k < d; k++) {
+                const long gi = (side ? N - d + k : k) * st;
+                double phi1 = ph[gi], rho2, th;
+                if (rho < 1e-7) {
+                    phi1 = phi0;
+                    vz = (vz * direct < 0) ? 0 : vz;
+                }
+                if (dir != 1) {
+                    rho2 = rho * exp(-invgm1 * (phi1 - phi0) / T);
+                    th = vy;
+                } else {
+                    double cp;
+                    th = vy + vFrame;
+                    cp = th * th * x0[k];
+                    th = th * qh[k] - vFrame * qq[k];
+                    rho2 = rho * exp(-invgm1 * (phi1 - phi0 - cp) / T);
+                }
+                r[gi] = rho2;
+                r[2 * P + gi] = rho2 * vx;
+                r[3 * P + gi] = rho2 * th;
+                r[4 * P + gi] = rho2 * vz;
+                r[P + gi] = rho2 * T + .5 * rho2 * (vx * vx + th * th + vz * vz);
+            }
+        }
+    return ORC_OK;
+}
+
+/* Static cells of one array (setArrayStaticCells and cukern_applySpecial_fade,
+ * gpuclass/cudaStatics.cu:288-350, 1251-1281): entry s of the table is the
+ * cell index table[s], the value f0 = table[s + blk] and the coefficient
+ * c = table[s + 2 blk]; c = 1 sets the cell to f0, otherwise it fades to
+ * f0 c + (1 - c) phi (f0 c where that is NaN). */
+void orc_static_cells(double *phi, const double *table, int nspec, long blk)
+{
+    int s;
+    for (s = 0; s < nspec; s++) {
+        const long x = (long)table[s];
+        const double f0 = table[s + blk], c = table[s + 2 * blk];
+        if (c != 1.0) {
+            double w = f0 * c + (1.0 - c) * phi[x];
+            if (isnan(w)) w = f0 * c;
+            phi[x] = w;
+        } else {
+            phi[x] = f0;
+        }
+    }
+}
+
 int orc_set_fluid_boundary(double *q, const orc_grid *g, int dir, const int *modes, int depth)
 {
     if (dir < 1 || dir > 3) return ORC_EINVAL;

@@ -66,6 +66,16 @@ int orc_xj_predict_1d(double *q, const orc_grid *g, int dir, double lambda, cons
 int orc_set_fluid_boundary(double *q, const orc_grid *g, int dir, const int *modes, int depth);
 /* outflow ghosts of one face (side 0 = minus, 1 = plus) of all 5 slabs */
 int orc_outflow_face(double *q, const orc_grid *g, int dir, int side, int depth);
+/* freebalance ghosts of one face (side 0 = minus, 1 = plus) of all 5 slabs:
+ * phi = the gravity potential (one slab, same grid), gamma the fluid's
+ * adiabatic index, Rinner / h0 / frameOmega / nx0 the geometry the radial
+ * (direction 1) balance reads (GeometryParams Rinner, h[0], frameOmega and the
+ * fluid's x size) */
+int orc_freebalance_face(double *q, const orc_grid *g, const double *phi, int dir, int side, int depth,
+                         double gamma, double Rinner, double h0, double frameOmega, int nx0);
+/* static cells of one array (setArrayStaticCells's fade): table holds nspec
+ * entries per column block of blk doubles: cell index, value f0, coefficient c */
+void orc_static_cells(double *phi, const double *table, int nspec, long blk);
 /* the boundaries orc_fluidstep sets after sweeping dir: memory-x conventions
  * along dir (flux.cu:99-107 after the rotations of flux.cu:46-48) */
 int orc_fluid_boundary_swept(double *q, const orc_grid *g, int dir, const int *modes, int depth);

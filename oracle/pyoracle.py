@@ -61,6 +61,11 @@ def lib():
                                         ctypes.POINTER(_Params)]
         L.orc_xj_predict_1d.restype = ctypes.c_int
         L.orc_set_fluid_boundary.argtypes = [dp, ctypes.POINTER(_Grid), ctypes.c_int, ip, ctypes.c_int]
+        L.orc_freebalance_face.argtypes = [dp, ctypes.POINTER(_Grid), dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_int]
+        L.orc_static_cells.argtypes = [dp, dp, ctypes.c_int, ctypes.c_long]
+        L.orc_static_cells.restype = None
         L.orc_set_fluid_boundary.restype = ctypes.c_int
         L.orc_fluid_boundary_swept.argtypes = [dp, ctypes.POINTER(_Grid), ctypes.c_int, ip, ctypes.c_int]
         L.orc_fluid_boundary_swept.restype = ctypes.c_int
@@ -145,6 +150,24 @@ def set_fluid_boundary(q, direction, modes, depth=4):
                                       m.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), depth)
     if rc:
         raise RuntimeError(f"orc_set_fluid_boundary returned {rc}")
+
+
+def freebalance_face(q, phi, direction, side, gamma, rinner=0.0, h0=1.0, frame_omega=0.0, depth=4):
+    """Freebalance ghosts of one face of q (5, nz, ny, nx) in place; phi (nz, ny, nx)."""
+    g = _grid(q)
+    ph = np.ascontiguousarray(phi, dtype=np.float64)
+    assert ph.shape == q.shape[1:]
+    rc = lib().orc_freebalance_face(_dptr(q), ctypes.byref(g), _dptr(ph), direction, side, depth, gamma, rinner,
+                                    h0, frame_omega, q.shape[3])
+    if rc:
+        raise RuntimeError(f"orc_freebalance_face returned {rc}")
+
+
+def static_cells(phi, table):
+    """Static-cell fade of one array in place: table (3, n) = cell index, f0, c."""
+    t = np.ascontiguousarray(table, dtype=np.float64)
+    assert phi.flags.c_contiguous and phi.dtype == np.float64
+    lib().orc_static_cells(_dptr(phi), _dptr(t), t.shape[1], t.shape[1])
 
 
 def fluid_boundary_swept(q, direction, modes, depth=4):
