@@ -335,6 +335,11 @@ int cfl_keep_range(MGArray *fluid, int dir, int *lo, int *hi)
     const int depth = fluid->haloSize > 0 ? fluid->haloSize : 4;
     *lo = 0;
     *hi = N;
+    // static cells change interior cells after the sweep
+    for (int i = 0; i < 5; i++) {
+        const ImogenBoundaryData *bd = (const ImogenBoundaryData *)fluid[i].boundaryConditions.externalData;
+        if (bd && bd->statics) return 1;
+    }
     for (int side = 0; side < 2; side++) {
         const BCModeTypes bt = fluid[0].boundaryConditions.mode[2 * (dir - 1) + side];
         if (bt == outflow || bt == freebalance) return 1;
