@@ -63,17 +63,30 @@ static int march_segments(int64_t columns, int N)
 struct XGeom {
     int K = 0, nl = 0;
     int64_t blocks = 0;
+    bool split = false;  // sweep_xh: halves of the line, one wave each, 2 lines per 256-thread block
 };
 
 static XGeom x_geom(int N, int64_t nlines)
 {
     XGeom g;
+    if (N > 256 && N <= 512 && N != 384 && N != 512) {
+        // halves of <= 184 / 248 / 256 cells plus 8 window cells in 3 / 4 / 5-cell lanes.
+        // Lines of exactly 384 or 512 cells fill every lane of a whole-line wave
+        // (6 or 8 cells each) and stay whole: 384^3 X sweep 1.13 ms whole,
+        // 1.43 split; 512^3 3.33 whole, 3.29 split; 320^3 1.29 whole (partly
+        // filled lanes), 0.78 split.
+        g.split = true;
+        g.K = (N <= 3 * 64 * 2 - 16) ? 3 : (N <= 4 * 64 * 2 - 16 ? 4 : 5);
+        g.nl = 64;
+        g.blocks = (nlines + 1) / 2;
+        return g;
+    }
     if (N <= 64) g.K = 1;
     else if (N <= 128) g.K = 2;
     else if (N <= 192) g.K = 3;
     else if (N <= 256) g.K = 4;
-    else if (N <= 384) g.K = 6;
-    else if (N <= 512) g.K = 8;
+    else if (N == 384) g.K = 6;
+    else if (N == 512) g.K = 8;
     else if (N <= 768) g.K = 12;
     else if (N <= 1024) g.K = 16;
     else return g;
@@ -117,10 +130,29 @@ static int launch_x(double *q, int64_t pitch, int N, int64_t nlines, double lamb
     const int K = g.K, nl = g.nl;
     const int64_t blocks = g.blocks;
     if (cf.part && blocks * 4 > cf.cap) return IM_FAIL(ERROR_INVALID_ARGS, "CFL partial buffer too small");
+    if (blocks > INT32_MAX) return IM_FAIL(ERROR_INVALID_ARGS, "too many lines for one X launch");
     dim3 grid((unsigned)blocks), block(256);
     const bool full = (nl * K == N) && (64 % nl == 0);
     double *dummy = (double *)scratch_block((size_t)blocks * 256 * 5 * sizeof(double));
     if (!dummy) return IM_FAIL(ERROR_NOMEM, "no scratch for the X sweep");
+    if (g.split) {
+#define IM_XH(KK)                                                                                        \
+    case KK:                                                                                             \
+        if (cf.part)                                                                                     \
+            hipLaunchKernelGGL((sweep_xh<M, KK, LIM, true>), grid, block, 0, s, q, pitch, N, nlines, lambda, \
+                               lamp, hc, rt, dummy, cf);                                                 \
+        else                                                                                             \
+            hipLaunchKernelGGL((sweep_xh<M, KK, LIM, false>), grid, block, 0, s, q, pitch, N, nlines,     \
+                               lambda, lamp, hc, rt, dummy, cf);                                         \
+        break;
+        switch (K) {
+            IM_XH(3)
+            IM_XH(4)
+            IM_XH(5)
+        }
+#undef IM_XH
+        return IM_LAUNCHED("sweep_xh");
+    }
 #define IM_XK(KK)                                                                                       \
     case KK:                                                                                            \
         if (cf.part) {                                                                                  \
@@ -138,17 +170,28 @@ static int launch_x(double *q, int64_t pitch, int N, int64_t nlines, double lamb
                                nlines, nl, lambda, lamp, hc, rt, dummy, cf);                            \
         }                                                                                               \
         break;
+#define IM_XK_FULL(KK)                                                                                  \
+    case KK:                                                                                            \
+        if (!full) return IM_FAIL(ERROR_INVALID_ARGS, "X lines of %d cells in %d-cell lanes", N, KK);     \
+        if (cf.part)                                                                                    \
+            hipLaunchKernelGGL((sweep_x<M, KK, LIM, true, true>), grid, block, 0, s, q, pitch, N, nlines, \
+                               nl, lambda, lamp, hc, rt, dummy, cf);                                    \
+        else                                                                                            \
+            hipLaunchKernelGGL((sweep_x<M, KK, LIM, true, false>), grid, block, 0, s, q, pitch, N, nlines, \
+                               nl, lambda, lamp, hc, rt, dummy, cf);                                    \
+        break;
     switch (K) {
         IM_XK(1)
         IM_XK(2)
         IM_XK(3)
         IM_XK(4)
-        IM_XK(6)
-        IM_XK(8)
+        IM_XK_FULL(6)
+        IM_XK_FULL(8)
         IM_XK(12)
         IM_XK(16)
     }
 #undef IM_XK
+#undef IM_XK_FULL
     return IM_LAUNCHED("sweep_x");
 }
 

@@ -238,6 +238,79 @@ __global__ __launch_bounds__(256, (K <= 4) ? 2 : 1) void sweep_x(double *__restr
     }
 }
 
+// X sweep of lines of 257-512 cells split into halves, one wave per half. A
+// wave holds its half and the 4 cells beyond each end (the two stages'
+// stencil, periodic within the line, as the whole-line sweep reads them), so
+// it forms its half's outputs alone; the window's outer cells only feed
+// discarded values. The two waves of a line are in one workgroup (4 waves: 2
+// lines), and every load of the workgroup precedes the barrier while every
+// store follows it, so no wave overwrites a cell its partner has still to
+// read. K = 3, 4 or 5 cells per lane keeps two waves per SIMD where a whole
+// line needs 6 or 8 at one.
+template <class M, int K, int LIM, bool CFL>
+__global__ __launch_bounds__(256, 2) void sweep_xh(double *__restrict__ q, int64_t pitch, int N, int64_t nlines,
+                                                  double lambda, const double *__restrict__ lamp, HydroConsts hc,
+                                                  int rt, double *__restrict__ dummy, CflFuse cf)
+{
+    if (lamp) lambda = lamp[0];  // time step kept on the device
+    const int lane = threadIdx.x & 63;
+    const int wib = threadIdx.x >> 6;
+    const int64_t wave = (int64_t)blockIdx.x * 4 + wib;
+    int64_t line = (int64_t)blockIdx.x * 2 + (wib >> 1);
+    const bool lineValid = line < nlines;
+    line = lineValid ? line : nlines - 1;  // waves past the last line still meet the barrier
+    const int part = wib & 1;
+    const int h = (N + 1) >> 1;
+    const int s0 = part ? h : 0;             // first output cell of this half
+    const int Wn = (part ? N - h : h) + 8;   // window: the half and 4 cells beyond each end
+    const int nl = (Wn + K - 1) / K;
+    const bool laneUsed = lane < nl;
+    const int srcL = laneUsed ? (lane == 0 ? nl - 1 : lane - 1) : lane;
+    const int srcR = laneUsed ? (lane == nl - 1 ? 0 : lane + 1) : lane;
+    const int w0 = lane * K;
+    int cnt = Wn - w0;
+    cnt = cnt < K ? cnt : K;
+    if (!laneUsed) cnt = 0;
+    double *row = q + line * (int64_t)N;
+    V5 Q[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        int g = s0 - 4 + w0 + k;  // window cell -> line cell, periodic
+        g = (g < 0) ? g + N : g;
+        g = (g >= N) ? g - N : g;
+        g = (g < N) ? g : N - 1;
+        Q[k] = (k < cnt) ? load_cell<M, 1>(row + g, pitch) : V5{1.0, 1.0, 0.0, 0.0, 0.0};
+    }
+    __syncthreads();
+    V5 F[K], Fm;
+    x_fluxes<M, K, LIM>(hc, rt, Q, cnt, srcL, srcR, F, Fm);
+    V5 U1[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) U1[k] = axpy_update(Q[k], lambda, F[k], k == 0 ? Fm : F[k > 0 ? k - 1 : 0]);
+    x_fluxes<M, K, LIM>(hc, rt, U1, cnt, srcL, srcR, F, Fm);
+    const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nthr = (int64_t)gridDim.x * blockDim.x;
+    double cmin = 1e38;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        if (k < cnt) {
+            const int w = w0 + k;
+            const bool own = lineValid && w >= 4 && w < Wn - 4;
+            const int x = s0 + w - 4;
+            const V5 o = avg_update(add5(U1[k], Q[k]), lambda, F[k], k == 0 ? Fm : F[k > 0 ? k - 1 : 0]);
+            store_cell<M, 1>(own ? row + x : dummy + gtid, own ? pitch : nthr, o);
+            if (CFL && own && x >= cf.lo && x < cf.hi) {
+                const double u = cfl_of_cell<M, 1>(cf.a, o);
+                cmin = (u < cmin) ? u : cmin;
+            }
+        }
+    }
+    if (CFL) {
+        cmin = wave_min(cmin);
+        if (lane == 0) cf.part[wave] = cmin;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Y/Z sweep: register-window march along strided columns
 // ---------------------------------------------------------------------------

@@ -123,8 +123,11 @@ def test_gpu_ctrl_reset_and_info():
     print(text, flush=True)
 
 
+# the last six: X lines of 300-512 cells, swept as two halves by a wave pair
+# (3, 4 and 5 cells per lane, both edges of the 3- and 4-cell classes, odd
+# lengths) or, at exactly 384 and 512 cells, whole with 6 and 8 cells per lane
 SHAPES = [(23, 29, 37), (4, 6, 256), (6, 5, 192), (5, 200, 8), (150, 3, 20), (1, 40, 48), (1, 1, 100),
-          (3, 7, 97)]
+          (3, 7, 97), (3, 4, 300), (2, 3, 368), (2, 3, 384), (2, 3, 401), (2, 2, 497), (2, 2, 512)]
 
 
 @pytest.mark.parametrize("sh", SHAPES)
@@ -187,7 +190,7 @@ def test_limiters_parity(limiter):
 
 @pytest.mark.parametrize("bc", ["const", "mirror", "circular"])
 @pytest.mark.parametrize("sh", [(20, 24, 28), (1, 36, 44), (1, 1, 64), (12, 11, 13), (3, 12, 128), (2, 3, 256),
-                                (3, 12, 32)])
+                                (3, 12, 32), (3, 12, 320), (2, 12, 451)])
 def test_fluidstep_parity_with_boundaries(bc, sh):
     """(12, 11, 13): lines of 11-13 cells, where a mirror face reads cells the
     opposite face writes, so the faces must be set one after the other.

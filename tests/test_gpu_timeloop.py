@@ -27,6 +27,13 @@ def make_fluids(kind):
         fl = [gc.FluidManager(gas, 5 / 3, minmass=1e-8, bc_modes=(abi.circular,) * 6),
               gc.FluidManager(dust, 1.01, minmass=1e-8, bc_modes=(abi.circular,) * 6, cisothermal=ciso)]
         return fl, [abi.METHOD_HLLC, abi.METHOD_HLL], gc.GeometryManager((24, 24, 24), h), abi.LIMITER_OSPRE
+    if kind == "wide":
+        # X lines of 320 cells: the last sweep of an iteration may be the
+        # split-half X kernel with the folded CFL minimum
+        q, h, _ = problems.sedov3d(32)
+        q = np.concatenate([q] * 10, axis=3)
+        fl = [gc.FluidManager(q, 5 / 3, minmass=1e-8, bc_modes=(abi.extrapConstant,) * 6)]
+        return fl, [abi.METHOD_HLLC], gc.GeometryManager((320, 32, 32), h), abi.LIMITER_VANLEER
     if kind == "mirror2d":
         from gpuimogen_amd import ensemble as ens
         _, st = ens.quadrant_problems(1, seed=13, nmin=40, nmax=40)
@@ -43,7 +50,7 @@ def make_fluids(kind):
     raise ValueError(kind)
 
 
-@pytest.mark.parametrize("kind", ["sedov", "twofluid", "mirror2d", "xinjin2d"])
+@pytest.mark.parametrize("kind", ["sedov", "wide", "twofluid", "mirror2d", "xinjin2d"])
 @pytest.mark.parametrize("graph", [False, True])
 def test_device_loop_matches_host_loop(kind, graph):
     from gpuimogen_amd import imogen
