@@ -277,7 +277,9 @@ int performFluidUpdate_1D(MGArray *fluid, FluidStepParams params, ParallelTopolo
  * array's static cells. Here BCSettings.externalData points to one of these
  * (or is NULL: no statics, and a freebalance face is an error); the MEX
  * gateways fill it from boundaryData. The freebalance fields are read from
- * the density array's, the statics of each array from its own. */
+ * the density array's, the statics of each array from its own. The struct
+ * and the arrays it names must stay valid while a call that received the
+ * array, or a device time loop created from it, can still run. */
 typedef struct ImogenBoundaryData {
     MGArray *gravpot;  /* boundaryData.gravpot: one slab of the fluid's dims, or NULL */
     double gamma;      /* boundaryData.gamma */
