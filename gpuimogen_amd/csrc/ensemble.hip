@@ -39,6 +39,8 @@ struct EnsProblem {
     double gamma;
     int owner;  // problem whose clock and time step this entry follows
     int dust;   // gas entries: the entry of the problem's dust, -1 if none
+    int fold;   // 1: the iteration's last sweep folds this entry's CFL minimum (fold_eligible)
+    int pad;
 };
 
 struct EnsItem {
@@ -49,6 +51,48 @@ struct EnsItem {
 };
 
 constexpr int kEnsCflBlocks = 32;  // CFL partial minima per problem
+constexpr int kEnsGhost = 4;       // ghost cells per face set by the boundary kernel
+constexpr double kEnsCflNone = 1e37;  // CFL minimum of an entry with no limiting cell
+
+// CFL minimum folded into the last sweep of an iteration. A problem without
+// dust whose sides both exceed 2 kEnsGhost cells ends its iteration with that
+// sweep and the const boundaries of the swept direction (the source step's
+// boundary calls copy cells too), and every boundary cell then holds a copy of
+// a cell the sweep kept (cells [kEnsGhost, N - kEnsGhost) of the swept
+// direction, every cell of the other): the minimum over the kept cells is the
+// minimum over the whole state. Problems with dust end the iteration with a
+// drag source, and take the two-pass reduction. gas: the entry is a problem's
+// gas (not a dust entry).
+static bool fold_eligible(const EnsProblem &P, bool gas)
+{
+    return gas && P.dust < 0 && P.nx > 2 * kEnsGhost && P.ny > 2 * kEnsGhost;
+}
+
+// Order-preserving map of doubles (NaN aside) to unsigned 64-bit keys, so that
+// waves of any problem can take a problem's minimum with one integer atomic.
+__host__ __device__ inline unsigned long long ens_order_key(double v)
+{
+    unsigned long long b;
+    memcpy(&b, &v, sizeof b);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+__device__ __forceinline__ double ens_order_value(unsigned long long k)
+{
+    const unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    return __longlong_as_double((long long)b);
+}
+
+// Per-wave end of a folding sweep: the wave's minimum into its problem's slot
+// (lane values past the kept cells stay kEnsCflNone), NaN limits counted as
+// the reduction pass counts them. All 64 lanes reach it.
+__device__ __forceinline__ void ens_fold_wave(double cmin, unsigned nan, unsigned long long *slot,
+                                              unsigned long long *err)
+{
+    wave_count(err + ERRW_CFL_NAN, nan);
+    cmin = wave_min(cmin);
+    if ((threadIdx.x & 63) == 0 && cmin < kEnsCflNone) atomicMin(slot, ens_order_key(cmin));
+}
 // X line classes: K cells per lane; lines of up to 256 cells are swept whole
 // (several short lines per wave), longer ones split into two halves swept by
 // a pair of waves (ens_sweep_xh)
@@ -71,13 +115,22 @@ static int x_class_of(int N)
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void ens_cfl_pass1(const EnsProblem *__restrict__ probs,
-                                                     double *__restrict__ part, unsigned long long *__restrict__ err)
+// the CFL arguments of a 2D entry (cflTimestep.cu:292-335, fused sound speed)
+__device__ __forceinline__ CflArgs ens_cfl_args(const EnsProblem &P)
 {
-    const EnsProblem P = probs[blockIdx.y];
+    return CflArgs{P.gamma * (P.gamma - 1), P.hx, P.hy, 1.0, P.hx < P.hy ? P.hx : P.hy, 1, 2, 0};
+}
+
+// partial minima of the entries listed in `entries` (workgroups (block, list index))
+__global__ __launch_bounds__(256) void ens_cfl_pass1(const EnsProblem *__restrict__ probs,
+                                                     const int *__restrict__ entries, double *__restrict__ part,
+                                                     unsigned long long *__restrict__ err)
+{
+    const int ent = entries[blockIdx.y];
+    const EnsProblem P = probs[ent];
     const int64_t n = (int64_t)P.nx * P.ny;
-    CflArgs a{P.gamma * (P.gamma - 1), P.hx, P.hy, 1.0, P.hx < P.hy ? P.hx : P.hy, 1, 2, 0};
-    double m = 1e37;
+    const CflArgs a = ens_cfl_args(P);
+    double m = kEnsCflNone;
     unsigned nan = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const double *c = P.q + i;
@@ -93,7 +146,7 @@ __global__ __launch_bounds__(256) void ens_cfl_pass1(const EnsProblem *__restric
     if (threadIdx.x == 0) {
         double r = w[0];
         for (int k = 1; k < (int)(blockDim.x >> 6); k++) r = (w[k] < r) ? w[k] : r;
-        part[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = r;
+        part[(int64_t)ent * gridDim.x + blockIdx.x] = r;
     }
 }
 
@@ -101,17 +154,26 @@ __global__ __launch_bounds__(256) void ens_cfl_pass1(const EnsProblem *__restric
 // lambda per direction, and the problem's clock t += 2 dt. A limit that is
 // not positive and below 1e36 (every cell NaN, or no cell limiting: the 1e37
 // sentinel) takes no time step (dt = 0), counted in the device error word.
+// useFold: the gas limit of problems whose entry folds comes from the previous
+// iteration's last sweep (fold[p]); every problem's fold slot is then reset for
+// this iteration's last sweep.
 __global__ __launch_bounds__(64) void ens_cfl_pass2(const EnsProblem *__restrict__ probs, const double *part,
                                                     int nb, double cfl, double *__restrict__ lam,
                                                     double *__restrict__ clock, double *__restrict__ dtp,
+                                                    unsigned long long *__restrict__ fold, int useFold,
                                                     unsigned long long *__restrict__ err)
 {
     const int p = blockIdx.x;
     const int de = probs[p].dust;
     // min over fluids of each fluid's halved limit, as TimeManager takes it
     double m = 1e38;
-    for (int k = threadIdx.x; k < nb; k += 64) m = (part[(int64_t)p * nb + k] < m) ? part[(int64_t)p * nb + k] : m;
-    m = wave_min(m);
+    if (useFold && probs[p].fold) {
+        m = ens_order_value(fold[p]);
+    } else {
+        for (int k = threadIdx.x; k < nb; k += 64)
+            m = (part[(int64_t)p * nb + k] < m) ? part[(int64_t)p * nb + k] : m;
+        m = wave_min(m);
+    }
     double md = 1e38;
     if (de >= 0) {
         for (int k = threadIdx.x; k < nb; k += 64)
@@ -134,6 +196,7 @@ __global__ __launch_bounds__(64) void ens_cfl_pass2(const EnsProblem *__restrict
         }
         dtp[p] = dt;
         clock[p] += 2 * dt;
+        fold[p] = ens_order_key(kEnsCflNone);
     }
 }
 
@@ -152,12 +215,14 @@ __device__ __forceinline__ HydroConsts ens_consts(double gamma)
 }
 
 // X sweep of whole lines (K <= 4 cells per lane, two waves per SIMD); one
-// work item (problem, first line) per wave
-template <int K, int LIM>
+// work item (problem, first line) per wave. CFL: fold the CFL minimum of the
+// output cells into fold[problem] (entries with P.fold set).
+template <int K, int LIM, bool CFL>
 __global__ __launch_bounds__(256, 2) void ens_sweep_x(const EnsProblem *__restrict__ probs,
                                                    const EnsItem *__restrict__ work, int nwork,
                                                    const double *__restrict__ lam, int rt,
-                                                   double *__restrict__ dummy)
+                                                   double *__restrict__ dummy, unsigned long long *__restrict__ fold,
+                                                   unsigned long long *__restrict__ err)
 {
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     if (wave >= nwork) return;
@@ -196,13 +261,23 @@ __global__ __launch_bounds__(256, 2) void ens_sweep_x(const EnsProblem *__restri
     const int64_t nthr = (int64_t)gridDim.x * blockDim.x;
     double *dst = own ? b : dummy + gtid;
     const int64_t st = own ? P.pitch : nthr;
+    const bool fl = CFL && own && P.fold;
+    const CflArgs ca = ens_cfl_args(P);
+    double cmin = kEnsCflNone;
+    unsigned nan = 0;
 #pragma unroll
     for (int k = 0; k < K; k++) {
         if (k < cnt) {
             const V5 o = avg_update(add5(U1[k], Q[k]), lambda, F[k], k == 0 ? Fm : F[k > 0 ? k - 1 : 0]);
             store_cell<Hllc, 1>(own ? dst + k : dst, st, o);
+            if (fl && x0 + k >= kEnsGhost && x0 + k < N - kEnsGhost) {
+                const double u = cfl_of_cell<Hllc, 1>(ca, o);
+                cmin = (u < cmin) ? u : cmin;
+                nan += (u != u);
+            }
         }
     }
+    if (CFL) ens_fold_wave(cmin, nan, fold + P.owner, err);
 }
 
 // X sweep of lines split into two halves, one wave per half. A wave holds its
@@ -213,11 +288,13 @@ __global__ __launch_bounds__(256, 2) void ens_sweep_x(const EnsProblem *__restri
 // the workgroup precedes the barrier while every store follows it, so no wave
 // overwrites a cell its partner has still to read. One work item (problem,
 // line) per wave pair; lines of at most 2 (64 K - 8) cells.
-template <int K, int LIM>
+template <int K, int LIM, bool CFL>
 __global__ __launch_bounds__(256, 2) void ens_sweep_xh(const EnsProblem *__restrict__ probs,
                                                     const EnsItem *__restrict__ work, int nwork,
                                                     const double *__restrict__ lam, int rt,
-                                                    double *__restrict__ dummy)
+                                                    double *__restrict__ dummy,
+                                                    unsigned long long *__restrict__ fold,
+                                                    unsigned long long *__restrict__ err)
 {
     const int lane = threadIdx.x & 63;
     const int wib = threadIdx.x >> 6;
@@ -260,15 +337,26 @@ __global__ __launch_bounds__(256, 2) void ens_sweep_xh(const EnsProblem *__restr
     x_fluxes<Hllc, K, LIM>(hc, rt, U1, cnt, srcL, srcR, F, Fm);
     const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nthr = (int64_t)gridDim.x * blockDim.x;
+    const bool fl = CFL && P.fold;
+    const CflArgs ca = ens_cfl_args(P);
+    double cmin = kEnsCflNone;
+    unsigned nan = 0;
 #pragma unroll
     for (int k = 0; k < K; k++) {
         if (k < cnt) {
             const int w = w0 + k;
             const bool own = itemValid && w >= 4 && w < Wn - 4;
+            const int x = s0 + w - 4;
             const V5 o = avg_update(add5(U1[k], Q[k]), lambda, F[k], k == 0 ? Fm : F[k > 0 ? k - 1 : 0]);
-            store_cell<Hllc, 1>(own ? row + (s0 + w - 4) : dummy + gtid, own ? P.pitch : nthr, o);
+            store_cell<Hllc, 1>(own ? row + x : dummy + gtid, own ? P.pitch : nthr, o);
+            if (fl && own && x >= kEnsGhost && x < N - kEnsGhost) {
+                const double u = cfl_of_cell<Hllc, 1>(ca, o);
+                cmin = (u < cmin) ? u : cmin;
+                nan += (u != u);
+            }
         }
     }
+    if (CFL) ens_fold_wave(cmin, nan, fold + P.owner, err);
 }
 
 // workgroups of an X class launch over n work items
@@ -278,11 +366,12 @@ static int64_t x_blocks(int cls, int64_t n)
 }
 
 // Y sweep: column march, one work item (problem, first column, S) per wave
-template <int LIM>
+template <int LIM, bool CFL>
 __global__ __launch_bounds__(256) void ens_sweep_y(const EnsProblem *__restrict__ probs,
                                                    const EnsItem *__restrict__ work, int nwork,
                                                    const double *__restrict__ lam, int rt,
-                                                   double *__restrict__ defer)
+                                                   double *__restrict__ defer, unsigned long long *__restrict__ fold,
+                                                   unsigned long long *__restrict__ err)
 {
     __shared__ double sring[4][4][5][64];
     const int wib = threadIdx.x >> 6;
@@ -308,10 +397,14 @@ __global__ __launch_bounds__(256) void ens_sweep_y(const EnsProblem *__restrict_
     double *col = P.q + (colValid ? x : 0);
     const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nthr = (int64_t)gridDim.x * blockDim.x;
-    const CflFuse nocfl{};
-    double cmin = 0.0;
-    march_body<Hllc, 2, LIM>(col, P.pitch, (int64_t)P.nx, N, seglen, j0, L, lambda, hc, rt, defer, gtid, nthr,
-                             sring[wib], lane, nocfl, cmin);
+    // cells [kEnsGhost, N - kEnsGhost) of the column fold their CFL limits
+    const bool fl = CFL && P.fold;
+    const CflFuse cf{nullptr, ens_cfl_args(P), fl ? kEnsGhost : 0, fl ? N - kEnsGhost : 0, 0};
+    double cmin = kEnsCflNone;
+    unsigned nan = 0;
+    march_body<Hllc, 2, LIM, CFL>(col, P.pitch, (int64_t)P.nx, N, seglen, j0, L, lambda, hc, rt, defer, gtid,
+                                  nthr, sring[wib], lane, cf, cmin, nan);
+    if (CFL) ens_fold_wave(cmin, nan, fold + P.owner, err);
 }
 
 // constant-extrapolation ghost cells (cudaStatics.cu:1339-1366, 1440-1466 with
@@ -389,6 +482,11 @@ struct EnsembleImpl {
     double *pool = nullptr;
     EnsProblem *dprobs = nullptr;
     double *lam = nullptr, *clock = nullptr, *cflPart = nullptr;
+    // folded CFL minima (one key per problem) and the entries the reduction
+    // pass reads: every entry (cflList[0..NE)), then those that do not fold
+    unsigned long long *fold = nullptr;
+    int *cflList = nullptr;
+    int nNoFold = 0;
     EnsItem *xwork[kXClasses] = {nullptr};
     int nx_work[kXClasses] = {0};
     EnsItem *ywork = nullptr;
@@ -400,7 +498,7 @@ struct EnsembleImpl {
     // streams (default priority) and join back, parallel branches of the graph
     hipStream_t side[2] = {nullptr, nullptr};
     hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
-    hipGraphExec_t graph[6] = {nullptr};
+    hipGraphExec_t graph[2][6] = {{nullptr}};  // [reduction reads the folded minima][sweep order]
     int64_t cells = 0;
     // dust (imogen_ensemble_set_dust)
     int Pd = 0;
@@ -418,10 +516,11 @@ struct EnsembleImpl {
 
 static void free_graphs(EnsembleImpl *e)
 {
-    for (int k = 0; k < 6; k++) {
-        if (e->graph[k]) (void)hipGraphExecDestroy(e->graph[k]);
-        e->graph[k] = nullptr;
-    }
+    for (int f = 0; f < 2; f++)
+        for (int k = 0; k < 6; k++) {
+            if (e->graph[f][k]) (void)hipGraphExecDestroy(e->graph[f][k]);
+            e->graph[f][k] = nullptr;
+        }
 }
 
 static int build_work(EnsembleImpl *e)
@@ -430,6 +529,23 @@ static int build_work(EnsembleImpl *e)
     if (!st) return ERROR_CUDA_BLEW_UP;
     // the tables below are replaced while no kernel can still read them
     if (IM_HIP(hipStreamSynchronize(st->stream))) return ERROR_CUDA_BLEW_UP;
+    // which entries fold their CFL minimum into the last sweep; the reduction
+    // pass lists every entry, then the entries that do not fold
+    std::vector<int> cl;
+    for (int p = 0; p < e->NE; p++) {
+        e->host[p].fold = fold_eligible(e->host[p], p < e->P) ? 1 : 0;
+        cl.push_back(p);
+    }
+    for (int p = 0; p < e->NE; p++)
+        if (!e->host[p].fold) cl.push_back(p);
+    e->nNoFold = (int)cl.size() - e->NE;
+    if (IM_HIP(hipMemcpy(e->dprobs, e->host.data(), e->NE * sizeof(EnsProblem), hipMemcpyHostToDevice)))
+        return ERROR_CUDA_BLEW_UP;
+    if (e->cflList) (void)hipFree(e->cflList);
+    e->cflList = nullptr;
+    if (IM_HIP(hipMalloc(&e->cflList, cl.size() * sizeof(int)))) return ERROR_CUDA_BLEW_UP;
+    if (IM_HIP(hipMemcpy(e->cflList, cl.data(), cl.size() * sizeof(int), hipMemcpyHostToDevice)))
+        return ERROR_CUDA_BLEW_UP;
     std::vector<EnsItem> xw[kXClasses], yw;
     for (int c = 0; c < kXClasses; c++) e->xcells[c] = 0;
     for (int p = 0; p < e->NE; p++) {
@@ -504,9 +620,12 @@ static int ens_source(EnsembleImpl *e, double fraction, hipStream_t s)
     return SUCCESSFUL;
 }
 
-template <int LIM>
+// one sweep of every entry in direction dir, then that direction's boundaries;
+// CFL: the sweep folds the CFL minima of the entries that fold
+template <int LIM, bool CFL>
 static int ens_sweep(EnsembleImpl *e, int dir, int rt, hipStream_t s)
 {
+    unsigned long long *err = state()->err;
     if (dir == 1) {
         // classes to the three streams, largest first to the least loaded
         int order[kXClasses], nc = 0;
@@ -524,11 +643,11 @@ static int ens_sweep(EnsembleImpl *e, int dir, int rt, hipStream_t s)
             hipStream_t sx = sl == 0 ? s : e->side[sl - 1];
             const unsigned blocks = (unsigned)x_blocks(c, e->nx_work[c]);
 #define IM_EX(KK)                                                                                              \
-    hipLaunchKernelGGL((ens_sweep_x<KK, LIM>), dim3(blocks), dim3(256), 0, sx, e->dprobs, e->xwork[c],          \
-                       e->nx_work[c], e->lam, rt, e->xdummy)
+    hipLaunchKernelGGL((ens_sweep_x<KK, LIM, CFL>), dim3(blocks), dim3(256), 0, sx, e->dprobs, e->xwork[c],     \
+                       e->nx_work[c], e->lam, rt, e->xdummy, e->fold, err)
 #define IM_EXH(KK)                                                                                             \
-    hipLaunchKernelGGL((ens_sweep_xh<KK, LIM>), dim3(blocks), dim3(256), 0, sx, e->dprobs, e->xwork[c],         \
-                       e->nx_work[c], e->lam, rt, e->xdummy)
+    hipLaunchKernelGGL((ens_sweep_xh<KK, LIM, CFL>), dim3(blocks), dim3(256), 0, sx, e->dprobs, e->xwork[c],    \
+                       e->nx_work[c], e->lam, rt, e->xdummy, e->fold, err)
             if (kXClassSplit[c]) {
                 switch (kXClassK[c]) {
                 case 3: IM_EXH(3); break;
@@ -554,40 +673,55 @@ static int ens_sweep(EnsembleImpl *e, int dir, int rt, hipStream_t s)
         }
     } else {
         const unsigned blocks = (unsigned)((e->ny_work + 3) / 4);
-        hipLaunchKernelGGL((ens_sweep_y<LIM>), dim3(blocks), dim3(256), 0, s, e->dprobs, e->ywork, e->ny_work,
-                           e->lam, rt, e->ydefer);
+        hipLaunchKernelGGL((ens_sweep_y<LIM, CFL>), dim3(blocks), dim3(256), 0, s, e->dprobs, e->ywork,
+                           e->ny_work, e->lam, rt, e->ydefer, e->fold, err);
         int rc = IM_LAUNCHED("ens_sweep_y");
         if (rc) return rc;
     }
     return ens_bc(e, dir, s, 1);
 }
 
+template <bool CFL>
+static int ens_sweep_lim(EnsembleImpl *e, int dir, hipStream_t s)
+{
+    switch (e->limiter) {
+    case LIMITER_OSPRE: return ens_sweep<0, CFL>(e, dir, 0, s);
+    case LIMITER_VANLEER: return ens_sweep<1, CFL>(e, dir, 1, s);
+    default: return ens_sweep<-1, CFL>(e, dir, e->limiter, s);
+    }
+}
+
 // one iteration (imogen.m:115-139): CFL over all fluids, source(1/2),
 // fluidstep(+1), source(1), fluidstep(-1), source(1/2); the sources act only
-// when some problems carry dust
-static int ens_iteration(EnsembleImpl *e, int it, hipStream_t s)
+// when some problems carry dust. The last sweep folds the CFL minima of the
+// entries that fold; useFold: this iteration's CFL takes those minima (left by
+// the previous iteration, whose state this one starts from) instead of
+// reducing the entries again.
+static int ens_iteration(EnsembleImpl *e, int it, hipStream_t s, bool useFold)
 {
     static const int kFluxcall[3][6] = {{1, 2, 1, 2, 3, 3}, {3, 1, 2, 3, 1, 2}, {2, 3, 3, 1, 2, 1}};
     unsigned long long *err = state()->err;
-    hipLaunchKernelGGL(ens_cfl_pass1, dim3(kEnsCflBlocks, (unsigned)e->NE), dim3(256), 0, s, e->dprobs, e->cflPart,
-                       err);
-    int rc = IM_LAUNCHED("ens_cfl_pass1");
-    if (rc) return rc;
+    const int npass = useFold ? e->nNoFold : e->NE;
+    const int *list = e->cflList + (useFold ? e->NE : 0);
+    int rc;
+    if (npass > 0) {
+        hipLaunchKernelGGL(ens_cfl_pass1, dim3(kEnsCflBlocks, (unsigned)npass), dim3(256), 0, s, e->dprobs, list,
+                           e->cflPart, err);
+        if ((rc = IM_LAUNCHED("ens_cfl_pass1"))) return rc;
+    }
     hipLaunchKernelGGL(ens_cfl_pass2, dim3((unsigned)e->P), dim3(64), 0, s, e->dprobs, e->cflPart, kEnsCflBlocks,
-                       e->cfl, e->lam, e->clock, e->dtp, err);
-    rc = IM_LAUNCHED("ens_cfl_pass2");
-    if (rc) return rc;
+                       e->cfl, e->lam, e->clock, e->dtp, e->fold, useFold ? 1 : 0, err);
+    if ((rc = IM_LAUNCHED("ens_cfl_pass2"))) return rc;
     if ((rc = ens_source(e, 0.5, s))) return rc;
     for (int dirn = 1; dirn >= -1; dirn -= 2) {
-        int sweep = (it + 3 * (dirn > 0)) % 6;
+        const int sweep = (it + 3 * (dirn > 0)) % 6;
+        int last = 0;
+        for (int n = 0; n < 3; n++)
+            if (kFluxcall[n][sweep] != 3) last = n;
         for (int n = 0; n < 3; n++) {
             const int d = kFluxcall[n][sweep];
             if (d == 3) continue;  // 2D problems
-            switch (e->limiter) {
-            case LIMITER_OSPRE: rc = ens_sweep<0>(e, d, 0, s); break;
-            case LIMITER_VANLEER: rc = ens_sweep<1>(e, d, 1, s); break;
-            default: rc = ens_sweep<-1>(e, d, e->limiter, s); break;
-            }
+            rc = (dirn < 0 && n == last) ? ens_sweep_lim<true>(e, d, s) : ens_sweep_lim<false>(e, d, s);
             if (rc) return rc;
         }
         if ((rc = ens_source(e, dirn > 0 ? 1.0 : 0.5, s))) return rc;
@@ -633,7 +767,7 @@ int imogen_ensemble_create(int nprob, const int *nx, const int *ny, const double
         const int64_t pitch = (n + 31) / 32 * 32;  // 256-byte slabs
         off[p] = total;
         total += 5 * pitch;
-        e->host.push_back(EnsProblem{nullptr, pitch, nx[p], ny[p], hx[p], hy[p], gamma, p, -1});
+        e->host.push_back(EnsProblem{nullptr, pitch, nx[p], ny[p], hx[p], hy[p], gamma, p, -1, 0, 0});
         e->cells += n;
         e->segS.push_back(ny[p] >= 256 ? 2 : 1);
     }
@@ -653,6 +787,12 @@ int imogen_ensemble_create(int nprob, const int *nx, const int *ny, const double
     if (!rc && IM_HIP(hipMemset(e->clock, 0, nprob * sizeof(double)))) rc = ERROR_CUDA_BLEW_UP;
     if (!rc && IM_HIP(hipMalloc(&e->dtp, nprob * sizeof(double)))) rc = ERROR_CUDA_BLEW_UP;
     if (!rc && IM_HIP(hipMalloc(&e->cflPart, (size_t)nprob * kEnsCflBlocks * sizeof(double)))) rc = ERROR_CUDA_BLEW_UP;
+    if (!rc && IM_HIP(hipMalloc(&e->fold, nprob * sizeof(unsigned long long)))) rc = ERROR_CUDA_BLEW_UP;
+    if (!rc) {
+        const std::vector<unsigned long long> none(nprob, ens_order_key(kEnsCflNone));
+        if (IM_HIP(hipMemcpy(e->fold, none.data(), nprob * sizeof(unsigned long long), hipMemcpyHostToDevice)))
+            rc = ERROR_CUDA_BLEW_UP;
+    }
     if (!rc) rc = build_work(e);
     if (rc) {
         imogen_ensemble_destroy(E);
@@ -829,16 +969,19 @@ int imogen_ensemble_run(ImogenEnsemble *E, int firstIteration, int iterations, i
     if (!st) return ERROR_CUDA_BLEW_UP;
     for (int k = 0; k < iterations; k++) {
         const int it = firstIteration + k;
+        // the first iteration of a call reduces every entry (the state may
+        // have been set since the last call); later ones take folded minima
+        const bool useFold = k > 0;
         if (!useGraph) {
-            int rc = ens_iteration(e, it, st->stream);
+            int rc = ens_iteration(e, it, st->stream, useFold);
             if (rc) return rc;
             continue;
         }
-        hipGraphExec_t &g = e->graph[it % 6];
+        hipGraphExec_t &g = e->graph[useFold][it % 6];
         if (!g) {
             hipGraph_t graph;
             if (IM_HIP(hipStreamBeginCapture(st->stream, hipStreamCaptureModeThreadLocal))) return ERROR_CUDA_BLEW_UP;
-            int rc = ens_iteration(e, it, st->stream);
+            int rc = ens_iteration(e, it, st->stream, useFold);
             hipError_t ec = hipStreamEndCapture(st->stream, &graph);
             if (rc) return rc;
             if (IM_HIP(ec)) return ERROR_CUDA_BLEW_UP;
@@ -884,9 +1027,9 @@ int imogen_ensemble_autotune(ImogenEnsemble *E, int trials, int *chosen)
             for (int p = 0; p < e->P; p++)
                 if (cls_of(e->host[p].ny) == cls) e->segS[p] = cand[ci];
             int rc = build_work(e);
-            if (!rc) rc = ens_iteration(e, 0, st->stream);  // warm
+            if (!rc) rc = ens_iteration(e, 0, st->stream, false);  // warm
             if (!rc) rc = IM_HIP(hipEventRecord(a, st->stream));
-            for (int k = 0; k < trials && !rc; k++) rc = ens_iteration(e, k, st->stream);
+            for (int k = 0; k < trials && !rc; k++) rc = ens_iteration(e, k, st->stream, true);
             if (!rc) rc = IM_HIP(hipEventRecord(b, st->stream));
             if (!rc) rc = IM_HIP(hipEventSynchronize(b));
             float ms = 0;
@@ -921,7 +1064,7 @@ int imogen_ensemble_destroy(ImogenEnsemble *E)
     free_graphs(e);
     for (int c = 0; c < kXClasses; c++)
         if (e->xwork[c]) (void)hipFree(e->xwork[c]);
-    void *bufs[] = {e->pool,   e->dprobs,   e->lam,       e->clock, e->cflPart, e->ywork, e->xdummy,
+    void *bufs[] = {e->pool,   e->dprobs,   e->lam,       e->clock, e->cflPart, e->fold, e->cflList, e->ywork, e->xdummy,
                     e->ydefer, e->dpool,    e->dtp,       e->dragTab, e->dragItems, e->queue, e->qcount};
     for (void *p : bufs)
         if (p) (void)hipFree(p);

@@ -27,7 +27,7 @@ int fail(int status, const char *file, int line, const char *fmt, ...);
 enum DeviceErrorSlot {
     ERRW_DRAG_INVALID = 0,  // drag cells with a density, |dv| or (moving) internal energy not positive/finite
     ERRW_DRAG_BUDGET = 1,   // cells whose adaptive drag integration ran out of steps
-    ERRW_CFL_NAN = 2,       // cells whose CFL limit is NaN (a standalone CFL pass)
+    ERRW_CFL_NAN = 2,       // cells whose CFL limit is NaN (a standalone CFL pass, the ensemble's folded CFL)
     ERRW_TIMESTEP = 3,      // time steps not taken: tau not positive and finite (device clocks)
     ERRW_SLOTS = 8
 };

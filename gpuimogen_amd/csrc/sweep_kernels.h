@@ -318,11 +318,14 @@ __global__ __launch_bounds__(256, 2) void sweep_xh(double *__restrict__ q, int64
 // walks cells [j0, j0+L) of the column starting at `col` (stride sstride, N
 // cells, periodic), the wave's stencil ring in LDS at `ring`. defer/gtid/nthr
 // address this lane's deferred-output slots. seglen is uniform over the wave.
+// CFL: cmin takes the minimum CFL limit of the outputs at column cells
+// [cf.lo, cf.hi), nanc counts the NaN limits among them.
 template <class M, int DIR, int LIM, bool CFL = false, bool NT = false>
 __device__ __forceinline__ void march_body(double *__restrict__ col, int64_t pitch, int64_t sstride, int N,
                                            int seglen, int j0, int L, double lambda, const HydroConsts &hc,
                                            int rt, double *__restrict__ defer, int64_t gtid, int64_t nthr,
-                                           double (*ring)[5][64], int lane, const CflFuse &cf, double &cmin)
+                                           double (*ring)[5][64], int lane, const CflFuse &cf, double &cmin,
+                                           unsigned &nanc)
 {
     // wrapped sweep index of the first load (m = j0 - 4), in [0, N)
     int mw = (j0 - 4) % N;
@@ -393,6 +396,7 @@ __device__ __forceinline__ void march_body(double *__restrict__ col, int64_t pit
         if (CFL && oi >= 0 && oi < L && j0 + oi >= cf.lo && j0 + oi < cf.hi) {
             const double u = cfl_of_cell<M, DIR>(cf.a, o);
             cmin = (u < cmin) ? u : cmin;
+            nanc += (u != u);
         }
         const bool direct = (oi >= 4) & (oi < L);
         const int slot = ((oi >= 0) & (oi < 4) & (oi < L)) ? oi : 4;
@@ -473,8 +477,9 @@ __global__ __launch_bounds__(256) void sweep_march(double *__restrict__ q, int64
     const int64_t nthr = (int64_t)gridDim.x * gridDim.y * blockDim.x;
     __shared__ double sring[4][4][5][64];  // [wave in block][slot][variable][lane]
     double cmin = 1e38;
+    unsigned nanc = 0;  // not reported by the single-array fold
     march_body<M, DIR, LIM, CFL, NT>(col, pitch, sstride, N, seglen, j0, L, lambda, hc, rt, defer, gtid, nthr,
-                                     sring[wib], lane, cf, cmin);
+                                     sring[wib], lane, cf, cmin, nanc);
     if (CFL) {
         cmin = wave_min(cmin);
         if (lane == 0) cf.part[waveId] = cmin;
