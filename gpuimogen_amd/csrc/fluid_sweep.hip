@@ -41,6 +41,10 @@ namespace imogen {
 //   96^3  S=1 0.140  S=2 0.077  S=4 0.051  -> 4     320^3 S=1 1.04  S=2 0.79  S=4 0.79  -> 4
 //   160^3 S=1 0.216  S=2 0.201  S=4 0.125  -> 4     384^3 S=1 1.37  S=2 1.24  S=4 1.26  -> 4
 //   192^3 S=1 0.249  S=2 0.236  S=4 0.203  -> 4     256^3 S=1 0.315 S=2 0.309..0.329 S=4 0.324 -> 1
+// Counts that do not divide 64 leave a wave's row segment off the 128-byte
+// lines: S = 3, 6 (21, 10 columns) ran 1.5-2x slower than S = 2 or 4 (smooth
+// state, 160^3-384^3; 192^3: S=3 0.374, S=6 0.395 against S=4 0.197), and
+// S = 8 (8 columns, 64 bytes) 1.3-1.5x slower.
 static int march_segments(int64_t columns, int N)
 {
     int best = 1;
