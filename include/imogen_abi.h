@@ -440,7 +440,10 @@ int imogen_ensemble_set_state(ImogenEnsemble *e, int p, const double *host);
 int imogen_ensemble_get_state(ImogenEnsemble *e, int p, double *host);
 /* Runs iterations firstIteration .. firstIteration+iterations-1 (the
  * iteration number picks the sweep order as stepNumber does); useGraph != 0
- * captures one HIP graph per sweep-order residue and replays it. Waits for
+ * captures one HIP graph per sweep-order residue and replays it. The first
+ * iteration of a call reduces every problem's CFL limit over its whole state;
+ * later ones take, for problems without dust, the minimum the previous
+ * iteration's last sweep folded in (the same value, bitwise). Waits for
  * the last iteration and returns ERROR_CRASH when the kernels met invalid
  * states (imogen_check_errors); a problem whose CFL limit is not positive and
  * finite takes no time step in that iteration. */
