@@ -240,11 +240,12 @@ def run_ensemble(args):
                     "sweep_bytes_GBs": sweep_gbs, "sweep_hbm_frac": sweep_gbs / HBM_PEAK_GBS,
                     "sim_time_advanced_range": [float(adv.min()), float(adv.max())]}
 
-    # the march segments are tuned on the hydrodynamics alone (the drag
-    # source would only add noise to the timings)
+    # the march segments and X cells per lane are tuned on the hydrodynamics
+    # alone (the drag source would only add noise to the timings)
     load(False)
     t0 = time.perf_counter()
     chosen = e.autotune(trials=2)
+    xlanes = e.x_lanes()
     tune_s = time.perf_counter() - t0
     load(False)
     _, hydro = timed("hydrodynamics", cells)
@@ -272,6 +273,8 @@ def run_ensemble(args):
             "warmup": args.warmup, "seconds": head["seconds"], "cell_updates_per_s": head["cell_updates_per_s"],
             "ms_per_iteration": head["ms_per_iteration"],
             "segments_by_ny_class": {"<=128": chosen[0], "<=256": chosen[1], "<=512": chosen[2]},
+            "x_cells_per_lane_by_nx_class": {f"<={b}": (f"{k} split" if sp else k) for b, (k, sp)
+                                             in zip((64, 128, 192, 256, 368, 496, 512), xlanes) if k},
             "autotune_s": tune_s, "finite": ok, "hydro": hydro, "dust": dusty}
 
 

@@ -187,6 +187,7 @@ SIGNATURES = {
     "imogen_ensemble_run": (c_int, [c_void_p, c_int, c_int, c_int]),
     "imogen_ensemble_times": (c_int, [c_void_p, _DP]),
     "imogen_ensemble_autotune": (c_int, [c_void_p, c_int, _IP]),
+    "imogen_ensemble_x_lanes": (c_int, [c_void_p, _IP, _IP]),
     "imogen_ensemble_destroy": (c_int, [c_void_p]),
     "imogen_ensemble_set_dust": (c_int, [c_void_p, _IP, c_double, _P(ThermoDetails), _P(ThermoDetails), c_int]),
     "imogen_ensemble_set_dust_state": (c_int, [c_void_p, c_int, _DP]),

@@ -112,6 +112,19 @@ static int x_class_of(int N)
     return -1;
 }
 
+// Class of lines of N cells swept with K cells per lane, whole or split into
+// halves (lines above 256 cells only); -1 when the line or half window does
+// not fit 64 lanes.
+static int x_class_for(int N, int K, bool split)
+{
+    if (split) {
+        if (N <= 256 || N > 512 || (N + 1) / 2 + 8 > 64 * K) return -1;
+        return K == 3 ? 4 : (K == 4 ? 5 : (K == 5 ? 6 : -1));
+    }
+    if (K < 1 || K > 4 || N > 64 * K) return -1;
+    return K - 1;
+}
+
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
@@ -479,6 +492,7 @@ struct EnsembleImpl {
     int limiter = 0;
     std::vector<EnsProblem> host;
     std::vector<int> segS;  // chosen Y segments per problem
+    std::vector<int> xcls;  // X line class per problem (x_class_of unless tuned)
     double *pool = nullptr;
     EnsProblem *dprobs = nullptr;
     double *lam = nullptr, *clock = nullptr, *cflPart = nullptr;
@@ -550,7 +564,7 @@ static int build_work(EnsembleImpl *e)
     for (int c = 0; c < kXClasses; c++) e->xcells[c] = 0;
     for (int p = 0; p < e->NE; p++) {
         const EnsProblem &P = e->host[p];
-        const int cls = x_class_of(P.nx);
+        const int cls = e->xcls[P.owner];
         const int K = kXClassK[cls];
         const int nl = (P.nx + K - 1) / K;
         const int lpw = kXClassSplit[cls] ? 1 : 64 / nl;  // split classes: one item per line
@@ -770,6 +784,7 @@ int imogen_ensemble_create(int nprob, const int *nx, const int *ny, const double
         e->host.push_back(EnsProblem{nullptr, pitch, nx[p], ny[p], hx[p], hy[p], gamma, p, -1, 0, 0});
         e->cells += n;
         e->segS.push_back(ny[p] >= 256 ? 2 : 1);
+        e->xcls.push_back(x_class_of(nx[p]));
     }
     int rc = SUCCESSFUL;
     if (IM_HIP(hipMalloc(&e->pool, (size_t)total * sizeof(double)))) rc = ERROR_CUDA_BLEW_UP;
@@ -1011,35 +1026,34 @@ int imogen_ensemble_autotune(ImogenEnsemble *E, int trials, int *chosen)
     DeviceState *st = state();
     if (!st) return ERROR_CUDA_BLEW_UP;
     if (trials < 1) trials = 1;
-    // Y-march segments per size class (ny <= 128, <= 256, <= 512) among {1, 2, 4},
-    // one class at a time, timing whole iterations with the other classes fixed.
-    // The state advances during tuning; callers tune before loading their data
-    // or reload it afterwards.
-    static const int cand[3] = {1, 2, 4};
-    auto cls_of = [](int ny) { return ny <= 128 ? 0 : (ny <= 256 ? 1 : 2); };
+    // Candidates are timed over whole iterations, one size class at a time with
+    // the others fixed. The state advances during tuning; callers tune before
+    // loading their data or reload it afterwards.
     hipEvent_t a, b;
     if (IM_HIP(hipEventCreate(&a)) || IM_HIP(hipEventCreate(&b))) return ERROR_CUDA_BLEW_UP;
-    int best[3];
-    for (int cls = 0; cls < 3; cls++) {
+    auto timed = [&](float *ms) {
+        int rc = build_work(e);
+        if (!rc) rc = ens_iteration(e, 0, st->stream, false);  // warm
+        if (!rc) rc = IM_HIP(hipEventRecord(a, st->stream));
+        for (int k = 0; k < trials && !rc; k++) rc = ens_iteration(e, k, st->stream, true);
+        if (!rc) rc = IM_HIP(hipEventRecord(b, st->stream));
+        if (!rc) rc = IM_HIP(hipEventSynchronize(b));
+        if (!rc) rc = IM_HIP(hipEventElapsedTime(ms, a, b));
+        return rc;
+    };
+    int rc = SUCCESSFUL;
+    // Y-march segments per size class (ny <= 128, <= 256, <= 512) among {1, 2, 4}
+    static const int cand[3] = {1, 2, 4};
+    auto cls_of = [](int ny) { return ny <= 128 ? 0 : (ny <= 256 ? 1 : 2); };
+    int best[3] = {1, 1, 1};
+    for (int cls = 0; cls < 3 && !rc; cls++) {
         float bestMs = 1e30f;
-        best[cls] = 1;
-        for (int ci = 0; ci < 3; ci++) {
+        for (int ci = 0; ci < 3 && !rc; ci++) {
             for (int p = 0; p < e->P; p++)
                 if (cls_of(e->host[p].ny) == cls) e->segS[p] = cand[ci];
-            int rc = build_work(e);
-            if (!rc) rc = ens_iteration(e, 0, st->stream, false);  // warm
-            if (!rc) rc = IM_HIP(hipEventRecord(a, st->stream));
-            for (int k = 0; k < trials && !rc; k++) rc = ens_iteration(e, k, st->stream, true);
-            if (!rc) rc = IM_HIP(hipEventRecord(b, st->stream));
-            if (!rc) rc = IM_HIP(hipEventSynchronize(b));
             float ms = 0;
-            if (!rc) rc = IM_HIP(hipEventElapsedTime(&ms, a, b));
-            if (rc) {
-                (void)hipEventDestroy(a);
-                (void)hipEventDestroy(b);
-                return rc;
-            }
-            if (ms < bestMs) {
+            rc = timed(&ms);
+            if (!rc && ms < bestMs) {
                 bestMs = ms;
                 best[cls] = cand[ci];
             }
@@ -1047,12 +1061,58 @@ int imogen_ensemble_autotune(ImogenEnsemble *E, int trials, int *chosen)
         for (int p = 0; p < e->P; p++)
             if (cls_of(e->host[p].ny) == cls) e->segS[p] = best[cls];
     }
+    // X cells per lane per line-length group (the groups of x_class_of): the
+    // group's least K and up to two more where the lines (whole lines: several
+    // per wave) or half windows still fit 64 lanes
+    for (int g = 0; g < kXClasses && !rc; g++) {
+        std::vector<int> members;
+        for (int p = 0; p < e->P; p++)
+            if (x_class_of(e->host[p].nx) == g) members.push_back(p);
+        if (members.empty()) continue;
+        std::vector<int> cands;
+        for (int K = kXClassK[g]; K <= kXClassK[g] + 2; K++) {
+            bool ok = true;
+            for (int p : members) ok = ok && x_class_for(e->host[p].nx, K, kXClassSplit[g]) >= 0;
+            if (ok) cands.push_back(x_class_for(e->host[members[0]].nx, K, kXClassSplit[g]));
+        }
+        if (cands.size() < 2) continue;
+        float bestMs = 1e30f;
+        int bestCls = g;
+        for (size_t ci = 0; ci < cands.size() && !rc; ci++) {
+            for (int p : members) e->xcls[p] = cands[ci];
+            float ms = 0;
+            rc = timed(&ms);
+            if (!rc && ms < bestMs) {
+                bestMs = ms;
+                bestCls = cands[ci];
+            }
+        }
+        for (int p : members) e->xcls[p] = bestCls;
+    }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
+    if (rc) return rc;
     if (chosen)
         for (int c = 0; c < 3; c++) chosen[c] = best[c];
     if (IM_HIP(hipMemset(e->clock, 0, e->P * sizeof(double)))) return ERROR_CUDA_BLEW_UP;
     return build_work(e);
+}
+
+int imogen_ensemble_x_lanes(ImogenEnsemble *E, int *cellsPerLane, int *split)
+{
+    if (!E || !cellsPerLane || !split) return IM_FAIL(ERROR_NULL_POINTER, "null argument");
+    EnsembleImpl *e = &E->impl;
+    for (int g = 0; g < kXClasses; g++) {
+        cellsPerLane[g] = 0;
+        split[g] = 0;
+        for (int p = 0; p < e->P; p++)
+            if (x_class_of(e->host[p].nx) == g) {
+                cellsPerLane[g] = kXClassK[e->xcls[p]];
+                split[g] = kXClassSplit[e->xcls[p]] ? 1 : 0;
+                break;
+            }
+    }
+    return SUCCESSFUL;
 }
 
 int imogen_ensemble_destroy(ImogenEnsemble *E)

@@ -112,6 +112,14 @@ class Ensemble:
         abi.call("imogen_ensemble_autotune", self.h, int(trials), chosen)
         return tuple(chosen)
 
+    def x_lanes(self):
+        """(cells per lane, split) of the X sweep per line-length group
+        (nx <= 64, 128, 192, 256, 368, 496, 512); (0, 0) for empty groups."""
+        k = (ctypes.c_int * 7)()
+        sp = (ctypes.c_int * 7)()
+        abi.call("imogen_ensemble_x_lanes", self.h, k, sp)
+        return tuple(zip(tuple(k), tuple(sp)))
+
     def free(self):
         if self.h:
             abi.call("imogen_ensemble_destroy", self.h)

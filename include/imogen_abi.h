@@ -452,9 +452,15 @@ int imogen_ensemble_run(ImogenEnsemble *e, int firstIteration, int iterations, i
 int imogen_ensemble_times(ImogenEnsemble *e, double *t);
 /* Times `trials` iterations per candidate and keeps, per size class (ny <= 128,
  * <= 256, <= 512), the fastest segment count S in {1, 2, 4} of the Y-sweep
- * march; chosen[3] (optional) receives them. The ensemble state advances
- * during tuning and the clocks are reset to 0 afterwards. */
+ * march (chosen[3], optional, receives them), then per line-length group of
+ * the X sweep (nx <= 64, <= 128, <= 192, <= 256, <= 368, <= 496, <= 512) the
+ * fastest cells per lane K among the least K that fits and up to two more.
+ * The ensemble state advances during tuning and the clocks are reset to 0
+ * afterwards. */
 int imogen_ensemble_autotune(ImogenEnsemble *e, int trials, int *chosen);
+/* The X sweep's cells per lane and split mode (1: lines swept as two halves)
+ * per line-length group as above (7 ints each; 0 for groups with no problem). */
+int imogen_ensemble_x_lanes(ImogenEnsemble *e, int *cellsPerLane, int *split);
 /* Gives the problems with hasDust[p] != 0 a second fluid, dust (HLLC with
  * gamma dustGamma, same grid and boundaries), coupled to the gas by the
  * two-fluid drag source of sourcefunction_2FluidDrag (method 4, 5, 6 or 7;

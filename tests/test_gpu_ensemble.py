@@ -76,13 +76,21 @@ def test_graph_replay_equals_eager_and_autotune_runs():
         assert np.array_equal(a.state(p), b.state(p))
     chosen = a.autotune(trials=1)
     assert all(c in (1, 2, 4) for c in chosen)
-    # after tuning (other segment counts) the ensemble still matches the single path
+    # cells per lane per nx group: the least that fits, or up to two more
+    lanes = a.x_lanes()
+    least = {0: 1, 1: 2, 2: 3, 3: 4, 4: 3, 5: 4, 6: 5}
+    for g, (k, sp) in enumerate(lanes):
+        assert k in range(least[g], least[g] + 3) and sp == (g >= 4), (g, k, sp)
+    # after tuning (other segment counts and cells per lane) every problem still
+    # matches the single path
     for p, s in enumerate(states):
         a.set_state(p, s)
     a.iteration = 0
     a.run(3, graph=True)
-    ref, _ = single_path(states[3], 3, SIZES[3])
-    assert np.abs(a.state(3) - ref).max() <= 1e-12 * np.abs(ref).max()
+    for p, n in enumerate(SIZES):
+        print(f"problem {p}: {n}^2 after tuning {chosen} {lanes}", flush=True)
+        ref, _ = single_path(states[p], 3, n)
+        assert np.abs(a.state(p) - ref).max() <= 1e-12 * np.abs(ref).max(), p
     a.free()
     b.free()
 
