@@ -190,13 +190,14 @@ def test_limiters_parity(limiter):
 
 @pytest.mark.parametrize("bc", ["const", "mirror", "circular"])
 @pytest.mark.parametrize("sh", [(20, 24, 28), (1, 36, 44), (1, 1, 64), (12, 11, 13), (3, 12, 128), (2, 3, 256),
-                                (3, 12, 32), (3, 12, 320), (2, 12, 451)])
+                                (3, 12, 32), (3, 12, 320), (2, 12, 451), (10, 12, 256)])
 def test_fluidstep_parity_with_boundaries(bc, sh):
     """(12, 11, 13): lines of 11-13 cells, where a mirror face reads cells the
     opposite face writes, so the faces must be set one after the other.
     (3, 12, 128), (2, 3, 256), (3, 12, 32): whole X lines of 2, 4 and 1 cells
     per lane (the last with two lines per wave) between swept y lines and
-    unswept z lines."""
+    unswept z lines. (10, 12, 256): 4-cell lanes over whole X lines with y
+    and z swept too."""
     a = abi()
     mode = {"const": a.extrapConstant, "mirror": a.mirror, "circular": a.circular}[bc]
     omode = {"const": po.BC_CONST, "mirror": po.BC_MIRROR, "circular": po.BC_CIRCULAR}[bc]

@@ -2,7 +2,8 @@
 iteration it replaces (imogen.Simulation: imogen_cflTimestepFused per fluid,
 dTime = CFL * min tau, performFluidUpdate_3D(+1), (-1) per fluid), which the
 other GPU tests pin to the oracle. Same kernels, same dTime arithmetic: the
-states and clocks must agree bitwise, eager or replayed from graphs. The loop
+states and clocks must agree bitwise, eager or replayed from graphs (X lines of
+256 cells: clocks bitwise, states to an ulp, see INSTANCE_ULP). The loop
 folds the CFL minimum into each iteration's last sweep where the boundaries
 allow it (const with the +y/+z innermost-ghost convention, mirror, circular;
 HLLC and HLL) and takes a full CFL pass otherwise (Xin-Jin) and on the first
@@ -34,6 +35,13 @@ def make_fluids(kind):
         q = np.concatenate([q] * 10, axis=3)
         fl = [gc.FluidManager(q, 5 / 3, minmass=1e-8, bc_modes=(abi.extrapConstant,) * 6)]
         return fl, [abi.METHOD_HLLC], gc.GeometryManager((320, 32, 32), h), abi.LIMITER_VANLEER
+    if kind == "wide256":
+        # X lines of 256 cells (4-cell lanes over whole lines), the last sweep
+        # of some iterations with the folded CFL
+        q, h, _ = problems.sedov3d(32)
+        q = np.concatenate([q] * 8, axis=3)
+        fl = [gc.FluidManager(q, 5 / 3, minmass=1e-8, bc_modes=(abi.extrapConstant,) * 6)]
+        return fl, [abi.METHOD_HLLC], gc.GeometryManager((256, 32, 32), h), abi.LIMITER_VANLEER
     if kind == "mirror2d":
         from gpuimogen_amd import ensemble as ens
         _, st = ens.quadrant_problems(1, seed=13, nmin=40, nmax=40)
@@ -50,7 +58,7 @@ def make_fluids(kind):
     raise ValueError(kind)
 
 
-@pytest.mark.parametrize("kind", ["sedov", "wide", "twofluid", "mirror2d", "xinjin2d"])
+@pytest.mark.parametrize("kind", ["sedov", "wide", "wide256", "twofluid", "mirror2d", "xinjin2d"])
 @pytest.mark.parametrize("graph", [False, True])
 def test_device_loop_matches_host_loop(kind, graph):
     from gpuimogen_amd import imogen
@@ -67,10 +75,24 @@ def test_device_loop_matches_host_loop(kind, graph):
     t, _ = loop.clock()
     assert t == sim.time
     for a, b in zip(fa, fb):
-        assert np.array_equal(a.state, b.state)
+        if kind in INSTANCE_ULP:
+            x, y = a.state, b.state
+            rel = [float(np.abs(x[v] - y[v]).sum() / np.abs(x[v]).sum()) if np.abs(x[v]).sum() else 0.0
+                   for v in range(5)]
+            print(f"{kind}: relative L1 difference per variable {rel}", flush=True)
+            assert max(rel) <= 1e-14, rel
+        else:
+            assert np.array_equal(a.state, b.state)
     loop.free()
     for f in fa + fb:
         f.free()
+
+
+# Lines of 256 cells are swept with 4 cells per lane over whole lines; the
+# folded-CFL form of that kernel (the device loop's last sweep) is a template
+# instance of its own whose FMA contraction differs from the plain one by an
+# ulp at some cells, so there the loops agree to rounding, not bitwise.
+INSTANCE_ULP = {"wide256"}
 
 
 def make_gasdust(n=16, seed=6, grain_mass=1.0, isothermal_dust=False):
