@@ -422,7 +422,9 @@ __global__ __launch_bounds__(256) void ens_sweep_y(const EnsProblem *__restrict_
 
 // constant-extrapolation ghost cells (cudaStatics.cu:1339-1366, 1440-1466 with
 // the +y innermost-ghost convention), depth d, both faces of one direction;
-// workgroups (entry, 4)
+// workgroups (entry, 16): on the 512-problem ensemble 16 per entry took the x
+// faces from 64 to 46 us per launch and the y faces from 21 to 16 (4 per
+// entry), 32 gained nothing more
 // xrules: the +y face copies the last interior cell, as the fluid step sets
 // its boundaries (flux.cu:99-107, the swept direction rotated into memory x);
 // 0: the y kernel's innermost-ghost copy (cudaStatics.cu:1461-1466), as
@@ -445,21 +447,21 @@ __global__ __launch_bounds__(256) void ens_bc_const(const EnsProblem *__restrict
     for (int pass = 0; pass < (together ? 1 : 2); pass++) {
         if (pass) __syncthreads();
         if (dir == 1) {
-            // a thread per (row, slab, side): the d ghosts of that row end
-            const int total = ny * 5 * nsides;
+            // a thread per (ghost, side, slab, row): the d ghosts of one row end
+            // are adjacent lanes, so a store instruction writes whole row-end
+            // segments (a thread per row end writing its d ghosts one by one:
+            // 104 against 64 us per launch, 4 workgroups per entry)
+            const int total = ny * 5 * nsides * d;
             for (int t = t0; t < total; t += tstride) {
-                const int side = together ? (t & 1) : pass;
-                const int r = together ? (t >> 1) : t;
+                const int g = t % d;
+                int r = t / d;
+                const int side = together ? (r & 1) : pass;
+                r = together ? (r >> 1) : r;
                 const int slab = r % 5;
                 const int row = r / 5;
                 double *a = P.q + slab * P.pitch + (int64_t)row * nx;
-                if (side == 0) {
-                    const double f = a[d];
-                    for (int g = 0; g < d; g++) a[g] = f;
-                } else {
-                    const double f = a[nx - d - 1];
-                    for (int g = 0; g < d; g++) a[nx - d + g] = f;
-                }
+                if (side == 0) a[g] = a[d];
+                else a[nx - d + g] = a[nx - d - 1];
             }
         } else {
             // a thread per (column, slab, side)
@@ -614,7 +616,7 @@ static int build_work(EnsembleImpl *e)
 
 static int ens_bc(EnsembleImpl *e, int dir, hipStream_t s, int xrules)
 {
-    hipLaunchKernelGGL(ens_bc_const, dim3((unsigned)e->NE, 4), dim3(256), 0, s, e->dprobs, dir, 4, xrules);
+    hipLaunchKernelGGL(ens_bc_const, dim3((unsigned)e->NE, 16), dim3(256), 0, s, e->dprobs, dir, 4, xrules);
     return IM_LAUNCHED("ens_bc_const");
 }
 
