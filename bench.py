@@ -244,7 +244,7 @@ def run_ensemble(args):
     # alone (the drag source would only add noise to the timings)
     load(False)
     t0 = time.perf_counter()
-    chosen = e.autotune(trials=2)
+    chosen = e.autotune(trials=4)
     xlanes = e.x_lanes()
     tune_s = time.perf_counter() - t0
     load(False)

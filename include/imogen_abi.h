@@ -454,9 +454,10 @@ int imogen_ensemble_times(ImogenEnsemble *e, double *t);
  * <= 256, <= 512), the fastest segment count S in {1, 2, 4} of the Y-sweep
  * march (chosen[3], optional, receives them), then per line-length group of
  * the X sweep (nx <= 64, <= 128, <= 192, <= 256, <= 368, <= 496, <= 512) the
- * fastest cells per lane K among the least K that fits and up to two more.
- * The ensemble state advances during tuning and the clocks are reset to 0
- * afterwards. */
+ * fastest cells per lane K among the least K that fits and up to two more. A
+ * candidate replaces the default (S = 1, 1, 2; the least K) only when it is 1 %
+ * faster. The ensemble state advances during tuning and the clocks are reset
+ * to 0 afterwards. */
 int imogen_ensemble_autotune(ImogenEnsemble *e, int trials, int *chosen);
 /* The X sweep's cells per lane and split mode (1: lines swept as two halves)
  * per line-length group as above (7 ints each; 0 for groups with no problem). */
